@@ -1,0 +1,137 @@
+/* vlhip.h -- device-resident C ABI of the MI355X GMM / Fisher / VLAD engine.
+ *
+ * The VLFeat-compatible entry points (vl/gmm.h, vl/fisher.h, vl/vlad.h,
+ * vl/kmeans.h) take host buffers, as the reference does.  The functions below
+ * are the layer those entry points call: every array argument is a DEVICE
+ * pointer on the current GPU unless its name ends in _host, sizes are element
+ * counts, and dtype is VL_TYPE_FLOAT (1) or VL_TYPE_DOUBLE (2).  Results are
+ * complete when a function returns (the library synchronises its stream before
+ * returning a host value, and otherwise leaves work queued in stream order;
+ * vlhip_synchronize waits for it).
+ *
+ * Each entry point cites the reference routine whose semantics it computes.
+ * None of them allocates host memory visible to the caller; device scratch is
+ * owned by the library and reused across calls.  Errors (bad arguments, HIP
+ * failures, no GPU) print a message through the VLFeat printf hook and abort(),
+ * like the reference's abort() on an unknown type (gmm.c:1514).
+ */
+#ifndef VLHIP_H
+#define VLHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+#  define VLHIP_API extern "C" __attribute__((visibility("default")))
+#else
+#  define VLHIP_API extern __attribute__((visibility("default")))
+#endif
+
+/* Precision / algorithm of the Fisher encoder (mode argument). */
+#define VLHIP_MODE_FP32     0  /* direct-form FP32 (FP64 for double data): the parity path */
+#define VLHIP_MODE_BF16X3   1  /* compensated split-BF16 MFMA contractions, FP32 accumulation */
+#define VLHIP_MODE_GATED    2  /* learned gate + exact top-M posteriors, dense fallback */
+
+/* ---- runtime ------------------------------------------------------------ */
+VLHIP_API int    vlhip_available (void) ;                 /* 1 when a GPU can be used (never aborts) */
+VLHIP_API void   vlhip_init (void) ;                      /* create the context on device 0 */
+VLHIP_API void   vlhip_set_stream (void * hipStream) ;    /* NULL = legacy null stream */
+VLHIP_API void * vlhip_malloc (size_t bytes) ;
+VLHIP_API void   vlhip_free (void * ptr) ;
+VLHIP_API void   vlhip_memcpy_h2d (void * dst, void const * src_host, size_t bytes) ;
+VLHIP_API void   vlhip_memcpy_d2h (void * dst_host, void const * src, size_t bytes) ;
+VLHIP_API void   vlhip_memset (void * dst, int value, size_t bytes) ;
+VLHIP_API void   vlhip_synchronize (void) ;
+VLHIP_API int    vlhip_num_compute_units (void) ;
+
+/* Library-owned, grow-only device buffers for staging host arrays.  A buffer
+ * keeps its contents until the next vlhip_buffer call for the same index with
+ * a larger size.  Callers that stage data and then run kernels hold the
+ * library lock across the sequence so concurrent host threads cannot reuse
+ * the same buffer in between. */
+#define VLHIP_BUF_DATA    0
+#define VLHIP_BUF_PARAMS  1
+#define VLHIP_BUF_POST    2
+#define VLHIP_BUF_ENC     3
+#define VLHIP_BUF_AUX     4
+#define VLHIP_BUF_AUX2    5
+VLHIP_API void * vlhip_buffer (int which, size_t bytes) ;
+VLHIP_API void   vlhip_lock (void) ;
+VLHIP_API void   vlhip_unlock (void) ;
+
+/* Per-kernel timing with hipEvents recorded on the library stream around each
+ * launch.  vlhip_timing_get returns the accumulated milliseconds and launch
+ * count of kernel `index` (0 <= index < vlhip_timing_count()). */
+VLHIP_API void   vlhip_timing_enable (int on) ;
+VLHIP_API void   vlhip_timing_reset (void) ;
+VLHIP_API int    vlhip_timing_count (void) ;
+VLHIP_API int    vlhip_timing_get (int index, char * name, size_t nameCapacity,
+                                   double * total_ms, unsigned long long * count) ;
+
+/* ---- GMM (vl/gmm.c) ----------------------------------------------------- */
+
+/* E-step, gmm.c:711-798.  posteriors (N x K, row-major, may be NULL when only
+ * the log-likelihood is wanted) receives the normalised posteriors; returns the
+ * log-likelihood summed over the data in FP64. */
+VLHIP_API double vlhip_gmm_posteriors (int dtype, void * posteriors, size_t numClusters, size_t numData,
+                                       void const * priors, void const * means, size_t dimension,
+                                       void const * covariances, void const * data) ;
+
+/* M-step sufficient statistics, gmm.c:1064-1167: for every component k with
+ * posterior p_ik >= threshold,  S0[k] = sum p,  S1[k,:] = sum p x,
+ * S2[k,:] = sum p (x - oldMeans[k,:])^2, and mass[k] = sum of all p_ik
+ * (gmm.c:836-844).  out_host is a HOST array of 2K + 2KD doubles laid out
+ * [S0 (K) | mass (K) | S1 (K x D) | S2 (K x D)], reduced in a fixed order. */
+VLHIP_API void vlhip_gmm_moments (int dtype, double * out_host,
+                                  void const * posteriors, void const * data, void const * oldMeans,
+                                  size_t numData, size_t dimension, size_t numClusters, double threshold) ;
+
+/* Empty-mode split, gmm.c:968-990: datum i moves its mass p_ij + p_ii to
+ * component iCl when data[i, dim] < mu, otherwise to jCl. */
+VLHIP_API void vlhip_gmm_split (int dtype, void * posteriors, void const * data, size_t numData,
+                                size_t dimension, size_t numClusters, size_t iCl, size_t jCl,
+                                size_t dim, double mu) ;
+
+/* One-hot posteriors from hard assignments, gmm.c:1361-1364. */
+VLHIP_API void vlhip_gmm_onehot (int dtype, void * posteriors, uint32_t const * assignments,
+                                 size_t numData, size_t numClusters) ;
+
+/* ---- Fisher (vl/fisher.c) ------------------------------------------------ */
+
+/* Fisher vectors of numImages images, fisher.c:363-488 per image.  Image i
+ * owns descriptor rows [offsets[i], offsets[i+1]) of data; enc receives
+ * numImages x 2*D*K values.  numTerms (device, may be NULL) receives one
+ * count per image.  mode is a VLHIP_MODE_* value. */
+VLHIP_API void vlhip_fisher_encode_batch (int dtype, void * enc,
+                                          void const * means, void const * covariances, void const * priors,
+                                          size_t dimension, size_t numClusters,
+                                          void const * data, uint64_t const * offsets, size_t numImages,
+                                          int flags, int mode, unsigned long long * numTerms) ;
+
+/* ---- VLAD (vl/vlad.c) ---------------------------------------------------- */
+
+/* VLAD of numImages images, vlad.c:163-242 per image.  Exactly one of
+ * assignments (dense N x K soft assignments, dtype) and hardAssignments (N
+ * cluster indices) is non-NULL. */
+VLHIP_API void vlhip_vlad_encode_batch (int dtype, void * enc, void const * means,
+                                        size_t dimension, size_t numClusters,
+                                        void const * data, uint64_t const * offsets, size_t numImages,
+                                        void const * assignments, uint32_t const * hardAssignments,
+                                        int flags) ;
+
+/* ---- k-means (vl/kmeans.c) ----------------------------------------------- */
+
+/* Exact L2 quantizer, kmeans.c:655-702: argmin over centers with strict '<'
+ * (first index wins), distances evaluated in the host's vector-lane order
+ * (lanes = 8 for the AVX path, 4/2 for SSE2, 1 scalar; mathop.c:472-550) so the
+ * assignments match the reference bit for bit.  distances may be NULL. */
+VLHIP_API void vlhip_kmeans_quantize (int dtype, uint32_t * assignments, void * distances,
+                                      void const * data, void const * centers,
+                                      size_t numData, size_t dimension, size_t numCenters, int lanes) ;
+
+/* Squared L2 distance of every datum to one point, same lane order
+ * (used by k-means++ seeding, kmeans.c:610-616). */
+VLHIP_API void vlhip_l2_to_point (int dtype, void * distances, void const * data, void const * point,
+                                  size_t numData, size_t dimension, int lanes) ;
+
+#endif /* VLHIP_H */

@@ -1,0 +1,235 @@
+// kmeans_kernels.hip -- exact L2 quantizer (vl/kmeans.c:655-702) and the
+// point-to-data distances used by k-means++ seeding (kmeans.c:610-616).
+//
+// Bit-exact argmin parity: the reference evaluates each squared L2 distance
+// with the host's SIMD kernel -- LANES partial sums over d = l (mod LANES) for
+// the first floor(D/LANES)*LANES coordinates, a fixed horizontal-add tree, then
+// the remaining coordinates one by one (mathop_avx.c:43-111,
+// mathop_sse2.c:43-100, mathop.c:296-307) -- with separate multiply and add
+// (no fused multiply-add).  The kernels below reproduce that order with FP
+// contraction disabled, so every distance is bitwise the reference's and the
+// strict '<' / first-index rule (kmeans.c:690-695) yields the same assignment.
+#include "vlhip_runtime.h"
+#include "vlhip_device.h"
+#include "vlhip_kernels.h"
+#include "vlhip.h"
+
+#include <algorithm>
+
+namespace vlhip {
+
+// Horizontal sum of the lane accumulators in the order of the host kernel.
+//  8 lanes (AVX float):   ((a0+a1)+(a4+a5)) + ((a2+a3)+(a6+a7))
+//  4 lanes (AVX double, SSE2 float): (a0+a2) + (a1+a3)
+//  2 lanes (SSE2 double): a0 + a1
+template <typename T, int LANES>
+__device__ __forceinline__ T lane_hsum(const T (&a)[LANES]) {
+#pragma clang fp contract(off)
+  if constexpr (LANES == 8) {
+    return ((a[0] + a[1]) + (a[4] + a[5])) + ((a[2] + a[3]) + (a[6] + a[7]));
+  } else if constexpr (LANES == 4) {
+    return (a[0] + a[2]) + (a[1] + a[3]);
+  } else if constexpr (LANES == 2) {
+    return a[0] + a[1];
+  } else {
+    return a[0];
+  }
+}
+
+constexpr int QZ_ROWS = 32;
+constexpr int QZ_COLS = 32;
+constexpr int QZ_DC = 16;  // multiple of every lane count
+
+// Thread (ty, tx) of a 16x16 grid owns rows ty*2.., centers tx*2...
+template <typename T, int LANES>
+__global__ __launch_bounds__(256) void quantize_kernel(const T* __restrict__ x, const T* __restrict__ centers,
+                                                       long long N, int D, int K, uint32_t* __restrict__ assign,
+                                                       T* __restrict__ dist) {
+#pragma clang fp contract(off)
+  __shared__ T xs[QZ_DC][QZ_ROWS + 1];
+  __shared__ T cs[QZ_DC][QZ_COLS + 1];
+  __shared__ T red_v[QZ_ROWS][16];
+  __shared__ int red_i[QZ_ROWS][16];
+
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const long long row0 = (long long)blockIdx.x * QZ_ROWS;
+  const int Dv = (LANES > 1) ? (D / LANES) * LANES : 0;  // vectorised prefix
+
+  T best[2];
+  int bidx[2];
+  best[0] = best[1] = (T)INFINITY;
+  bidx[0] = bidx[1] = 0;
+
+  for (int k0 = 0; k0 < K; k0 += QZ_COLS) {
+    T acc[2][2][LANES];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int l = 0; l < LANES; ++l) acc[i][j][l] = T(0);
+    for (int d0 = 0; d0 < Dv; d0 += QZ_DC) {
+      const int dc = min(QZ_DC, Dv - d0);
+      __syncthreads();
+      for (int idx = tid; idx < QZ_ROWS * QZ_DC; idx += 256) {
+        const int r = idx / QZ_DC, dd = idx % QZ_DC;
+        const long long row = row0 + r;
+        xs[dd][r] = (dd < dc && row < N) ? x[row * D + d0 + dd] : T(0);
+      }
+      for (int idx = tid; idx < QZ_COLS * QZ_DC; idx += 256) {
+        const int c = idx / QZ_DC, dd = idx % QZ_DC;
+        const int k = k0 + c;
+        cs[dd][c] = (dd < dc && k < K) ? centers[(size_t)k * D + d0 + dd] : T(0);
+      }
+      __syncthreads();
+      for (int dd = 0; dd < dc; dd += LANES) {
+#pragma unroll
+        for (int l = 0; l < LANES; ++l) {
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+              const T delta = xs[dd + l][ty * 2 + i] - cs[dd + l][tx * 2 + j];
+              const T d2 = delta * delta;
+              acc[i][j][l] = acc[i][j][l] + d2;
+            }
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const long long row = row0 + ty * 2 + i;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int k = k0 + tx * 2 + j;
+        T s = (LANES > 1) ? lane_hsum<T, LANES>(acc[i][j]) : T(0);
+        if (row < N && k < K) {
+          for (int d = Dv; d < D; ++d) {
+            const T delta = x[row * D + d] - centers[(size_t)k * D + d];
+            const T d2 = delta * delta;
+            s = s + d2;
+          }
+          if (s < best[i]) { best[i] = s; bidx[i] = k; }
+        }
+      }
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    red_v[ty * 2 + i][tx] = best[i];
+    red_i[ty * 2 + i][tx] = bidx[i];
+  }
+  __syncthreads();
+  if (tid < QZ_ROWS) {
+    const long long row = row0 + tid;
+    if (row < N) {
+      // thread tx visited centers in increasing order within itself; merge
+      // with the smallest distance, ties to the smallest index.
+      T bv = red_v[tid][0];
+      int bi = red_i[tid][0];
+      for (int t = 1; t < 16; ++t) {
+        const T v = red_v[tid][t];
+        const int a = red_i[tid][t];
+        if (v < bv || (v == bv && a < bi)) { bv = v; bi = a; }
+      }
+      assign[row] = (uint32_t)bi;
+      if (dist) dist[row] = bv;
+    }
+  }
+}
+
+// Squared L2 distance of each datum to `point`, host lane order.
+template <typename T, int LANES>
+__global__ __launch_bounds__(256) void l2_to_point_kernel(const T* __restrict__ x, const T* __restrict__ point,
+                                                          long long N, int D, T* __restrict__ out) {
+#pragma clang fp contract(off)
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const T* xi = x + i * D;
+  const int Dv = (LANES > 1) ? (D / LANES) * LANES : 0;
+  T acc[LANES];
+#pragma unroll
+  for (int l = 0; l < LANES; ++l) acc[l] = T(0);
+  for (int d = 0; d < Dv; d += LANES) {
+#pragma unroll
+    for (int l = 0; l < LANES; ++l) {
+      const T delta = point[d + l] - xi[d + l];
+      const T d2 = delta * delta;
+      acc[l] = acc[l] + d2;
+    }
+  }
+  T s = (LANES > 1) ? lane_hsum<T, LANES>(acc) : T(0);
+  for (int d = Dv; d < D; ++d) {
+    const T delta = point[d] - xi[d];
+    const T d2 = delta * delta;
+    s = s + d2;
+  }
+  out[i] = s;
+}
+
+template <typename T>
+void quantize_t(uint32_t* assign, T* dist, const T* x, const T* centers, size_t N, size_t D, size_t K, int lanes) {
+  VLHIP_REQUIRE(K >= 1 && D >= 1, "numCenters and dimension must be positive");
+  if (N == 0) return;
+  const unsigned blocks = (unsigned)((N + QZ_ROWS - 1) / QZ_ROWS);
+  switch (lanes) {
+    case 8: VLHIP_LAUNCH("kmeans_quantize", (quantize_kernel<T, 8>), dim3(blocks), dim3(256), 0, x, centers,
+                         (long long)N, (int)D, (int)K, assign, dist); break;
+    case 4: VLHIP_LAUNCH("kmeans_quantize", (quantize_kernel<T, 4>), dim3(blocks), dim3(256), 0, x, centers,
+                         (long long)N, (int)D, (int)K, assign, dist); break;
+    case 2: VLHIP_LAUNCH("kmeans_quantize", (quantize_kernel<T, 2>), dim3(blocks), dim3(256), 0, x, centers,
+                         (long long)N, (int)D, (int)K, assign, dist); break;
+    case 1: VLHIP_LAUNCH("kmeans_quantize", (quantize_kernel<T, 1>), dim3(blocks), dim3(256), 0, x, centers,
+                         (long long)N, (int)D, (int)K, assign, dist); break;
+    default: fatal("vlhip_kmeans_quantize", __FILE__, __LINE__, "lanes must be 1, 2, 4 or 8");
+  }
+}
+
+template <typename T>
+void l2_to_point_t(T* out, const T* x, const T* point, size_t N, size_t D, int lanes) {
+  if (N == 0) return;
+  const unsigned blocks = (unsigned)((N + 255) / 256);
+  switch (lanes) {
+    case 8: VLHIP_LAUNCH("l2_to_point", (l2_to_point_kernel<T, 8>), dim3(blocks), dim3(256), 0, x, point,
+                         (long long)N, (int)D, out); break;
+    case 4: VLHIP_LAUNCH("l2_to_point", (l2_to_point_kernel<T, 4>), dim3(blocks), dim3(256), 0, x, point,
+                         (long long)N, (int)D, out); break;
+    case 2: VLHIP_LAUNCH("l2_to_point", (l2_to_point_kernel<T, 2>), dim3(blocks), dim3(256), 0, x, point,
+                         (long long)N, (int)D, out); break;
+    case 1: VLHIP_LAUNCH("l2_to_point", (l2_to_point_kernel<T, 1>), dim3(blocks), dim3(256), 0, x, point,
+                         (long long)N, (int)D, out); break;
+    default: fatal("vlhip_l2_to_point", __FILE__, __LINE__, "lanes must be 1, 2, 4 or 8");
+  }
+}
+
+}  // namespace vlhip
+
+using namespace vlhip;
+
+extern "C" {
+
+void vlhip_kmeans_quantize(int dtype, uint32_t* assign, void* dist, void const* data, void const* centers,
+                           size_t N, size_t D, size_t K, int lanes) {
+  ensure_init();
+  std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+  if (dtype == 1)
+    quantize_t<float>(assign, (float*)dist, (const float*)data, (const float*)centers, N, D, K, lanes);
+  else if (dtype == 2)
+    quantize_t<double>(assign, (double*)dist, (const double*)data, (const double*)centers, N, D, K, lanes);
+  else
+    fatal("vlhip_kmeans_quantize", __FILE__, __LINE__, "unknown data type");
+}
+
+void vlhip_l2_to_point(int dtype, void* out, void const* data, void const* point, size_t N, size_t D, int lanes) {
+  ensure_init();
+  std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+  if (dtype == 1)
+    l2_to_point_t<float>((float*)out, (const float*)data, (const float*)point, N, D, lanes);
+  else if (dtype == 2)
+    l2_to_point_t<double>((double*)out, (const double*)data, (const double*)point, N, D, lanes);
+  else
+    fatal("vlhip_l2_to_point", __FILE__, __LINE__, "unknown data type");
+}
+
+}  // extern "C"

@@ -1,0 +1,217 @@
+// vlad_kernels.hip -- VLAD encoder (vl/vlad.c:163-242).
+//
+// Workgroup (image, block of VL_COMPS clusters) accumulates, for each of its
+// clusters k and every dimension d, enc[k,d] = sum_i q_ik x_id over the image's
+// descriptors in index order, with the reference's arithmetic: the product and
+// sum are formed in double and the running sum is rounded back to the data
+// type after every term (vlad.c:181-188, where q is double and enc is TYPE),
+// and only q > 0 contributes.  The per-cluster corrections, signed square root
+// and per-cluster L2 normalisation follow vlad.c:191-227 in the same order and
+// precision; the global normalisation (vlad.c:230-241) sums the squared
+// entries sequentially in index order, so for float data the result matches
+// the reference's float path element for element.
+#include "vlhip_runtime.h"
+#include "vlhip_device.h"
+#include "vlhip_kernels.h"
+#include "vlhip.h"
+
+#include <algorithm>
+
+namespace vlhip {
+
+constexpr int VL_COMPS = 16;
+constexpr int VL_ROWS = 64;
+
+template <typename T, int DPT, bool HARD>
+__global__ __launch_bounds__(256) void vlad_accum_kernel(
+    const T* __restrict__ x, const uint64_t* __restrict__ offsets, int numImages, int D, int K,
+    const T* __restrict__ means, const T* __restrict__ assign, const uint32_t* __restrict__ hard,
+    int flags, T* __restrict__ enc, int numKB) {
+#pragma clang fp contract(off)
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* xs = reinterpret_cast<T*>(smem_raw);           // [VL_ROWS][D + 1]
+  T* qs = xs + (size_t)VL_ROWS * (D + 1);            // [VL_ROWS][VL_COMPS]
+  T* zs = qs + VL_ROWS * VL_COMPS;                   // [VL_COMPS][D] final values
+  __shared__ double massSh[VL_COMPS];
+
+  const int b = blockIdx.x;
+  int img, kb;
+  if (numImages >= 8) {
+    const int xcd = b & 7, slot = b >> 3;
+    img = (slot / numKB) * 8 + xcd;
+    kb = slot % numKB;
+  } else {
+    img = b / numKB;
+    kb = b % numKB;
+  }
+  if (img >= numImages) return;
+
+  const int tid = threadIdx.x;
+  const int c = tid >> 4, dg = tid & 15;
+  const int k = kb * VL_COMPS + c;
+  const long long rBegin = (long long)offsets[img], rEnd = (long long)offsets[img + 1];
+
+  T acc[DPT];
+#pragma unroll
+  for (int j = 0; j < DPT; ++j) acc[j] = T(0);
+  double mass = 0.0;
+
+  for (long long r0 = rBegin; r0 < rEnd; r0 += VL_ROWS) {
+    const int nr = (int)min((long long)VL_ROWS, rEnd - r0);
+    __syncthreads();
+    for (int idx = tid; idx < nr * D; idx += 256) {
+      const int r = idx / D, d = idx - r * D;
+      xs[r * (D + 1) + d] = x[(r0 + r) * D + d];
+    }
+    for (int idx = tid; idx < VL_ROWS * VL_COMPS; idx += 256) {
+      const int r = idx / VL_COMPS, cc = idx - r * VL_COMPS;
+      const int kk = kb * VL_COMPS + cc;
+      T qv = T(0);
+      if (r < nr && kk < K) {
+        if (HARD) qv = (hard[r0 + r] == (uint32_t)kk) ? T(1) : T(0);
+        else qv = assign[(r0 + r) * K + kk];
+      }
+      qs[idx] = qv;
+    }
+    __syncthreads();
+    if (k < K) {
+      for (int r = 0; r < nr; ++r) {
+        const T qv = qs[r * VL_COMPS + c];
+        if (qv > T(0)) {
+          const double qd = (double)qv;
+          mass += qd;
+#pragma unroll
+          for (int j = 0; j < DPT; ++j) {
+            const int d = dg + 16 * j;
+            if (d < D) acc[j] = (T)((double)acc[j] + qd * (double)xs[r * (D + 1) + d]);
+          }
+        }
+      }
+    }
+  }
+
+  // vlad.c:191-213: mean subtraction and signed square root
+  if (k < K) {
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) {
+      const int d = dg + 16 * j;
+      if (d >= D) continue;
+      T e = acc[j];
+      const T mu = means[(size_t)k * D + d];
+      if (mass > 0) {
+        if (flags & 8) {
+          e = (T)((double)e / mass);
+          e = e - mu;
+        } else {
+          e = (T)((double)e - mass * (double)mu);
+        }
+      }
+      if (flags & 2) e = signed_sqrt(e);
+      zs[c * D + d] = e;
+    }
+  }
+  __syncthreads();
+  // vlad.c:215-227: per-cluster L2 normalisation, sequential sum over d
+  if ((flags & 1) && dg == 0 && k < K) {
+    T n = T(0);
+    for (int d = 0; d < D; ++d) {
+      const T z = zs[c * D + d];
+      n = n + z * z;
+    }
+    n = dev_sqrt<T>(n);
+    massSh[c] = ((double)n > 1e-12) ? (double)n : 1e-12;
+  }
+  __syncthreads();
+  if (k < K) {
+    T* e = enc + (size_t)img * K * D + (size_t)k * D;
+    const T n = (flags & 1) ? (T)massSh[c] : T(1);
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) {
+      const int d = dg + 16 * j;
+      if (d >= D) continue;
+      const T z = zs[c * D + d];
+      e[d] = (flags & 1) ? z / n : z;
+    }
+  }
+}
+
+// vlad.c:230-241: enc /= max(||enc||, 1e-12), the squared norm summed in
+// index order by one lane (bitwise the reference's sequential loop).
+template <typename T>
+__global__ __launch_bounds__(256) void vlad_normalize_kernel(T* __restrict__ enc, size_t len) {
+#pragma clang fp contract(off)
+  __shared__ T nsh;
+  T* e = enc + (size_t)blockIdx.x * len;
+  if (threadIdx.x == 0) {
+    T n = T(0);
+    for (size_t i = 0; i < len; ++i) {
+      const T z = e[i];
+      n = n + z * z;
+    }
+    n = dev_sqrt<T>(n);
+    nsh = (T)(((double)n > 1e-12) ? (double)n : 1e-12);
+  }
+  __syncthreads();
+  const T n = nsh;
+  for (size_t i = threadIdx.x; i < len; i += blockDim.x) e[i] = e[i] / n;
+}
+
+template <typename T, int DPT>
+static void launch_vlad(T* enc, const T* means, int D, int K, const T* x, const uint64_t* offsets, int numImages,
+                        const T* assign, const uint32_t* hard, int flags) {
+  const int numKB = (K + VL_COMPS - 1) / VL_COMPS;
+  const size_t shmem = ((size_t)VL_ROWS * (D + 1) + VL_ROWS * VL_COMPS + (size_t)VL_COMPS * D) * sizeof(T);
+  const int padImages = numImages >= 8 ? ((numImages + 7) / 8) * 8 : numImages;
+  const unsigned blocks = (unsigned)((size_t)padImages * numKB);
+  if (hard) {
+    auto kern = vlad_accum_kernel<T, DPT, true>;
+    VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+    VLHIP_LAUNCH("vlad_accum", kern, dim3(blocks), dim3(256), shmem, x, offsets, numImages, D, K, means, assign,
+                 hard, flags, enc, numKB);
+  } else {
+    auto kern = vlad_accum_kernel<T, DPT, false>;
+    VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+    VLHIP_LAUNCH("vlad_accum", kern, dim3(blocks), dim3(256), shmem, x, offsets, numImages, D, K, means, assign,
+                 hard, flags, enc, numKB);
+  }
+  if (!(flags & 4)) {
+    VLHIP_LAUNCH("vlad_normalize", (vlad_normalize_kernel<T>), dim3((unsigned)numImages), dim3(256), 0, enc,
+                 (size_t)K * D);
+  }
+}
+
+template <typename T>
+void vlad_t(T* enc, const T* means, size_t D, size_t K, const T* x, const uint64_t* offsets, size_t numImages,
+            const T* assign, const uint32_t* hard, int flags) {
+  VLHIP_REQUIRE(K >= 1 && D >= 1, "numClusters and dimension must be positive");
+  VLHIP_REQUIRE((assign != nullptr) != (hard != nullptr), "exactly one of assignments / hardAssignments");
+  VLHIP_REQUIRE(D <= 512, "GPU VLAD encoder supports dimension <= 512");
+  if (numImages == 0) return;
+  const int d = (int)D, k = (int)K, n = (int)numImages;
+  if (D <= 16) launch_vlad<T, 1>(enc, means, d, k, x, offsets, n, assign, hard, flags);
+  else if (D <= 32) launch_vlad<T, 2>(enc, means, d, k, x, offsets, n, assign, hard, flags);
+  else if (D <= 64) launch_vlad<T, 4>(enc, means, d, k, x, offsets, n, assign, hard, flags);
+  else if (D <= 80) launch_vlad<T, 5>(enc, means, d, k, x, offsets, n, assign, hard, flags);
+  else if (D <= 128) launch_vlad<T, 8>(enc, means, d, k, x, offsets, n, assign, hard, flags);
+  else if (D <= 256) launch_vlad<T, 16>(enc, means, d, k, x, offsets, n, assign, hard, flags);
+  else launch_vlad<T, 32>(enc, means, d, k, x, offsets, n, assign, hard, flags);
+}
+
+}  // namespace vlhip
+
+using namespace vlhip;
+
+extern "C" void vlhip_vlad_encode_batch(int dtype, void* enc, void const* means, size_t D, size_t K,
+                                        void const* data, uint64_t const* offsets, size_t numImages,
+                                        void const* assignments, uint32_t const* hard, int flags) {
+  ensure_init();
+  std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+  if (dtype == 1)
+    vlad_t<float>((float*)enc, (const float*)means, D, K, (const float*)data, offsets, numImages,
+                  (const float*)assignments, hard, flags);
+  else if (dtype == 2)
+    vlad_t<double>((double*)enc, (const double*)means, D, K, (const double*)data, offsets, numImages,
+                   (const double*)assignments, hard, flags);
+  else
+    fatal("vlhip_vlad_encode_batch", __FILE__, __LINE__, "unknown data type");
+}

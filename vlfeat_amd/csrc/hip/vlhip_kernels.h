@@ -1,0 +1,36 @@
+// vlhip_kernels.h -- tile constants and launchers shared between the GMM,
+// Fisher, VLAD and k-means translation units.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+namespace vlhip {
+
+// E-step register tiling: 64 descriptors x 64 components per sweep step,
+// 16x16 threads each owning a 4x4 tile, D staged through LDS in chunks.
+constexpr int EM_ROWS = 64;
+constexpr int EM_COLS = 64;
+constexpr int EM_DC = 16;
+
+// M-step moments: 32 components x 64-row chunks per workgroup step.
+constexpr int MS_COMPS = 32;
+constexpr int MS_ROWS = 64;
+
+template <typename T>
+struct GmmPrep {
+  T* inv;         // K x D, 1 / sigma^2
+  T* isig;        // K x D, sqrt(1 / sigma^2)
+  double* cterm;  // K, log prior - D/2 log 2pi - 1/2 log det
+};
+
+template <typename T>
+GmmPrep<T> gmm_prepare(const T* priors, const T* covs, size_t D, size_t K);
+
+// Runs the E-step over N rows; q (N x K) optional, row statistics optional.
+// Returns the FP64 log-likelihood (sum over rows).
+template <typename T>
+double run_estep(const T* x, size_t N, size_t D, size_t K, const T* means, const GmmPrep<T>& prep,
+                 T* q, T* row_m, T* row_s, int* row_arg, const char* name, bool want_ll);
+
+}  // namespace vlhip

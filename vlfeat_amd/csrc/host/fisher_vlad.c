@@ -1,0 +1,150 @@
+/* fisher_vlad.c -- host entry points of the Fisher and VLAD encoders.
+ *
+ * vl_fisher_encode  replaces vl/fisher.c:548-579 (algorithm 363-488)
+ * vl_vlad_encode    replaces vl/vlad.c:292-315   (algorithm 163-242)
+ *
+ * Both stage the caller's host arrays into library-owned device buffers, run
+ * the encoder kernels (vlhip_fisher_encode_batch / vlhip_vlad_encode_batch)
+ * and copy the encoding back into the caller's buffer.  An unknown data type
+ * aborts, as in the reference.
+ */
+#include "vl/fisher.h"
+#include "vl/vlad.h"
+#include "vlhip.h"
+
+#include <assert.h>
+#include <stdio.h>
+#include <string.h>
+
+static void
+check_type (vl_type t, char const * who)
+{
+  if (t != VL_TYPE_FLOAT && t != VL_TYPE_DOUBLE) {
+    VL_PRINTF("%s: unsupported data type %u\n", who, (unsigned)t) ;
+    abort() ;
+  }
+}
+
+static vl_size
+fisher_run (void * enc, vl_type dataType, void const * means, vl_size dimension, vl_size numClusters,
+            void const * covariances, void const * priors, void const * data,
+            vl_uint64 const * offsets, vl_size numImages, int flags, vl_size * numTermsOut)
+{
+  vl_size es = vl_get_type_size(dataType) ;
+  vl_size KD = dimension * numClusters ;
+  vl_size numData = offsets[numImages] ;
+  vl_size total = 0 ;
+  unsigned char * params ;
+  void * dData, * dEnc ;
+  vl_uint64 * dOff ;
+  unsigned long long * dTerms ;
+  vl_uindex i ;
+
+  vlhip_lock() ;
+  params = (unsigned char*) vlhip_buffer(VLHIP_BUF_PARAMS, es * (2 * KD + numClusters)) ;
+  dData = vlhip_buffer(VLHIP_BUF_DATA, es * dimension * (numData ? numData : 1)) ;
+  dEnc = vlhip_buffer(VLHIP_BUF_ENC, es * 2 * KD * numImages) ;
+  dOff = (vl_uint64*) vlhip_buffer(VLHIP_BUF_AUX, sizeof(vl_uint64) * (numImages + 1)) ;
+  dTerms = (unsigned long long*) vlhip_buffer(VLHIP_BUF_AUX2, sizeof(unsigned long long) * numImages) ;
+  vlhip_memcpy_h2d(params, means, es * KD) ;
+  vlhip_memcpy_h2d(params + es * KD, covariances, es * KD) ;
+  vlhip_memcpy_h2d(params + 2 * es * KD, priors, es * numClusters) ;
+  if (numData) vlhip_memcpy_h2d(dData, data, es * dimension * numData) ;
+  vlhip_memcpy_h2d(dOff, offsets, sizeof(vl_uint64) * (numImages + 1)) ;
+  vlhip_fisher_encode_batch((int)dataType, dEnc, params, params + es * KD, params + 2 * es * KD,
+                            dimension, numClusters, dData, (uint64_t const*)dOff, numImages, flags,
+                            VLHIP_MODE_FP32, dTerms) ;
+  vlhip_memcpy_d2h(enc, dEnc, es * 2 * KD * numImages) ;
+  if (numTermsOut) {
+    unsigned long long * h = (unsigned long long*) malloc(sizeof(unsigned long long) * numImages) ;
+    vlhip_memcpy_d2h(h, dTerms, sizeof(unsigned long long) * numImages) ;
+    for (i = 0 ; i < numImages ; ++i) { numTermsOut[i] = h[i] ; total += h[i] ; }
+    free(h) ;
+  }
+  vlhip_unlock() ;
+  return total ;
+}
+
+VL_EXPORT vl_size
+vl_fisher_encode (void * enc, vl_type dataType,
+                  void const * means, vl_size dimension, vl_size numClusters,
+                  void const * covariances, void const * priors,
+                  void const * data, vl_size numData, int flags)
+{
+  vl_uint64 offsets [2] ;
+  vl_size terms = 0 ;
+  check_type(dataType, "vl_fisher_encode") ;
+  assert(numClusters >= 1) ;
+  assert(dimension >= 1) ;
+  offsets[0] = 0 ;
+  offsets[1] = numData ;
+  fisher_run(enc, dataType, means, dimension, numClusters, covariances, priors, data,
+             offsets, 1, flags, &terms) ;
+  return terms ;
+}
+
+VL_EXPORT void
+vl_fisher_encode_batch (void * enc, vl_type dataType,
+                        void const * means, vl_size dimension, vl_size numClusters,
+                        void const * covariances, void const * priors,
+                        void const * data, vl_uint64 const * offsets, vl_size numImages,
+                        int flags, vl_size * numTerms)
+{
+  check_type(dataType, "vl_fisher_encode_batch") ;
+  assert(numClusters >= 1) ;
+  assert(dimension >= 1) ;
+  if (numImages == 0) return ;
+  fisher_run(enc, dataType, means, dimension, numClusters, covariances, priors, data,
+             offsets, numImages, flags, numTerms) ;
+}
+
+static void
+vlad_run (void * enc, vl_type dataType, void const * means, vl_size dimension, vl_size numClusters,
+          void const * data, vl_size numData, void const * soft, vl_uint32 const * hard, int flags)
+{
+  vl_size es = vl_get_type_size(dataType) ;
+  vl_size KD = dimension * numClusters ;
+  vl_uint64 offsets [2] ;
+  void * dMeans, * dData, * dEnc, * dAssign ;
+  vl_uint64 * dOff ;
+
+  offsets[0] = 0 ;
+  offsets[1] = numData ;
+  vlhip_lock() ;
+  dMeans = vlhip_buffer(VLHIP_BUF_PARAMS, es * KD) ;
+  dData = vlhip_buffer(VLHIP_BUF_DATA, es * dimension * (numData ? numData : 1)) ;
+  dEnc = vlhip_buffer(VLHIP_BUF_ENC, es * KD) ;
+  dOff = (vl_uint64*) vlhip_buffer(VLHIP_BUF_AUX, sizeof(offsets)) ;
+  dAssign = vlhip_buffer(VLHIP_BUF_POST, (soft ? es * numClusters : sizeof(vl_uint32)) * (numData ? numData : 1)) ;
+  vlhip_memcpy_h2d(dMeans, means, es * KD) ;
+  if (numData) {
+    vlhip_memcpy_h2d(dData, data, es * dimension * numData) ;
+    if (soft) vlhip_memcpy_h2d(dAssign, soft, es * numClusters * numData) ;
+    else vlhip_memcpy_h2d(dAssign, hard, sizeof(vl_uint32) * numData) ;
+  }
+  vlhip_memcpy_h2d(dOff, offsets, sizeof(offsets)) ;
+  vlhip_vlad_encode_batch((int)dataType, dEnc, dMeans, dimension, numClusters, dData, (uint64_t const*)dOff, 1,
+                          soft ? dAssign : NULL, soft ? NULL : (uint32_t const*)dAssign, flags) ;
+  vlhip_memcpy_d2h(enc, dEnc, es * KD) ;
+  vlhip_unlock() ;
+}
+
+VL_EXPORT void
+vl_vlad_encode (void * enc, vl_type dataType,
+                void const * means, vl_size dimension, vl_size numClusters,
+                void const * data, vl_size numData,
+                void const * assignments, int flags)
+{
+  check_type(dataType, "vl_vlad_encode") ;
+  vlad_run(enc, dataType, means, dimension, numClusters, data, numData, assignments, NULL, flags) ;
+}
+
+VL_EXPORT void
+vl_vlad_encode_hard (void * enc, vl_type dataType,
+                     void const * means, vl_size dimension, vl_size numClusters,
+                     void const * data, vl_size numData,
+                     vl_uint32 const * assignment, int flags)
+{
+  check_type(dataType, "vl_vlad_encode_hard") ;
+  vlad_run(enc, dataType, means, dimension, numClusters, data, numData, NULL, assignment, flags) ;
+}

@@ -1,0 +1,299 @@
+/* gmm_impl.h -- type-generic part of gmm.c, included once with
+ * GMM_TYPE = float / GMM_SFX = f and once with double / d. */
+
+#define FN(name) VL_XCAT(name##_, GMM_SFX)
+#define TYPE GMM_TYPE
+#define TYPE_CODE ((sizeof(TYPE) == sizeof(float)) ? VL_TYPE_FLOAT : VL_TYPE_DOUBLE)
+
+/* gmm.c:1005-1030 */
+static void
+FN(gmm_apply_bounds) (VlGMM * self)
+{
+  vl_uindex k, d ;
+  vl_size numAdjusted = 0 ;
+  TYPE * cov = (TYPE*) self->covariances ;
+  double const * lbs = self->sigmaLowBound ;
+  for (k = 0 ; k < self->numClusters ; ++k) {
+    vl_bool adjusted = VL_FALSE ;
+    for (d = 0 ; d < self->dimension ; ++d) {
+      if (cov[k * self->dimension + d] < lbs[d]) {
+        cov[k * self->dimension + d] = (TYPE) lbs[d] ;
+        adjusted = VL_TRUE ;
+      }
+    }
+    if (adjusted) numAdjusted ++ ;
+  }
+  if (numAdjusted > 0 && self->verbosity > 0) {
+    VL_PRINT("gmm: detected %d of %d modes with at least one dimension "
+             "with covariance too small (set to lower bound)\n",
+             (int)numAdjusted, (int)self->numClusters) ;
+  }
+}
+
+/* Moments of the current device posteriors against the current means. */
+static void
+FN(gmm_moments) (VlGMM * self, double * out, vl_size numData)
+{
+  vl_size KD = self->numClusters * self->dimension ;
+  unsigned char * p = (unsigned char*) self->dParams ;
+  TYPE * dOld = (TYPE*)(p + sizeof(TYPE) * (2 * KD + self->numClusters)) ;
+  vlhip_memcpy_h2d(dOld, self->means, sizeof(TYPE) * KD) ;
+  vlhip_gmm_moments(TYPE_CODE, out, self->dPost, self->dData, dOld, numData,
+                    self->dimension, self->numClusters, GMM_MIN_POSTERIOR / self->numClusters) ;
+}
+
+/* M-step finalisation, gmm.c:1169-1202, from moments computed with
+ * oldMeans = the current means.  out = [S0 K | mass K | S1 KD | S2 KD]. */
+static void
+FN(gmm_finalize) (VlGMM * self, double const * out)
+{
+  vl_size K = self->numClusters, D = self->dimension ;
+  TYPE * priors = (TYPE*) self->priors ;
+  TYPE * means = (TYPE*) self->means ;
+  TYPE * covs = (TYPE*) self->covariances ;
+  double const * S0 = out ;
+  double const * S1 = out + 2 * K ;
+  double const * S2 = S1 + K * D ;
+  vl_uindex k, d ;
+  TYPE sum = 0 ;
+  for (k = 0 ; k < K ; ++k) {
+    TYPE mass = (TYPE) S0[k] ;
+    vl_bool update = (mass >= 1e-6 / K) ;
+    priors[k] = mass ;
+    for (d = 0 ; d < D ; ++d) {
+      TYPE oldMu = means[k * D + d] ;
+      if (update) {
+        TYPE mu = (TYPE)(S1[k * D + d] / S0[k]) ;
+        TYPE cv = (TYPE)(S2[k * D + d] / S0[k]) ;
+        TYPE diff = mu - oldMu ;
+        means[k * D + d] = mu ;
+        covs[k * D + d] = cv - diff * diff ;
+      } else {
+        means[k * D + d] = (TYPE) S1[k * D + d] ;
+        covs[k * D + d] = (TYPE) S2[k * D + d] ;
+      }
+    }
+  }
+  FN(gmm_apply_bounds)(self) ;
+  for (k = 0 ; k < K ; ++k) sum += priors[k] ;
+  sum = (TYPE) VL_MAX((double)sum, 1e-12) ;
+  for (k = 0 ; k < K ; ++k) priors[k] /= sum ;
+}
+
+/* Empty-mode restarts, gmm.c:815-999.  mass = total posterior mass per
+ * component (no threshold).  Returns the number of splits performed. */
+static vl_size
+FN(gmm_restart) (VlGMM * self, vl_size numData, double const * massIn)
+{
+  vl_size K = self->numClusters, D = self->dimension ;
+  TYPE * priors = (TYPE*) self->priors ;
+  TYPE * means = (TYPE*) self->means ;
+  TYPE * covs = (TYPE*) self->covariances ;
+  double * mass ;
+  double * out ;
+  vl_index i_cl, j_cl, d ;
+  vl_size splits = 0 ;
+  if (K <= 1) return 0 ;
+  mass = (double*) vl_malloc(sizeof(double) * K) ;
+  memcpy(mass, massIn, sizeof(double) * K) ;
+  out = (double*) vl_malloc(sizeof(double) * (2 * K + 2 * K * D)) ;
+
+  for (i_cl = 0 ; i_cl < (signed)K ; ++i_cl) {
+    double size = -VL_INFINITY_D ;
+    vl_index best = -1 ;
+    if (mass[i_cl] >= GMM_MIN_POSTERIOR * VL_MAX(1.0, (double)numData / K)) continue ;
+    if (self->verbosity) {
+      VL_PRINTF("gmm: mode %d is nearly empty (mass %f)\n", (int)i_cl, mass[i_cl]) ;
+    }
+    /* component with the largest entropy contribution, gmm.c:908-935 */
+    for (j_cl = 0 ; j_cl < (signed)K ; ++j_cl) {
+      double size_ ;
+      if (priors[j_cl] < GMM_MIN_PRIOR) continue ;
+      size_ = 0.5 * D * (1.0 + log(2 * VL_PI)) ;
+      for (d = 0 ; d < (signed)D ; ++d) size_ += 0.5 * log((double)covs[j_cl * D + d]) ;
+      size_ = priors[j_cl] * (size_ - log((double)priors[j_cl])) ;
+      if (size_ > size) { size = size_ ; best = j_cl ; }
+    }
+    j_cl = best ;
+    if (j_cl == i_cl || j_cl < 0) {
+      if (self->verbosity) {
+        VL_PRINTF("gmm: mode %d is empty, but no other mode to split could be found\n", (int)i_cl) ;
+      }
+      continue ;
+    }
+    if (self->verbosity) {
+      VL_PRINTF("gmm: reinitializing empty mode %d with mode %d (prior %f, mass %f, score %f)\n",
+                (int)i_cl, (int)j_cl, (double)priors[j_cl], mass[j_cl], size) ;
+    }
+    /* dimension of maximum variance, gmm.c:958-966 */
+    size = -VL_INFINITY_D ;
+    best = -1 ;
+    for (d = 0 ; d < (signed)D ; ++d) {
+      double s2 = covs[j_cl * D + d] ;
+      if (s2 > size) { size = s2 ; best = d ; }
+    }
+    vlhip_gmm_split(TYPE_CODE, self->dPost, self->dData, numData, D, K, (size_t)i_cl, (size_t)j_cl,
+                    (size_t)best, (double)means[best + j_cl * D]) ;
+    FN(gmm_moments)(self, out, numData) ;
+    FN(gmm_finalize)(self, out) ;
+    splits ++ ;
+  }
+  vl_free(out) ;
+  vl_free(mass) ;
+  return splits ;
+}
+
+/* EM iterations, gmm.c:1220-1308. */
+static double
+FN(gmm_em) (VlGMM * self, TYPE const * data, vl_size numData)
+{
+  vl_size K = self->numClusters, D = self->dimension ;
+  vl_size KD = K * D ;
+  vl_size iteration ;
+  double previousLL = -VL_INFINITY_D ;
+  double LL = -VL_INFINITY_D ;
+  double * out = (double*) vl_malloc(sizeof(double) * (2 * K + 2 * KD)) ;
+  unsigned char * p ;
+
+  vlhip_lock() ;
+  _vl_gmm_prepare_for_data(self, numData) ;
+  gmm_stage_data(self, data, numData) ;
+  self->postRows = numData ;
+  p = (unsigned char*) self->dParams ;
+  FN(gmm_apply_bounds)(self) ;
+
+  for (iteration = 0 ; 1 ; ++iteration) {
+    double eps ;
+    double time = vl_get_cpu_time() ;
+    gmm_upload_params(self) ;
+    LL = vlhip_gmm_posteriors(TYPE_CODE, self->dPost, K, numData,
+                              p + 2 * sizeof(TYPE) * KD, p, D, p + sizeof(TYPE) * KD, self->dData) ;
+    if (self->verbosity > 1) {
+      VL_PRINTF("gmm: em: expectation step completed in %.2f s\n", vl_get_cpu_time() - time) ;
+    }
+    if (self->verbosity) {
+      VL_PRINTF("gmm: em: iteration %d: loglikelihood = %f (variation = %f)\n",
+                (int)iteration, LL, LL - previousLL) ;
+    }
+    if (iteration >= self->maxNumIterations) {
+      if (self->verbosity) {
+        VL_PRINTF("gmm: em: terminating because the maximum number of iterations (%d) has been reached.\n",
+                  (int)self->maxNumIterations) ;
+      }
+      break ;
+    }
+    eps = fabs((LL - previousLL) / LL) ;
+    if ((iteration > 0) && (eps < 0.00001)) {
+      if (self->verbosity) {
+        VL_PRINTF("gmm: em: terminating because the algorithm fully converged (log-likelihood variation = %f).\n", eps) ;
+      }
+      break ;
+    }
+    previousLL = LL ;
+
+    FN(gmm_moments)(self, out, numData) ;
+    if (iteration > 1) {
+      if (FN(gmm_restart)(self, numData, out + K) > 0) {
+        FN(gmm_moments)(self, out, numData) ;
+      }
+    }
+    FN(gmm_finalize)(self, out) ;
+  }
+  self->hostPostValid = VL_FALSE ;
+  vlhip_unlock() ;
+  vl_free(out) ;
+  return LL ;
+}
+
+/* gmm.c:1315-1370 */
+static void
+FN(gmm_init_kmeans) (VlGMM * self, TYPE const * data, vl_size numData, VlKMeans * kmeansInit)
+{
+  vl_size K = self->numClusters, D = self->dimension ;
+  vl_uint32 * assignments = (vl_uint32*) vl_malloc(sizeof(vl_uint32) * (numData ? numData : 1)) ;
+  double * out = (double*) vl_malloc(sizeof(double) * (2 * K + 2 * K * D)) ;
+  vl_uint32 * dAssign ;
+
+  _vl_gmm_prepare_for_data(self, numData) ;
+  memset(self->means, 0, sizeof(TYPE) * K * D) ;
+  memset(self->priors, 0, sizeof(TYPE) * K) ;
+  memset(self->covariances, 0, sizeof(TYPE) * K * D) ;
+
+  if (kmeansInit) vl_gmm_set_kmeans_init_object(self, kmeansInit) ;
+  if (self->kmeansInit == NULL) {
+    /* the reference's default initialiser, gmm.c:1336-1355 */
+    vl_size ncomparisons = VL_MAX(numData / 4, 10) ;
+    VlKMeans * km = vl_kmeans_new(self->dataType, VlDistanceL2) ;
+    vl_kmeans_set_initialization(km, VlKMeansRandomSelection) ;
+    vl_kmeans_set_max_num_iterations(km, 5) ;
+    vl_kmeans_set_max_num_comparisons(km, ncomparisons) ;
+    vl_kmeans_set_num_trees(km, 1) ;
+    vl_kmeans_set_algorithm(km, VlKMeansANN) ;
+    vl_kmeans_set_num_repetitions(km, 1) ;
+    vl_kmeans_set_verbosity(km, self->verbosity) ;
+    self->kmeansInit = km ;
+    self->kmeansInitIsOwner = VL_TRUE ;
+  }
+
+  vl_kmeans_cluster(self->kmeansInit, data, D, numData, K) ;
+  vl_kmeans_quantize(self->kmeansInit, assignments, NULL, data, numData) ;
+
+  vlhip_lock() ;
+  gmm_stage_data(self, data, numData) ;
+  dAssign = (vl_uint32*) vlhip_buffer(VLHIP_BUF_AUX, sizeof(vl_uint32) * (numData ? numData : 1)) ;
+  if (numData) vlhip_memcpy_h2d(dAssign, assignments, sizeof(vl_uint32) * numData) ;
+  vlhip_gmm_onehot(TYPE_CODE, self->dPost, dAssign, numData, K) ;
+  self->postRows = numData ;
+  self->hostPostValid = VL_FALSE ;
+  FN(gmm_moments)(self, out, numData) ;
+  FN(gmm_finalize)(self, out) ;
+  vlhip_unlock() ;
+  vl_free(out) ;
+  vl_free(assignments) ;
+}
+
+/* gmm.c:1376-1413: per-dimension unbiased variance, summed in TYPE in data order. */
+static void
+FN(gmm_init_sigma) (TYPE const * data, TYPE * initSigma, vl_size D, vl_size numData)
+{
+  vl_uindex i, d ;
+  TYPE * dataMean ;
+  memset(initSigma, 0, sizeof(TYPE) * D) ;
+  if (numData <= 1) return ;
+  dataMean = (TYPE*) vl_calloc(D, sizeof(TYPE)) ;
+  for (d = 0 ; d < D ; ++d) {
+    for (i = 0 ; i < numData ; ++i) dataMean[d] += data[i * D + d] ;
+    dataMean[d] /= numData ;
+  }
+  for (d = 0 ; d < D ; ++d) {
+    for (i = 0 ; i < numData ; ++i) {
+      TYPE diff = data[i * D + d] - dataMean[d] ;
+      initSigma[d] += diff * diff ;
+    }
+    initSigma[d] /= numData - 1 ;
+  }
+  vl_free(dataMean) ;
+}
+
+/* gmm.c:1415-1443 */
+static void
+FN(gmm_init_rand) (VlGMM * self, TYPE const * data, vl_size numData)
+{
+  vl_size K = self->numClusters, D = self->dimension ;
+  vl_uindex i, k, d ;
+  VlKMeans * kmeans ;
+  _vl_gmm_prepare_for_data(self, numData) ;
+  for (i = 0 ; i < K ; ++i) ((TYPE*)self->priors)[i] = (TYPE)(1.0 / K) ;
+  FN(gmm_init_sigma)(data, (TYPE*)self->covariances, D, numData) ;
+  for (k = 1 ; k < K ; ++k)
+    for (d = 0 ; d < D ; ++d)
+      ((TYPE*)self->covariances)[k * D + d] = ((TYPE*)self->covariances)[d] ;
+  kmeans = vl_kmeans_new(self->dataType, VlDistanceL2) ;
+  vl_kmeans_init_centers_plus_plus(kmeans, data, D, numData, K) ;
+  memcpy(self->means, vl_kmeans_get_centers(kmeans), sizeof(TYPE) * D * K) ;
+  vl_kmeans_delete(kmeans) ;
+}
+
+#undef FN
+#undef TYPE
+#undef TYPE_CODE
