@@ -1,0 +1,256 @@
+"""GPU parity: the HIP path (through the C ABI) against the reference's own
+float/double implementation (compiled from source, when present) and the FP64
+oracle, on identical seeded inputs.
+
+Tolerances (stated per test): Fisher / VLAD vectors within 1e-4 relative L2 of
+the reference float path; GMM means / variances within 1e-4 relative; hard
+assignments bit-exact; double-precision paths within 1e-9 of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import vlfeat_amd as va
+from oracle import oracle, reference
+from vlfeat_amd import synth
+
+pytestmark = pytest.mark.gpu
+needs_ref = pytest.mark.skipif(reference() is None, reason="reference library not built")
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def gmm_params(x, K, seed=0, dtype=np.float32):
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(x.shape[0], K, replace=False)
+    means = x[idx].astype(dtype)
+    covs = (np.var(x, 0)[None, :] * rng.uniform(0.1, 0.5, (K, x.shape[1]))).astype(dtype)
+    pri = rng.dirichlet(np.ones(K)).astype(dtype)
+    return means, covs, pri
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    x = synth.mixture(6000, 64, 64, seed=0)
+    return x, *gmm_params(x, 64)
+
+
+@pytest.fixture(scope="module")
+def cfg2():
+    x = synth.sift_like(4000, 80, seed=1)
+    return x, *gmm_params(x, 128, seed=2)
+
+
+# ---------------------------------------------------------------- posteriors
+
+@pytest.mark.parametrize("case", ["cfg1", "cfg2"])
+def test_posteriors_float(case, request):
+    x, m, c, p = request.getfixturevalue(case)
+    q, ll = va.lib().gmm_posteriors(x, m, c, p)
+    qo, llo = oracle().posteriors(x, m, c, p)
+    assert rel(q, qo) < 1e-5
+    assert abs(ll - llo) / abs(llo) < 1e-6
+    np.testing.assert_allclose(q.sum(1), 1, atol=1e-5)
+    R = reference()
+    if R is not None:
+        qr, llr = R.gmm_posteriors(x, m, c, p)
+        assert rel(q, qr) < 1e-4
+        assert abs(ll - llr) / abs(llr) < 1e-6
+
+
+def test_posteriors_double(cfg1):
+    x, m, c, p = [a.astype(np.float64) for a in cfg1]
+    q, ll = va.lib().gmm_posteriors(x, m, c, p)
+    qo, llo = oracle().posteriors(x, m, c, p)
+    assert rel(q, qo) < 1e-12
+    assert abs(ll - llo) / abs(llo) < 1e-12
+
+
+def test_posteriors_tiny_prior_and_single_component():
+    x = synth.mixture(500, 16, 4, seed=3)
+    m, c, p = gmm_params(x, 3)
+    p = np.array([1 - 5e-7, 5e-7, 0], np.float32)
+    q, _ = va.lib().gmm_posteriors(x, m, c, p)
+    assert np.all(q[:, 1:] == 0)
+    q1, ll1 = va.lib().gmm_posteriors(x, m[:1], c[:1], np.ones(1, np.float32))
+    assert np.all(q1 == 1)
+    _, llo = oracle().posteriors(x, m[:1], c[:1], np.ones(1))
+    assert abs(ll1 - llo) / abs(llo) < 1e-6
+
+
+# ---------------------------------------------------------------- EM
+
+def test_em_custom_init_float_vs_oracle_and_reference(cfg1):
+    x, m, c, p = cfg1
+    out = va.vl_gmm(x, 64, init="custom", means=m, covs=c, priors=p, max_iter=15)
+    mo, co, po, llo, _ = oracle().em(x, m, c, p, 15)
+    assert rel(out[0], mo) < 1e-4 and rel(out[1], co) < 1e-4 and rel(out[2], po) < 1e-4
+    assert abs(out[3] - llo) / abs(llo) < 1e-5
+    R = reference()
+    if R is not None:
+        R.set_num_threads(1)
+        mr, cr, pr, llr, qr = R.gmm(x, 64, init="custom", means=m, covs=c, priors=p, max_iter=15)
+        assert rel(out[0], mr) < 1e-4 and rel(out[1], cr) < 1e-4 and rel(out[2], pr) < 1e-4
+        assert rel(out[4], qr) < 1e-3
+
+
+def test_em_double_with_restart_vs_oracle():
+    rng = np.random.default_rng(8)
+    N, D, K = 3000, 12, 8
+    x = np.concatenate([rng.standard_normal((N // 2, D)), rng.standard_normal((N // 2, D)) + 4])
+    m = rng.standard_normal((K, D))
+    m[5] = 60.0  # far from all data: the restart path must split a mode
+    c = np.ones((K, D))
+    p = np.ones(K) / K
+    out = va.vl_gmm(x, K, init="custom", means=m, covs=c, priors=p, max_iter=10)
+    mo, co, po, llo, qo = oracle().em(x, m, c, p, 10)
+    assert rel(out[0], mo) < 1e-9 and rel(out[1], co) < 1e-9 and rel(out[2], po) < 1e-9
+    assert abs(out[3] - llo) / abs(llo) < 1e-10
+    assert rel(out[4], qo) < 1e-9
+
+
+@needs_ref
+@pytest.mark.parametrize("init", ["kmeans", "rand"])
+def test_gmm_cluster_seeded_init_matches_reference(init):
+    """Full vl_gmm_cluster with the reference's own initialisers (k-means via
+    ANN kd-forest, or k-means++) from the same generator seed."""
+    x = synth.mixture(20000, 32, 32, seed=5)
+    R = reference()
+    R.set_num_threads(1)
+    res = []
+    for lib in (va.lib(), R):
+        lib.seed(49000)
+        res.append(lib.gmm(x, 32, init=init, max_iter=20, cov_bound=1e-4))
+    (m, c, p, ll, q), (mr, cr, pr, llr, qr) = res
+    assert rel(m, mr) < 1e-4 and rel(c, cr) < 1e-4 and rel(p, pr) < 1e-4
+    assert abs(ll - llr) / abs(llr) < 1e-5
+
+
+# ---------------------------------------------------------------- Fisher
+
+FLAGS = {"plain": 0, "sqrt": 1, "norm": 2, "improved": 3, "fast": 7}
+
+
+@pytest.mark.parametrize("case", ["cfg1", "cfg2"])
+@pytest.mark.parametrize("flags", list(FLAGS.values()), ids=list(FLAGS))
+def test_fisher_float(case, flags, request):
+    x, m, c, p = request.getfixturevalue(case)
+    xs = x[:2000]
+    enc, n = va.vl_fisher(xs, m, c, p, flags)
+    eo, no = oracle().fisher(xs, m, c, p, flags)
+    assert rel(enc, eo) < 1e-4
+    R = reference()
+    if R is not None:
+        er, nr = R.fisher(xs, m, c, p, flags)
+        assert rel(enc, er) < 1e-4
+        assert abs(int(n) - int(nr)) <= max(2, nr // 10000)
+
+
+@pytest.mark.parametrize("flags", [0, 3, 7])
+def test_fisher_double(cfg1, flags):
+    x, m, c, p = [a.astype(np.float64) for a in cfg1]
+    enc, n = va.vl_fisher(x[:1500], m, c, p, flags)
+    eo, no = oracle().fisher(x[:1500], m, c, p, flags)
+    assert rel(enc, eo) < 1e-10
+    assert n == no
+
+
+def test_fisher_batch_uneven_equals_per_image(cfg2):
+    x, m, c, p = cfg2
+    counts = [0, 1, 700, 64, 65, 1, 1300, 3, 0, 257, 129]
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint64)
+    xs = x[: int(off[-1])]
+    enc, terms = va.vl_fisher_batch(xs, off, m, c, p, 3)
+    for i in range(len(counts)):
+        e1, n1 = va.vl_fisher(xs[off[i]:off[i + 1]], m, c, p, 3)
+        assert np.array_equal(enc[i], e1), i
+        assert terms[i] == n1
+    assert np.all(enc[0] == 0) and np.all(enc[8] == 0)
+
+
+def test_fisher_skips_tiny_prior_component(cfg1):
+    x, m, c, p = cfg1
+    p = p.copy()
+    p[3] = 1e-7
+    p /= p.sum()
+    enc, _ = va.vl_fisher(x[:500], m, c, p, 0)
+    K, D = m.shape
+    assert np.all(enc[3 * D:4 * D] == 0) and np.all(enc[K * D + 3 * D:K * D + 4 * D] == 0)
+    eo, _ = oracle().fisher(x[:500], m, c, p, 0)
+    assert rel(enc, eo) < 1e-4
+
+
+# ---------------------------------------------------------------- VLAD
+
+@pytest.mark.parametrize("flags", [0, 1, 2, 4, 8, 15, 4 | 8])
+def test_vlad_soft_float(flags):
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((256, 128)).astype(np.float32)
+    mu = rng.standard_normal((16, 128)).astype(np.float32)
+    a = rng.uniform(size=(256, 16)).astype(np.float32)
+    a /= a.sum(1, keepdims=True)
+    enc = va.vl_vlad(x, mu, a, flags)
+    assert rel(enc, oracle().vlad(x, mu, a, flags)) < 1e-5
+    R = reference()
+    if R is not None:
+        assert np.array_equal(enc, R.vlad(x, mu, a, flags))  # same arithmetic, same order
+
+
+def test_vlad_known_answers():
+    x = np.array([[1.0], [2.0], [3.0]], np.float32)
+    eye = np.eye(3, dtype=np.float32)
+    assert np.array_equal(va.vl_vlad(x, np.zeros((3, 1), np.float32), eye, 4), [1, 2, 3])
+    mu = np.array([[0.0], [1.0], [2.0]], np.float32)
+    assert np.array_equal(va.vl_vlad(x, mu, eye, 4), [1, 1, 1])
+    assert np.array_equal(va.vl_vlad(np.concatenate([x, x]), mu, np.concatenate([eye, eye]), 4), [2, 2, 2])
+
+
+def test_vlad_hard_equals_one_hot_and_double(cfg2):
+    x, m, c, p = cfg2
+    a, _ = va.vl_kmeans_quantize(x, m)
+    onehot = np.eye(m.shape[0], dtype=np.float32)[a]
+    for flags in (0, 3, 8):
+        assert np.array_equal(va.vl_vlad_hard(x, m, a, flags), va.vl_vlad(x, m, onehot, flags))
+    xd, md = x.astype(np.float64), m.astype(np.float64)
+    assert rel(va.vl_vlad(xd, md, onehot.astype(np.float64), 1), oracle().vlad(xd, md, onehot, 1)) < 1e-12
+
+
+# ---------------------------------------------------------------- k-means
+
+@pytest.mark.parametrize("dtype,D", [(np.float32, 80), (np.float32, 13), (np.float64, 64)])
+def test_quantize_bit_exact(dtype, D):
+    rng = np.random.default_rng(D)
+    x = rng.standard_normal((20000, D)).astype(dtype)
+    cen = rng.standard_normal((300, D)).astype(dtype)
+    a, d = va.vl_kmeans_quantize(x, cen)
+    ao, do = oracle().quantize(x, cen)
+    assert np.mean(a == ao) > 0.999
+    R = reference()
+    if R is not None:
+        km = R.kmeans_new(dtype)
+        R.vl_kmeans_set_centers(km, cen.ctypes.data, D, 300)
+        ar, dr = R.kmeans_quantize(km, x)
+        R.vl_kmeans_delete(km)
+        assert np.array_equal(a, ar)
+        assert np.array_equal(d, dr)
+
+
+@needs_ref
+@pytest.mark.parametrize("algo,comparisons", [(va.VlKMeansLloyd, 100), (va.VlKMeansANN, 20),
+                                              (va.VlKMeansANN, 5000)])
+def test_kmeans_cluster_matches_reference(algo, comparisons):
+    x = synth.mixture(5000, 16, 20, seed=9)
+    R = reference()
+    R.set_num_threads(1)
+    res = []
+    for lib in (va.lib(), R):
+        lib.seed(1)
+        km = lib.kmeans_new(np.float32, algorithm=algo, max_iter=10, num_trees=2, max_comparisons=comparisons)
+        e = lib.vl_kmeans_cluster(km, x.ctypes.data, 16, 5000, 40)
+        res.append((lib.kmeans_centers(km, np.float32), e))
+        lib.vl_kmeans_delete(km)
+    assert np.array_equal(res[0][0], res[1][0])
+    assert res[0][1] == res[1][1]
