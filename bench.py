@@ -83,7 +83,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", default="fp32", choices=["fp32", "bf16x3", "gated"])
+    ap.add_argument("--mode", default="fp32", choices=["fp32", "bf16x3", "f16x3", "gated"])
     ap.add_argument("--images", type=int, default=NUM_IMAGES)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -119,7 +119,7 @@ def main() -> int:
     od = torch.as_tensor(offsets.astype(np.int64), device=dev)
     enc = torch.empty((n_img, 2 * K_COMP * DIM), dtype=torch.float32, device=dev)
     terms = torch.zeros(n_img, dtype=torch.int64, device=dev)
-    mode = {"fp32": va.MODE_FP32, "bf16x3": va.MODE_BF16X3, "gated": va.MODE_GATED}[args.mode]
+    mode = {"fp32": va.MODE_FP32, "bf16x3": va.MODE_BF16X3, "f16x3": va.MODE_F16X3, "gated": va.MODE_GATED}[args.mode]
     torch.cuda.synchronize()
 
     def step():
@@ -162,7 +162,7 @@ def main() -> int:
         "metric": "Fisher descriptors/s at K=1024,D=80: gated BF16 vs plain FP32 GPU, rel-err<=1e-3",
         "value": value, "unit": "descriptors/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"fp32": "fp32", "bf16x3": "bf16", "gated": "bf16"}[args.mode],
+        "dtype": {"fp32": "fp32", "bf16x3": "bf16", "f16x3": "fp16", "gated": "bf16"}[args.mode],
         "data": "synthetic (SIFT-like rectified/clipped Gaussian projected to 80-d, seed 1; GMM fitted by GPU EM)",
         "config": {"workload": "improved Fisher vectors, 4096 images x 3000 descriptors, K=1024, D=80",
                    "K": K_COMP, "D": DIM, "images": n_img, "descriptors_per_image": PER_IMAGE,

@@ -39,4 +39,16 @@ VL_EXPORT void vl_fisher_encode_batch
  void const * data, vl_uint64 const * offsets, vl_size numImages,
  int flags, vl_size * numTerms) ;
 
+/* Arithmetic of the float encoder for the calling thread (no counterpart in
+ * the reference, which has one float path).  VL_FISHER_PRECISION_FP32, the
+ * default, follows the reference's float arithmetic and is the parity path;
+ * the SPLIT16 settings run both contractions on the GPU's matrix cores with
+ * compensated 16-bit operands (relative error against an FP64 evaluation of
+ * order 1e-5 for FP16, 1e-4 for BF16).  Double data always uses FP64. */
+#define VL_FISHER_PRECISION_FP32     0
+#define VL_FISHER_PRECISION_BF16X3   1
+#define VL_FISHER_PRECISION_F16X3    3
+VL_EXPORT void vl_fisher_set_precision (int precision) ;
+VL_EXPORT int vl_fisher_get_precision (void) ;
+
 #endif /* VL_FISHER_H */

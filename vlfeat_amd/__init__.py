@@ -17,7 +17,7 @@ import numpy as np
 from .capi import (  # noqa: F401
     CLib, VL_TYPE_DOUBLE, VL_TYPE_FLOAT, FISHER_FAST, FISHER_IMPROVED, FISHER_NORMALIZED,
     FISHER_SQUARE_ROOT, VLAD_NORMALIZE_COMPONENTS, VLAD_NORMALIZE_MASS, VLAD_SQUARE_ROOT,
-    VLAD_UNNORMALIZED, MODE_BF16X3, MODE_FP32, MODE_GATED, VlKMeansANN, VlKMeansElkan,
+    VLAD_UNNORMALIZED, MODE_BF16X3, MODE_F16X3, MODE_FP32, MODE_GATED, VlKMeansANN, VlKMeansElkan,
     VlKMeansLloyd, VlKMeansPlusPlus, VlKMeansRandomSelection, dtype_code, ptr,
 )
 
@@ -51,6 +51,22 @@ def vl_fisher(data, means, covariances, priors, flags=0):
     d = np.ascontiguousarray(data)
     return lib().fisher(d, np.ascontiguousarray(means, d.dtype), np.ascontiguousarray(covariances, d.dtype),
                         np.ascontiguousarray(priors, d.dtype), flags)
+
+
+class fisher_precision:
+    """Context manager selecting the float Fisher arithmetic (MODE_FP32,
+    MODE_BF16X3 or MODE_F16X3) for the calling thread."""
+
+    def __init__(self, mode):
+        self.mode = mode
+
+    def __enter__(self):
+        self.prev = lib().vl_fisher_get_precision()
+        lib().vl_fisher_set_precision(self.mode)
+        return self
+
+    def __exit__(self, *exc):
+        lib().vl_fisher_set_precision(self.prev)
 
 
 def vl_fisher_batch(data, offsets, means, covariances, priors, flags=0):

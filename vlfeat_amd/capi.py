@@ -32,7 +32,7 @@ VLAD_SQUARE_ROOT = 2
 VLAD_UNNORMALIZED = 4
 VLAD_NORMALIZE_MASS = 8
 
-MODE_FP32, MODE_BF16X3, MODE_GATED = 0, 1, 2
+MODE_FP32, MODE_BF16X3, MODE_GATED, MODE_F16X3 = 0, 1, 2, 3
 
 _p = C.c_void_p
 _sz = C.c_uint64
@@ -108,6 +108,8 @@ _SIGS = {
 # Entry points present only in this library (batched forms and the device ABI).
 _EXT_SIGS = {
     "vl_fisher_encode_batch": (None, [_p, C.c_uint32, _p, _sz, _sz, _p, _p, _p, _p, _sz, C.c_int, _p]),
+    "vl_fisher_set_precision": (None, [C.c_int]),
+    "vl_fisher_get_precision": (C.c_int, []),
     "vl_vlad_encode_hard": (None, [_p, C.c_uint32, _p, _sz, _sz, _p, _sz, _p, C.c_int]),
     "vlhip_available": (C.c_int, []),
     "vlhip_init": (None, []),

@@ -275,6 +275,11 @@ __global__ __launch_bounds__(256) void fisher_normalize_kernel(T* __restrict__ e
   for (size_t i = threadIdx.x; i < len; i += blockDim.x) e[i] = e[i] / n;
 }
 
+void launch_fisher_normalize(float* enc, const double* normpart, int numKB, size_t numImages, size_t len) {
+  VLHIP_LAUNCH("fisher_normalize", (fisher_normalize_kernel<float>), dim3((unsigned)numImages), dim3(256), 0, enc,
+               normpart, numKB, len);
+}
+
 template <typename T, int DPT, bool RESIDENT>
 static void launch_fisher_accum(const T* x, const uint64_t* offsets, int numImages, int D, int K, const T* means,
                                 const GmmPrep<T>& prep, const T* priors, const T* rm, const T* rs, const int* ra,
@@ -358,6 +363,9 @@ extern "C" void vlhip_fisher_encode_batch(int dtype, void* enc, void const* mean
   if (mode == VLHIP_MODE_FP32) {
     fisher_fp_t<float>((float*)enc, (const float*)means, (const float*)covs, (const float*)priors, D, K,
                        (const float*)data, offsets, numImages, flags, numTerms);
+  } else if (mode == VLHIP_MODE_BF16X3 || mode == VLHIP_MODE_F16X3) {
+    fisher_split16((float*)enc, (const float*)means, (const float*)covs, (const float*)priors, D, K,
+                   (const float*)data, offsets, numImages, flags, numTerms, mode == VLHIP_MODE_F16X3);
   } else {
     fatal("vlhip_fisher_encode_batch", __FILE__, __LINE__, "unknown mode");
   }

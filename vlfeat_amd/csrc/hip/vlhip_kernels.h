@@ -33,4 +33,13 @@ template <typename T>
 double run_estep(const T* x, size_t N, size_t D, size_t K, const T* means, const GmmPrep<T>& prep,
                  T* q, T* row_m, T* row_s, int* row_arg, const char* name, bool want_ll);
 
+// Divides every image's encoding by max(||enc||, 1e-12) (fisher.c:474-485).
+void launch_fisher_normalize(float* enc, const double* normpart, int numKB, size_t numImages, size_t len);
+
+// Split-16 matrix-core Fisher encoder (fisher_mfma.hip); fp16 selects fp16
+// operands, otherwise bf16.
+void fisher_split16(float* enc, const float* means, const float* covs, const float* priors, size_t D, size_t K,
+                    const float* x, const uint64_t* offsets, size_t numImages, int flags,
+                    unsigned long long* numTerms, bool fp16);
+
 }  // namespace vlhip

@@ -16,6 +16,21 @@
 #include <stdio.h>
 #include <string.h>
 
+static __thread int g_precision = VL_FISHER_PRECISION_FP32 ;
+
+VL_EXPORT void
+vl_fisher_set_precision (int precision)
+{
+  if (precision != VL_FISHER_PRECISION_FP32 && precision != VL_FISHER_PRECISION_BF16X3 &&
+      precision != VL_FISHER_PRECISION_F16X3) {
+    VL_PRINTF("vl_fisher_set_precision: unknown precision %d\n", precision) ;
+    abort() ;
+  }
+  g_precision = precision ;
+}
+
+VL_EXPORT int vl_fisher_get_precision (void) { return g_precision ; }
+
 static void
 check_type (vl_type t, char const * who)
 {
@@ -53,7 +68,7 @@ fisher_run (void * enc, vl_type dataType, void const * means, vl_size dimension,
   vlhip_memcpy_h2d(dOff, offsets, sizeof(vl_uint64) * (numImages + 1)) ;
   vlhip_fisher_encode_batch((int)dataType, dEnc, params, params + es * KD, params + 2 * es * KD,
                             dimension, numClusters, dData, (uint64_t const*)dOff, numImages, flags,
-                            VLHIP_MODE_FP32, dTerms) ;
+                            dataType == VL_TYPE_FLOAT ? g_precision : VLHIP_MODE_FP32, dTerms) ;
   vlhip_memcpy_d2h(enc, dEnc, es * 2 * KD * numImages) ;
   if (numTermsOut) {
     unsigned long long * h = (unsigned long long*) malloc(sizeof(unsigned long long) * numImages) ;
