@@ -341,19 +341,21 @@ template <typename H, int NS>
 struct MfLds {
   static constexpr int Dp = NS * 16;
   static constexpr int RS = MF_ROWS + 8;  // row stride (halves) of feature-major tiles
-  float xf[MF_ROWS][Dp];                   // centred coordinates; later S1 of a component half
-  H xct[4 * Dp][RS];                       // feature-major split features; later FP32 scratch
+  static constexpr int RW = 4 * Dp + 8;   // row stride (halves) of row-major tiles
+  // row-major split features [x'^2_hi | x'_hi | x'^2_lo | x'_lo] (logit A operand)
+  H xr[MF_ROWS][RW];
+  // feature-major split features, planes x'_hi, x'_lo, x'^2_hi, x'^2_lo (moment B operand)
+  H xct[4 * Dp][RS];
   H qt[MF_COMPS][RS];                      // component-major rounded posteriors
   float rm[MF_ROWS], rinv[MF_ROWS];
   int ra[MF_ROWS];
   float s0[2][MF_COMPS];
-  // the finalisation reuses xf, xct and qt (contiguous) as [2][MF_COMPS][Dp] floats
+  // the finalisation reuses xr, xct and qt (contiguous) as [2][MF_COMPS][Dp] floats
   static constexpr bool kScratchFits =
-      sizeof(float) * MF_ROWS * Dp + sizeof(H) * 4 * Dp * RS + sizeof(H) * MF_COMPS * RS >=
+      sizeof(H) * MF_ROWS * RW + sizeof(H) * 4 * Dp * RS + sizeof(H) * MF_COMPS * RS >=
       sizeof(float) * 2 * MF_COMPS * Dp;
   static_assert(kScratchFits, "finalisation scratch does not fit");
 };
-
 
 __device__ __forceinline__ void fisher_mfma_coords(int numKB, int numImages, int& img, int& kb) {
   const int b = blockIdx.x;
@@ -422,28 +424,56 @@ __global__ __launch_bounds__(256, 1) void mfma_fisher_kernel(
 #pragma unroll
     for (int ft = 0; ft < NS; ++ft) macc[ct][ft] = f32x4{0.f, 0.f, 0.f, 0.f};
   float s0l[4] = {0.f, 0.f, 0.f, 0.f};
-  unsigned long long terms = 0;
+  unsigned terms = 0;
+  // (double)q < 1e-6 for a float q is q < the next float above float(1e-6)
+  const float qmin = __uint_as_float(0x358637beu);  // 1.0000001e-6
+
+  // staging items: (4-row group, dimension); the next chunk's coordinates are
+  // prefetched into registers while the current chunk computes.
+  constexpr int ITEMS = ((MF_ROWS / 4) * Dp + 255) / 256;
+  float xpre[ITEMS][4];
+  auto prefetch = [&](long long c0) {
+    const int n = (int)min((long long)MF_ROWS, rEnd - c0);
+#pragma unroll
+    for (int q = 0; q < ITEMS; ++q) {
+      const int it = tid + 256 * q;
+      const int rq = it / Dp, d = it - rq * Dp;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = 4 * rq + i;
+        xpre[q][i] = (it < (MF_ROWS / 4) * Dp && r < n && d < D) ? x[(c0 + r) * D + d] : 0.f;
+      }
+    }
+  };
+  if (rBegin < rEnd) prefetch(rBegin);
 
   for (long long r0 = rBegin; r0 < rEnd; r0 += MF_ROWS) {
     const int nr = (int)min((long long)MF_ROWS, rEnd - r0);
     __syncthreads();
-    // ---- stage: item = (4-row group, dimension)
-    for (int it = tid; it < (MF_ROWS / 4) * Dp; it += 256) {
-      const int rq = it / Dp, d = it - rq * Dp;
-      const float gd = d < D ? g[d] : 0.f;
-      H h1[4], l1[4], h2[4], l2[4];
+    // ---- stage the prefetched chunk
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int r = 4 * rq + i;
-        const float xc = (r < nr && d < D) ? x[(r0 + r) * D + d] - gd : 0.f;
-        sm.xf[r][d] = xc;
-        split16<H>(xc * sc.s1, h1[i], l1[i]);
-        split16<H>(xc * xc * sc.s2, h2[i], l2[i]);
+    for (int q = 0; q < ITEMS; ++q) {
+      const int it = tid + 256 * q;
+      if (it < (MF_ROWS / 4) * Dp) {
+        const int rq = it / Dp, d = it - rq * Dp;
+        const float gd = d < D ? g[d] : 0.f;
+        H h1[4], l1[4], h2[4], l2[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = 4 * rq + i;
+          const float xc = (r < nr && d < D) ? xpre[q][i] - gd : 0.f;
+          split16<H>(xc * sc.s1, h1[i], l1[i]);
+          split16<H>(xc * xc * sc.s2, h2[i], l2[i]);
+          sm.xr[r][d] = h2[i];
+          sm.xr[r][Dp + d] = h1[i];
+          sm.xr[r][2 * Dp + d] = l2[i];
+          sm.xr[r][3 * Dp + d] = l1[i];
+        }
+        store4<H>(&sm.xct[d][4 * rq], h1[0], h1[1], h1[2], h1[3]);
+        store4<H>(&sm.xct[Dp + d][4 * rq], l1[0], l1[1], l1[2], l1[3]);
+        store4<H>(&sm.xct[2 * Dp + d][4 * rq], h2[0], h2[1], h2[2], h2[3]);
+        store4<H>(&sm.xct[3 * Dp + d][4 * rq], l2[0], l2[1], l2[2], l2[3]);
       }
-      store4<H>(&sm.xct[d][4 * rq], h1[0], h1[1], h1[2], h1[3]);
-      store4<H>(&sm.xct[Dp + d][4 * rq], l1[0], l1[1], l1[2], l1[3]);
-      store4<H>(&sm.xct[2 * Dp + d][4 * rq], h2[0], h2[1], h2[2], h2[3]);
-      store4<H>(&sm.xct[3 * Dp + d][4 * rq], l2[0], l2[1], l2[2], l2[3]);
     }
     if (tid < MF_ROWS) {
       const bool ok = tid < nr;
@@ -451,6 +481,7 @@ __global__ __launch_bounds__(256, 1) void mfma_fisher_kernel(
       sm.rinv[tid] = ok ? 1.0f / row_s[r0 + tid] : 0.f;
       sm.ra[tid] = ok ? row_arg[r0 + tid] : -1;
     }
+    if (r0 + MF_ROWS < rEnd) prefetch(r0 + MF_ROWS);
     __syncthreads();
 
     // ---- logits of rows 32wr.. x comps 64wc..
@@ -462,18 +493,12 @@ __global__ __launch_bounds__(256, 1) void mfma_fisher_kernel(
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
       const int f0 = 32 * j + 8 * lg;
-      const bool sq = f0 < Dp;
-      const int d0 = sq ? f0 : f0 - Dp;
       V8 ah[2], al[2];
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
         const int rl = 32 * wr + 16 * t + lr;
-        float xc[8];
-        const float4 a = *reinterpret_cast<const float4*>(&sm.xf[rl][d0]);
-        const float4 b = *reinterpret_cast<const float4*>(&sm.xf[rl][d0 + 4]);
-        xc[0] = a.x; xc[1] = a.y; xc[2] = a.z; xc[3] = a.w;
-        xc[4] = b.x; xc[5] = b.y; xc[6] = b.z; xc[7] = b.w;
-        make_frag<H>(xc, sq, sc.s1, sc.s2, ah[t], al[t]);
+        ah[t] = *reinterpret_cast<const V8*>(&sm.xr[rl][f0]);
+        al[t] = *reinterpret_cast<const V8*>(&sm.xr[rl][2 * Dp + f0]);
       }
 #pragma unroll
       for (int t = 0; t < 2; ++t)
@@ -507,7 +532,7 @@ __global__ __launch_bounds__(256, 1) void mfma_fisher_kernel(
               const float l = acc[t][c][r] + bcomp[c];
               q = __builtin_amdgcn_exp2f((l - sm.rm[rl]) * kLog2e) * sm.rinv[rl];
             }
-            if ((double)q < 1e-6) q = 0.f; else terms += 1;
+            if (q < qmin) q = 0.f; else terms += 1;
           }
           hq[r] = H(q * sc.sq);
           s0l[c] += (float)hq[r];
@@ -552,7 +577,7 @@ __global__ __launch_bounds__(256, 1) void mfma_fisher_kernel(
   // ---- finalisation: every wave dumps its (plane pair, component half)
   // moments into LDS as FP32 -- the staging buffers are free now -- and all
   // threads then finish (component, dimension) pairs.
-  float* F = &sm.xf[0][0];  // [2 planes: S1, S2][128 comps][Dp]; spans xf, xct and qt
+  float* F = reinterpret_cast<float*>(&sm.xr[0][0]);  // [2 planes: S1, S2][128 comps][Dp]; spans xr, xct, qt
 #pragma unroll
   for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
