@@ -1,20 +1,17 @@
 // fisher_kernels.hip -- Fisher vector encoder (vl/fisher.c), FP32/FP64
 // direct-form path.
 //
-// Two passes per batch of images:
-//   1. the E-step kernel (gmm_kernels.hip) sweeps all K components for every
-//      descriptor and keeps only the row statistics (max logit, sum of
-//      exponentials, argmax) -- no N x K posterior matrix is written;
-//   2. fisher_accum_kernel: workgroup (image, component block) recomputes the
-//      logits of its FA_COMPS components with the same instructions as pass 1
-//      (so the posteriors are bit-identical to the E-step's), thresholds them
-//      as fisher.c:436-441 does, stages them in LDS and accumulates
+// Per group of images whose N x K posterior matrix fits a 4 GiB budget:
+//   1. the E-step kernel (gmm_kernels.hip) writes the normalised posteriors
+//      of the group's descriptors (gmm.c:758-791) and their argmax;
+//   2. fisher_accum_kernel: workgroup (image, component block) thresholds the
+//      posteriors as fisher.c:436-441 does, stages them in LDS and accumulates
 //        u[k,d] = sum_i q_ik t_ikd,  v[k,d] = sum_i q_ik t_ikd^2 - sum_i q_ik,
 //        t_ikd = (x_id - mu_kd) * sigma_kd^-1            (fisher.c:442-447)
 //      in registers, then applies the prefixes, the signed square root and
 //      writes its slice of the encoding plus its share of the squared norm;
-//   3. fisher_normalize_kernel divides each image's encoding by its L2 norm
-//      (fisher.c:474-485).
+// then fisher_normalize_kernel divides each image's encoding by its L2 norm
+// (fisher.c:474-485).
 #include "vlhip_runtime.h"
 #include "vlhip_device.h"
 #include "vlhip_kernels.h"
@@ -29,21 +26,9 @@ constexpr int FA_ROWS = 64;   // descriptors per chunk
 constexpr int FA_COMPS = 64;  // components per workgroup
 constexpr int FA_PAD = 4;
 
-// LDS layout of fisher_accum_kernel (in elements of T):
-//   xs [D][FA_ROWS + FA_PAD]      descriptors of the chunk, transposed
-//   ms [D][FA_COMPS + FA_PAD]     means of the block, transposed   (RESIDENT)
-//   ss [D][FA_COMPS + FA_PAD]     1/sigma^2 of the block, transposed (RESIDENT)
-//   qs [FA_ROWS][FA_COMPS + FA_PAD] thresholded posteriors
-//   rd [16][FA_COMPS]             per-thread-row partial posterior mass
-inline size_t fisher_accum_lds_elems(int D, bool resident) {
-  size_t e = (size_t)D * (FA_ROWS + FA_PAD) + (size_t)FA_ROWS * (FA_COMPS + FA_PAD) + 16 * FA_COMPS;
-  if (resident) e += 2 * (size_t)D * (FA_COMPS + FA_PAD);
-  return e;
-}
-
 // XCD-aware block mapping: the hardware deals workgroups to the 8 XCDs round
 // robin, so all component blocks of one image are placed on the same XCD and
-// share its L2 copy of the image's descriptors.
+// share its L2 copy of the image's descriptors and posteriors.
 __device__ __forceinline__ void fisher_block_coords(int numKB, int numImages, int& img, int& kb) {
   const int b = blockIdx.x;
   if (numImages >= 8) {
@@ -56,55 +41,47 @@ __device__ __forceinline__ void fisher_block_coords(int numKB, int numImages, in
   }
 }
 
-template <typename T, int DPT, bool RESIDENT>
+// Posterior-weighted accumulation, fisher.c:419-458.  The normalised
+// posteriors q (rows of this image group x K) come from the E-step kernel.
+// Workgroup (image, 64 components): per 64-row chunk the descriptors
+// (transposed) and the thresholded posteriors are staged in LDS; thread
+// (kg, dg) owns components kg*4.. and dimensions dg + 16 j and accumulates
+//   u[k,d] = sum_i q_ik t_ikd,  v[k,d] = sum_i q_ik t_ikd^2 - sum_i q_ik,
+//   t_ikd = (x_id - mu_kd) * sigma_kd^-1
+// in registers, following the reference's per-component loop over data in
+// index order.  Posteriors below 1e-6 and components with prior below 1e-6
+// are skipped and counted as in fisher.c:436-441; with FAST the posterior is
+// the one-hot argmax of the E-step (fisher.c:399-417).
+template <typename T, int DPT>
 __global__ __launch_bounds__(256) void fisher_accum_kernel(
-    const T* __restrict__ x, const uint64_t* __restrict__ offsets, int numImages, int D, int K,
-    const T* __restrict__ means, const T* __restrict__ inv, const T* __restrict__ isig,
-    const double* __restrict__ cterm, const T* __restrict__ priors,
-    const T* __restrict__ row_m, const T* __restrict__ row_s, const int* __restrict__ row_arg,
-    int flags, T* __restrict__ enc, double* __restrict__ normpart,
-    unsigned long long* __restrict__ numTerms, int numKB) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  int img, kb;
-  fisher_block_coords(numKB, numImages, img, kb);
-  if (img >= numImages) return;
-
-  const int XS = FA_ROWS + FA_PAD, CS = FA_COMPS + FA_PAD;
-  T* xs = reinterpret_cast<T*>(smem_raw);
-  T* qs = xs + (size_t)D * XS;
-  T* rd = qs + (size_t)FA_ROWS * CS;
-  T* ms = rd + 16 * FA_COMPS;
-  T* ss = ms + (size_t)D * CS;
+    const T* __restrict__ x, const uint64_t* __restrict__ offsets, int imgBegin, int numImages,
+    long long qRowBase, int D, int K, const T* __restrict__ means, const T* __restrict__ isig,
+    const T* __restrict__ priors, const T* __restrict__ q, const int* __restrict__ row_arg, int flags,
+    T* __restrict__ enc, double* __restrict__ normpart, unsigned long long* __restrict__ numTerms, int numKB,
+    int chunk) {
+  __shared__ __align__(16) T qs[FA_ROWS][FA_COMPS + FA_PAD];
   __shared__ double red[256];
+  __shared__ T s0sh[4][FA_COMPS];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* xs = reinterpret_cast<T*>(smem_raw);  // [D][chunk + FA_PAD]
+  const int XS = chunk + FA_PAD;
+
+  int li, kb;
+  fisher_block_coords(numKB, numImages, li, kb);
+  if (li >= numImages) return;
+  const int img = imgBegin + li;
 
   const int tid = threadIdx.x;
-  const int tx = tid & 15, ty = tid >> 4;  // phase A: rows ty*4.., comps tx*4..
-  const int kg = tid >> 4, dg = tid & 15;  // phase B: comps kg*4.., dims dg + 16 j
+  const int sc = tid & 63, sr = tid >> 6;  // staging: component sc, rows sr + 4 j
+  const int kg = tid >> 4, dg = tid & 15;  // accumulation: comps kg*4.., dims dg + 16 j
   const int k0 = kb * FA_COMPS;
   const long long rBegin = (long long)offsets[img];
   const long long rEnd = (long long)offsets[img + 1];
   const long long nImg = rEnd - rBegin;
   const bool fast = (flags & 4) != 0;
+  const int ks = k0 + sc;
+  const bool ksok = ks < K && (double)priors[ks] >= 1e-6;
 
-  if (RESIDENT) {
-    for (int idx = tid; idx < FA_COMPS * D; idx += 256) {
-      const int c = idx / D, d = idx - c * D;
-      const int k = k0 + c;
-      ms[d * CS + c] = (k < K) ? means[(size_t)k * D + d] : T(0);
-      ss[d * CS + c] = (k < K) ? inv[(size_t)k * D + d] : T(0);
-    }
-  }
-
-  // phase-A constants of this thread's components
-  double ct[4];
-  bool kok[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int k = k0 + tx * 4 + j;
-    kok[j] = (k < K) && ((double)priors[k] >= 1e-6);
-    ct[j] = (k < K) ? cterm[k] : 0.0;
-  }
-  // phase-B constants of this thread's (component, dimension) pairs
   T mu[4][DPT], sg[4][DPT], u[4][DPT], v[4][DPT];
 #pragma unroll
   for (int c = 0; c < 4; ++c)
@@ -117,77 +94,32 @@ __global__ __launch_bounds__(256) void fisher_accum_kernel(
       u[c][j] = T(0);
       v[c][j] = T(0);
     }
-  T s0acc[4] = {T(0), T(0), T(0), T(0)};
-  unsigned long long terms = 0;
+  T s0acc = T(0);
+  unsigned terms = 0;
 
-  for (long long r0 = rBegin; r0 < rEnd; r0 += FA_ROWS) {
-    const int nr = (int)min((long long)FA_ROWS, rEnd - r0);
+  for (long long r0 = rBegin; r0 < rEnd; r0 += chunk) {
+    const int nr = (int)min((long long)chunk, rEnd - r0);
     __syncthreads();
-    for (int idx = tid; idx < FA_ROWS * D; idx += 256) {
+    for (int idx = tid; idx < chunk * D; idx += 256) {
       const int r = idx / D, d = idx - r * D;
       xs[d * XS + r] = (r < nr) ? x[(r0 + r) * D + d] : T(0);
     }
-    __syncthreads();
-
-    // ---- phase A: logits of FA_ROWS x FA_COMPS, posteriors into LDS
-    T acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
-    for (int d = 0; d < D; ++d) {
-      T xv[4], mv[4], sv[4];
-      lds4(&xs[d * XS + ty * 4], xv);
-      if (RESIDENT) {
-        lds4(&ms[d * CS + tx * 4], mv);
-        lds4(&ss[d * CS + tx * 4], sv);
-      } else {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int k = min(k0 + tx * 4 + j, K - 1);
-          mv[j] = means[(size_t)k * D + d];
-          sv[j] = inv[(size_t)k * D + d];
-        }
+    for (int j = 0; j < chunk / 4; ++j) {
+      const int r = sr + 4 * j;
+      T p = T(0);
+      if (r < nr && ksok) {
+        const long long row = r0 + r;
+        if (fast) p = (ks == row_arg[row]) ? T(1) : T(0);
+        else p = q[(row - qRowBase) * K + ks];
+        if ((double)p < 1e-6) p = T(0); else terms += 1;
       }
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const T t = xv[i] - mv[j];
-          acc[i][j] = fma(t * t, sv[j], acc[i][j]);
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int r = ty * 4 + i;
-      const bool rok = r < nr;
-      const long long row = r0 + r;
-      const T M = rok ? row_m[row] : T(0);
-      const T S = rok ? row_s[row] : T(1);
-      const int arg = rok ? row_arg[row] : -1;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int k = k0 + tx * 4 + j;
-        T p = T(0);
-        if (rok && kok[j]) {
-          if (fast) {
-            p = (k == arg) ? T(1) : T(0);
-          } else {
-            const T l = (T)(ct[j] - 0.5 * (double)acc[i][j]);
-            p = dev_exp<T>(l - M) / S;
-          }
-          if ((double)p < 1e-6) p = T(0); else terms += 1;
-        }
-        qs[r * CS + tx * 4 + j] = p;
-        s0acc[j] += p;
-      }
+      qs[r][sc] = p;
+      s0acc += p;
     }
     __syncthreads();
-
-    // ---- phase B: posterior-weighted first and second order residuals
     for (int r = 0; r < nr; ++r) {
       T qv[4];
-      lds4(&qs[r * CS + kg * 4], qv);
+      lds4(&qs[r][kg * 4], qv);
       if (qv[0] == T(0) && qv[1] == T(0) && qv[2] == T(0) && qv[3] == T(0)) continue;
       T xv[DPT];
 #pragma unroll
@@ -209,23 +141,20 @@ __global__ __launch_bounds__(256) void fisher_accum_kernel(
     }
   }
 
-  // ---- per-component posterior mass (sum over the 16 phase-A row groups)
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < 4; ++j) rd[ty * FA_COMPS + tx * 4 + j] = s0acc[j];
+  // posterior mass per component: the 4 staging threads of each component
+  s0sh[sr][sc] = s0acc;
   __syncthreads();
 
-  // ---- finalize, fisher.c:450-472
   const bool sqrtFlag = (flags & 1) != 0;
   double nrm = 0.0;
   T* encU = enc + (size_t)img * 2 * K * D;
   T* encV = encU + (size_t)K * D;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    const int k = k0 + kg * 4 + c;
+    const int cl = kg * 4 + c;
+    const int k = k0 + cl;
     if (k >= K) continue;
-    T s0 = T(0);
-    for (int t = 0; t < 16; ++t) s0 += rd[t * FA_COMPS + kg * 4 + c];
+    const T s0 = ((s0sh[0][cl] + s0sh[1][cl]) + s0sh[2][cl]) + s0sh[3][cl];
     const double prior = (double)priors[k];
     const bool skip = prior < 1e-6;
     T up = T(1), vp = T(1);
@@ -280,63 +209,61 @@ void launch_fisher_normalize(float* enc, const double* normpart, int numKB, size
                normpart, numKB, len);
 }
 
-template <typename T, int DPT, bool RESIDENT>
-static void launch_fisher_accum(const T* x, const uint64_t* offsets, int numImages, int D, int K, const T* means,
-                                const GmmPrep<T>& prep, const T* priors, const T* rm, const T* rs, const int* ra,
-                                int flags, T* enc, double* normpart, unsigned long long* numTerms, int numKB) {
-  const size_t shmem = fisher_accum_lds_elems(D, RESIDENT) * sizeof(T);
-  auto kern = fisher_accum_kernel<T, DPT, RESIDENT>;
+constexpr size_t kAccumLds = 96 * 1024;
+
+template <typename T, int DPT>
+static void launch_fisher_accum(const T* x, const uint64_t* offsets, int imgBegin, int numImages, long long qRowBase,
+                                int D, int K, const T* means, const GmmPrep<T>& prep, const T* priors, const T* q,
+                                const int* ra, int flags, T* enc, double* normpart, unsigned long long* numTerms,
+                                int numKB) {
+  // rows per chunk: 64, fewer when the transposed descriptor tile would exceed 96 KiB of LDS
+  int chunk = FA_ROWS;
+  while (chunk > 4 && (size_t)D * (chunk + FA_PAD) * sizeof(T) > kAccumLds) chunk -= 4;
+  const size_t shmem = (size_t)D * (chunk + FA_PAD) * sizeof(T);
+  auto kern = fisher_accum_kernel<T, DPT>;
   VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
   const int padImages = numImages >= 8 ? ((numImages + 7) / 8) * 8 : numImages;
   const unsigned blocks = (unsigned)((size_t)padImages * numKB);
-  VLHIP_LAUNCH("fisher_accum", kern, dim3(blocks), dim3(256), shmem, x, offsets, numImages, D, K, means,
-               prep.inv, prep.isig, prep.cterm, priors, rm, rs, ra, flags, enc, normpart, numTerms, numKB);
+  VLHIP_LAUNCH("fisher_accum", kern, dim3(blocks), dim3(256), shmem, x, offsets, imgBegin, numImages, qRowBase, D, K,
+               means, prep.isig, priors, q, ra, flags, enc, normpart, numTerms, numKB, chunk);
 }
 
-template <typename T, int DPT>
-static void dispatch_resident(bool resident, const T* x, const uint64_t* offsets, int numImages, int D, int K,
-                              const T* means, const GmmPrep<T>& prep, const T* priors, const T* rm, const T* rs,
-                              const int* ra, int flags, T* enc, double* normpart, unsigned long long* numTerms,
-                              int numKB) {
-  if (resident)
-    launch_fisher_accum<T, DPT, true>(x, offsets, numImages, D, K, means, prep, priors, rm, rs, ra, flags, enc,
-                                      normpart, numTerms, numKB);
-  else
-    launch_fisher_accum<T, DPT, false>(x, offsets, numImages, D, K, means, prep, priors, rm, rs, ra, flags, enc,
-                                       normpart, numTerms, numKB);
-}
-
-constexpr size_t kMaxLds = 160 * 1024;
+// Rows of posteriors materialised at once (bounds the q buffer to 4 GiB of FP32).
+constexpr size_t kPosteriorBudget = size_t(4) << 30;
 
 template <typename T>
 void fisher_fp_t(T* enc, const T* means, const T* covs, const T* priors, size_t D, size_t K, const T* x,
                  const uint64_t* offsets, size_t numImages, int flags, unsigned long long* numTerms) {
   VLHIP_REQUIRE(K >= 1 && D >= 1, "numClusters and dimension must be positive");
+  VLHIP_REQUIRE(D <= 256, "GPU Fisher encoder supports dimension <= 256");
   if (numImages == 0) return;
-  uint64_t total = 0;
-  download(&total, offsets + numImages, sizeof(uint64_t));
+  std::vector<uint64_t> off(numImages + 1);
+  download(off.data(), offsets, (numImages + 1) * sizeof(uint64_t));
+  const uint64_t total = off[numImages];
   GmmPrep<T> prep = gmm_prepare<T>(priors, covs, D, K);
-  unsigned char* rsb = (unsigned char*)scratch(SLOT_ROWSTAT, (size_t)(total + 1) * (2 * sizeof(T) + sizeof(int)) + 512);
-  T* rm = (T*)rsb;
-  T* rs = rm + total + 1;
-  int* ra = (int*)(rs + total + 1);
-  run_estep<T>(x, total, D, K, means, prep, nullptr, rm, rs, ra, "fisher_rowstats", false);
-
+  int* ra = (int*)scratch(SLOT_ROWSTAT, (size_t)(total + 1) * sizeof(int) + 512);
   const int numKB = (int)((K + FA_COMPS - 1) / FA_COMPS);
   double* normpart = (double*)scratch(SLOT_PARTIAL, numImages * (size_t)numKB * sizeof(double));
   if (numTerms) VLHIP_CHECK(hipMemsetAsync(numTerms, 0, numImages * sizeof(unsigned long long), ctx().stream));
-  const bool resident = fisher_accum_lds_elems((int)D, true) * sizeof(T) + 2048 <= kMaxLds;
-  VLHIP_REQUIRE(fisher_accum_lds_elems((int)D, false) * sizeof(T) + 2048 <= kMaxLds,
-                "dimension too large for the GPU Fisher encoder's LDS tiles");
-  const int d = (int)D, k = (int)K, n = (int)numImages;
-  if (D <= 16)       dispatch_resident<T, 1>(resident, x, offsets, n, d, k, means, prep, priors, rm, rs, ra, flags, enc, normpart, numTerms, numKB);
-  else if (D <= 32)  dispatch_resident<T, 2>(resident, x, offsets, n, d, k, means, prep, priors, rm, rs, ra, flags, enc, normpart, numTerms, numKB);
-  else if (D <= 64)  dispatch_resident<T, 4>(resident, x, offsets, n, d, k, means, prep, priors, rm, rs, ra, flags, enc, normpart, numTerms, numKB);
-  else if (D <= 80)  dispatch_resident<T, 5>(resident, x, offsets, n, d, k, means, prep, priors, rm, rs, ra, flags, enc, normpart, numTerms, numKB);
-  else if (D <= 128) dispatch_resident<T, 8>(resident, x, offsets, n, d, k, means, prep, priors, rm, rs, ra, flags, enc, normpart, numTerms, numKB);
-  else if (D <= 256) dispatch_resident<T, 16>(resident, x, offsets, n, d, k, means, prep, priors, rm, rs, ra, flags, enc, normpart, numTerms, numKB);
-  else VLHIP_REQUIRE(false, "GPU Fisher encoder supports dimension <= 256");
-
+  const int d = (int)D, k = (int)K;
+  // image groups whose posterior matrix fits the budget (at least one image)
+  size_t g0 = 0;
+  while (g0 < numImages) {
+    size_t g1 = g0 + 1;
+    while (g1 < numImages && (off[g1 + 1] - off[g0]) * K * sizeof(T) <= kPosteriorBudget) ++g1;
+    const uint64_t rows = off[g1] - off[g0];
+    T* q = (T*)scratch(SLOT_FISHER_Q, (size_t)(rows ? rows : 1) * K * sizeof(T));
+    run_estep<T>(x + off[g0] * D, rows, D, K, means, prep, q, nullptr, nullptr, ra + off[g0], "fisher_estep", false);
+    const int n = (int)(g1 - g0), b = (int)g0;
+    const long long qb = (long long)off[g0];
+    if (D <= 16)       launch_fisher_accum<T, 1>(x, offsets, b, n, qb, d, k, means, prep, priors, q, ra, flags, enc, normpart, numTerms, numKB);
+    else if (D <= 32)  launch_fisher_accum<T, 2>(x, offsets, b, n, qb, d, k, means, prep, priors, q, ra, flags, enc, normpart, numTerms, numKB);
+    else if (D <= 64)  launch_fisher_accum<T, 4>(x, offsets, b, n, qb, d, k, means, prep, priors, q, ra, flags, enc, normpart, numTerms, numKB);
+    else if (D <= 80)  launch_fisher_accum<T, 5>(x, offsets, b, n, qb, d, k, means, prep, priors, q, ra, flags, enc, normpart, numTerms, numKB);
+    else if (D <= 128) launch_fisher_accum<T, 8>(x, offsets, b, n, qb, d, k, means, prep, priors, q, ra, flags, enc, normpart, numTerms, numKB);
+    else               launch_fisher_accum<T, 16>(x, offsets, b, n, qb, d, k, means, prep, priors, q, ra, flags, enc, normpart, numTerms, numKB);
+    g0 = g1;
+  }
   if (flags & 2) {
     VLHIP_LAUNCH("fisher_normalize", (fisher_normalize_kernel<T>), dim3((unsigned)numImages), dim3(256), 0, enc,
                  normpart, numKB, (size_t)2 * K * D);

@@ -44,6 +44,7 @@ enum Slot {
   SLOT_MISC2,
   SLOT_GATE,          // learned-gate activations / candidate lists
   SLOT_GATE2,
+  SLOT_FISHER_Q,      // posteriors of one image group inside the FP32 Fisher encoder
   SLOT_COUNT
 };
 
