@@ -252,7 +252,7 @@ void fisher_fp_t(T* enc, const T* means, const T* covs, const T* priors, size_t 
     size_t g1 = g0 + 1;
     while (g1 < numImages && (off[g1 + 1] - off[g0]) * K * sizeof(T) <= kPosteriorBudget) ++g1;
     const uint64_t rows = off[g1] - off[g0];
-    T* q = (T*)scratch(SLOT_FISHER_Q, (size_t)(rows ? rows : 1) * K * sizeof(T));
+    T* q = (T*)scratch(SLOT_FISHER_WORK, (size_t)(rows ? rows : 1) * K * sizeof(T));
     run_estep<T>(x + off[g0] * D, rows, D, K, means, prep, q, nullptr, nullptr, ra + off[g0], "fisher_estep", false);
     const int n = (int)(g1 - g0), b = (int)g0;
     const long long qb = (long long)off[g0];

@@ -19,9 +19,10 @@
 // scaled by powers of two chosen from the mixture's range so that hi and lo
 // parts stay normal.
 //
-// Kernels: mfma_prep (tables), mfma_lse (row max / sum / argmax over all
-// components, 128 rows per workgroup), mfma_fisher (workgroup per image x 64
-// components: logits -> posteriors -> moments -> finalised slice of the FV).
+// Kernels: mfma_center / mfma_tables (scales and split weights),
+// mfma_lse (first pass: split operand records and row statistics per group of
+// 32 descriptors), mfma_fisher (second pass: workgroup per image x component
+// block: logits -> posteriors -> moments -> finalised slice of the FV).
 #include "vlhip_runtime.h"
 #include "vlhip_device.h"
 #include "vlhip_kernels.h"
@@ -30,6 +31,7 @@
 #include <algorithm>
 #include <cmath>
 #include <type_traits>
+#include <vector>
 
 namespace vlhip {
 
@@ -48,7 +50,7 @@ __device__ __forceinline__ f32x4 mfma16(const bf16x8& a, const bf16x8& b, const 
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-constexpr float kLog2e = 1.4426950408889634f;
+constexpr double kLog2eD = 1.4426950408889634;  // log2(e)
 
 // Power-of-two operand scales and the feature centre (device resident).
 struct MfmaScales {
@@ -146,8 +148,8 @@ __global__ void mfma_tables_kernel(const T* __restrict__ means, const T* __restr
       const double m = (double)means[(size_t)k * D + d] - (double)g[d];
       logdet += log(c);
       quad += s * m * m;
-      w2 = (float)(-0.5 * s) * e2;
-      w1 = (float)(s * m) * e1;
+      w2 = (float)(-0.5 * s * kLog2eD) * e2;
+      w1 = (float)(s * m * kLog2eD) * e1;
       mu = (float)m;
       is = (float)sqrt(s);
       si = (float)s;
@@ -163,7 +165,7 @@ __global__ void mfma_tables_kernel(const T* __restrict__ means, const T* __restr
   }
   const double p = (double)priors[k];
   const double logw = (p < 1e-6) ? -(double)INFINITY : log(p);
-  bias[k] = (float)(logw - 0.5 * (double)D * kLog2Pi - 0.5 * logdet - 0.5 * quad);
+  bias[k] = (float)((logw - 0.5 * (double)D * kLog2Pi - 0.5 * logdet - 0.5 * quad) * kLog2eD);
 }
 
 // Split v into two 16-bit parts.
@@ -190,27 +192,94 @@ __device__ __forceinline__ void make_frag(const float (&xc)[8], bool square, flo
 }
 
 // ---------------------------------------------------------------------------
-// Row statistics over all components (first pass).  Workgroup: 4 waves x 32
-// rows; per 64-component block the split weights are staged in LDS and every
-// wave runs 2 row tiles x 4 component tiles x NS steps x 3 products.
-constexpr int ML_ROWS = 128;
+// Group records.  Each image's descriptors are processed in groups of GR = 32
+// consecutive rows starting at the image's first row (the last group of an
+// image is padded), so an image's result does not depend on the other images
+// of the batch.  Image i owns groups gstart[i] .. gstart[i+1]-1.  The first
+// pass writes,
+// per group, the split 16-bit operands in the register layout of the second
+// pass's matrix instructions, and the row statistics:
+//   A part  [t 2][j NS][hl 2][lane 64] x 8 halves: logit A fragments; lane l
+//           holds row 16t + (l & 15), features 32j + 8(l >> 4) .. +7 of
+//           [x'^2 s2 | x' s1], hi or lo part;
+//   B part  [p 4][ft NS][lane 64] x 8 halves: moment B fragments; plane p of
+//           (x' s1 hi, x' s1 lo, x'^2 s2 hi, x'^2 s2 lo), lane l holds feature
+//           16ft + (l & 15) of rows rho(l >> 4, v), v = 0..7, where
+//           rho(lg, v) = 16(v >> 2) + 4 lg + (v & 3);
+//   stats   [32] x {max logit, 1 / sum_k 2^(logit_k - max), argmax, 0}.
+// Logits are in log2 units (the weight tables carry the factor log2 e).
+// rho is the order in which a lane holds rows of the logit accumulators
+// (C layout: rows 4(l >> 4) + r of each 16-row tile), so the posteriors become
+// A fragments of the moment contraction without leaving registers; the sum
+// over rows does not depend on the order, only on A and B agreeing on it.
+constexpr int GR = 32;
 constexpr int MB_COMPS = 64;
+constexpr int ML_GROUPS = 4;  // groups (waves) per first-pass workgroup
 
+template <int NS>
+struct GroupRec {
+  static constexpr size_t kA = (size_t)2 * NS * 2 * 64 * 16;
+  static constexpr size_t kB = (size_t)4 * NS * 64 * 16;
+  static constexpr size_t kStats = (size_t)GR * 16;
+  static constexpr size_t kBytes = kA + kB + kStats;
+};
+
+// Deterministic block sum for WAVES x 64 threads (fixed order).
+template <int WAVES>
+__device__ __forceinline__ double block_sum_waves(double v, double* red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double r = 0.0;
+#pragma unroll
+  for (int i = 0; i < WAVES; ++i) r += red[i];
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// First pass.  Workgroup: 4 waves, wave = one group of 32 rows.  The wave
+// builds its A fragments from the fp32 descriptors, stores them and the B
+// fragments in the group record, then sweeps all components (weights staged
+// in LDS per 64 components; 2 row tiles x 4 component tiles x NS steps x 3
+// products per block) keeping a lane-local online log-sum-exp, merged across
+// the 16 lanes that share rows at the end.
 template <typename H, int NS>
 __global__ __launch_bounds__(256, 2) void mfma_lse_kernel(
     const float* __restrict__ x, long long N, int D, int Kp,
     const H* __restrict__ Whi, const H* __restrict__ Wlo, const float* __restrict__ bias,
     const MfmaScales* __restrict__ scp, const float* __restrict__ g,
-    float* __restrict__ row_m, float* __restrict__ row_s, int* __restrict__ row_arg) {
+    unsigned char* __restrict__ rec, long long numGroups, const uint64_t* __restrict__ offsets,
+    int imgBegin, int numImages, const long long* __restrict__ gstart) {
   typedef typename VecOf<H>::type V8;
+  typedef GroupRec<NS> R;
   constexpr int Dp = NS * 16;
   constexpr int WS = 2 * Dp + 8;  // LDS row stride (halves) of the staged weights
   __shared__ __align__(16) H wsh[2][MB_COMPS][WS];
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int lr = lane & 15, lg = lane >> 4;
-  const long long row0 = (long long)blockIdx.x * ML_ROWS + 32 * w;
+  const long long grp = (long long)blockIdx.x * ML_GROUPS + w;  // relative to the batch
+  const bool live = grp < numGroups;  // uniform per wave; dead waves still take part in barriers
+  // owning image: the last i with gstart[i] <= group (empty images share their start)
+  long long row0 = 0, rowEnd = 0;  // batch-relative rows of this group
+  if (live) {
+    const long long gg = gstart[imgBegin] + grp;
+    int lo = imgBegin, hi = imgBegin + numImages;  // gstart[lo] <= gg < gstart[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (gstart[mid] <= gg) lo = mid; else hi = mid;
+    }
+    const long long rowBase = (long long)offsets[imgBegin];
+    row0 = (long long)offsets[lo] - rowBase + (gg - gstart[lo]) * GR;
+    rowEnd = (long long)offsets[lo + 1] - rowBase;
+  }
   const float s1 = scp->s1, s2 = scp->s2;
+  V8* Ap = reinterpret_cast<V8*>(rec + (size_t)grp * R::kBytes);
+  V8* Bp = reinterpret_cast<V8*>(rec + (size_t)grp * R::kBytes + R::kA);
+  float4* St = reinterpret_cast<float4*>(rec + (size_t)grp * R::kBytes + R::kA + R::kB);
 
   // A fragments of this wave's 32 rows, resident for the whole sweep
   V8 ahi[2][NS], alo[2][NS];
@@ -226,9 +295,38 @@ __global__ __launch_bounds__(256, 2) void mfma_lse_kernel(
 #pragma unroll
       for (int v = 0; v < 8; ++v) {
         const int d = d0 + v;
-        xc[v] = (row < N && d < D) ? x[row * D + d] - g[d] : 0.f;
+        xc[v] = (row < rowEnd && d < D) ? x[row * D + d] - g[d] : 0.f;
       }
       make_frag<H>(xc, sq, s1, s2, ahi[t][j], alo[t][j]);
+      if (live) {
+        Ap[((t * NS + j) * 2 + 0) * 64 + lane] = ahi[t][j];
+        Ap[((t * NS + j) * 2 + 1) * 64 + lane] = alo[t][j];
+      }
+    }
+  }
+  // B fragments: feature 16ft + lr of rows rho(lg, v)
+  if (live) {
+#pragma unroll
+    for (int ft = 0; ft < NS; ++ft) {
+      const int d = 16 * ft + lr;
+      const float gd = d < D ? g[d] : 0.f;
+      V8 b1h, b1l, b2h, b2l;
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        const long long row = row0 + 16 * (v >> 2) + 4 * lg + (v & 3);
+        const float xc = (row < rowEnd && d < D) ? x[row * D + d] - gd : 0.f;
+        H h, l;
+        split16<H>(xc * s1, h, l);
+        b1h[v] = h;
+        b1l[v] = l;
+        split16<H>(xc * xc * s2, h, l);
+        b2h[v] = h;
+        b2l[v] = l;
+      }
+      Bp[(0 * NS + ft) * 64 + lane] = b1h;
+      Bp[(1 * NS + ft) * 64 + lane] = b1l;
+      Bp[(2 * NS + ft) * 64 + lane] = b2h;
+      Bp[(3 * NS + ft) * 64 + lane] = b2l;
     }
   }
 
@@ -275,25 +373,27 @@ __global__ __launch_bounds__(256, 2) void mfma_lse_kernel(
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc4[c][t] = mfma16(ahi[t][j], bl[c], acc4[c][t]);
     }
+    float bc[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const f32x4* acc = acc4[c];
-      const int comp = cb + 16 * c + lr;
-      const float b = bias[comp];
+    for (int c = 0; c < 4; ++c) bc[c] = bias[cb + 16 * c + lr];
 #pragma unroll
-      for (int t = 0; t < 2; ++t)
+    for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const float l = acc[t][r] + b;
-          if (l > bval[t][r]) { bval[t][r] = l; bidx[t][r] = comp; }
-          const float mn = l > mrun[t][r] ? l : mrun[t][r];
-          if (mn != -INFINITY) {
-            srun[t][r] = srun[t][r] * __builtin_amdgcn_exp2f((mrun[t][r] - mn) * kLog2e)
-                       + __builtin_amdgcn_exp2f((l - mn) * kLog2e);
-            mrun[t][r] = mn;
-          }
+      for (int r = 0; r < 4; ++r) {
+        float lv[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          lv[c] = acc4[c][t][r] + bc[c];
+          if (lv[c] > bval[t][r]) { bval[t][r] = lv[c]; bidx[t][r] = cb + 16 * c + lr; }
         }
-    }
+        const float mx = fmaxf(fmaxf(lv[0], lv[1]), fmaxf(lv[2], lv[3]));
+        const float mn = fmaxf(mrun[t][r], mx);
+        const float ms = (mn == -INFINITY) ? 0.f : mn;
+        srun[t][r] = srun[t][r] * __builtin_amdgcn_exp2f(mrun[t][r] - ms)
+                   + ((__builtin_amdgcn_exp2f(lv[0] - ms) + __builtin_amdgcn_exp2f(lv[1] - ms))
+                   + (__builtin_amdgcn_exp2f(lv[2] - ms) + __builtin_amdgcn_exp2f(lv[3] - ms)));
+        mrun[t][r] = mn;
+      }
   }
 
   // merge the 16 lanes that hold the same rows (lanes lg*16 .. lg*16+15)
@@ -304,7 +404,8 @@ __global__ __launch_bounds__(256, 2) void mfma_lse_kernel(
       float m = mrun[t][r];
 #pragma unroll
       for (int o = 1; o < 16; o <<= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-      float s = (mrun[t][r] == -INFINITY) ? 0.f : srun[t][r] * __builtin_amdgcn_exp2f((mrun[t][r] - m) * kLog2e);
+      const float ms = (m == -INFINITY) ? 0.f : m;
+      float s = srun[t][r] * __builtin_amdgcn_exp2f(mrun[t][r] - ms);
 #pragma unroll
       for (int o = 1; o < 16; o <<= 1) s += __shfl_xor(s, o, 64);
       float bv = bval[t][r];
@@ -315,48 +416,13 @@ __global__ __launch_bounds__(256, 2) void mfma_lse_kernel(
         const int oi = __shfl_xor(bi, o, 64);
         if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
       }
-      const long long row = row0 + 16 * t + 4 * lg + r;
-      if (lr == 0 && row < N) {
-        row_m[row] = m;
-        row_s[row] = s;
-        row_arg[row] = bi;
-      }
+      if (live && lr == 0) St[16 * t + 4 * lg + r] = make_float4(m, 1.0f / s, __int_as_float(bi), 0.f);
     }
 }
 
-// ---------------------------------------------------------------------------
-// Moments and encoding (second pass).  Workgroup = (image, 128 components),
-// 8 waves.  Per 64-row chunk:
-//   stage   x' (fp32, row-major) and the split [x'_hi, x'_lo, x'^2_hi, x'^2_lo]
-//           (16-bit, feature-major) in LDS, once for all 128 components;
-//   logits  wave (wr, wc) = rows 32wr.., comps 32wc.. (weights in registers);
-//   q       exp(logit - M) / S, FAST one-hot, < 1e-6 dropped (fisher.c:436-441),
-//           rounded to 16 bits (scaled), written component-major to LDS;
-//   moments wave (plane, half) accumulates one feature plane (x'_hi, x'_lo,
-//           x'^2_hi or x'^2_lo) for 64 components over the chunk's rows.
-constexpr int MF_ROWS = 64;
-constexpr int MF_COMPS = 128;
-
-template <typename H, int NS>
-struct MfLds {
-  static constexpr int Dp = NS * 16;
-  static constexpr int RS = MF_ROWS + 8;  // row stride (halves) of feature-major tiles
-  static constexpr int RW = 4 * Dp + 8;   // row stride (halves) of row-major tiles
-  // row-major split features [x'^2_hi | x'_hi | x'^2_lo | x'_lo] (logit A operand)
-  H xr[MF_ROWS][RW];
-  // feature-major split features, planes x'_hi, x'_lo, x'^2_hi, x'^2_lo (moment B operand)
-  H xct[4 * Dp][RS];
-  H qt[MF_COMPS][RS];                      // component-major rounded posteriors
-  float rm[MF_ROWS], rinv[MF_ROWS];
-  int ra[MF_ROWS];
-  float s0[2][MF_COMPS];
-  // the finalisation reuses xr, xct and qt (contiguous) as [2][MF_COMPS][Dp] floats
-  static constexpr bool kScratchFits =
-      sizeof(H) * MF_ROWS * RW + sizeof(H) * 4 * Dp * RS + sizeof(H) * MF_COMPS * RS >=
-      sizeof(float) * 2 * MF_COMPS * Dp;
-  static_assert(kScratchFits, "finalisation scratch does not fit");
-};
-
+// XCD-aware block mapping: workgroups are dealt to the 8 XCDs round robin, so
+// all component blocks of one image land on the same XCD and share its L2
+// copy of the image's group records.
 __device__ __forceinline__ void fisher_mfma_coords(int numKB, int numImages, int& img, int& kb) {
   const int b = blockIdx.x;
   if (numImages >= 8) {
@@ -369,270 +435,255 @@ __device__ __forceinline__ void fisher_mfma_coords(int numKB, int numImages, int
   }
 }
 
-template <typename H>
-__device__ __forceinline__ void store4(H* dst, H a, H b, H c, H d) {
-  union { H h[4]; uint2 u; } pk;
-  pk.h[0] = a; pk.h[1] = b; pk.h[2] = c; pk.h[3] = d;
-  *reinterpret_cast<uint2*>(dst) = pk.u;
+// Copy of one group record between HBM, registers and LDS (NT threads, 16 bytes each).
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+template <int PER, int NT, int NV>
+__device__ __forceinline__ void rec_load(u32x4 (&pre)[PER], const unsigned char* src) {
+  const u32x4* s = reinterpret_cast<const u32x4*>(src);
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int i = threadIdx.x + NT * q;
+    pre[q] = s[i < NV ? i : NV - 1];  // unconditional, so that pre stays in registers
+  }
+}
+template <int PER, int NT, int NV>
+__device__ __forceinline__ void rec_store(const u32x4 (&pre)[PER], unsigned char* dst) {
+  u32x4* d = reinterpret_cast<u32x4*>(dst);
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int i = threadIdx.x + NT * q;
+    if (i < NV) d[i] = pre[q];
+  }
 }
 
-template <typename T, typename H, int NS>
-__global__ __launch_bounds__(256, 1) void mfma_fisher_kernel(
-    const float* __restrict__ x, const uint64_t* __restrict__ offsets, int numImages, int D, int K,
-    const H* __restrict__ Whi, const H* __restrict__ Wlo, const float* __restrict__ bias,
-    const MfmaScales* __restrict__ scp, const float* __restrict__ g,
-    const float* __restrict__ mup, const float* __restrict__ isg, const float* __restrict__ sinv,
-    const T* __restrict__ priors, const float* __restrict__ row_m, const float* __restrict__ row_s,
-    const int* __restrict__ row_arg, int flags, T* __restrict__ enc, double* __restrict__ normpart,
-    unsigned long long* __restrict__ numTerms, int numKB) {
+// ---------------------------------------------------------------------------
+// Second pass.  Workgroup = (image, WAVES x 16CT components); wave w owns the
+// CT component tiles 16CT w .. with their split weights in registers and
+// walks the image's groups (records double-buffered in LDS, the next one
+// prefetched into registers):
+//   logits   2 row tiles x CT component tiles x NS steps x 3 products;
+//   q        2^(logit - max) / sum, FAST one-hot, < 1e-6 dropped
+//            (fisher.c:436-441), rows outside the image masked, rounded to
+//            16 bits (scaled) -- the same rounded value enters S0;
+//   moments  q (A, straight from the accumulators) x the 4 planes of B:
+//            S1 += q x'_hi + q x'_lo,  S2 += q x'^2_hi + q x'^2_lo.
+// The finalisation works on the accumulators in registers: lane l holds
+// features 16ft + (l & 15) of components 16c + 4(l >> 4) + r.
+template <typename T, typename H, int NS, int CT, int WAVES>
+__global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
+    const unsigned char* __restrict__ rec, const uint64_t* __restrict__ offsets, const long long* __restrict__ gstart,
+    int imgBegin, int numImages, int D, int K, const H* __restrict__ Whi, const H* __restrict__ Wlo,
+    const float* __restrict__ bias, const MfmaScales* __restrict__ scp, const float* __restrict__ mup,
+    const float* __restrict__ isg, const float* __restrict__ sinv, const T* __restrict__ priors, int flags,
+    T* __restrict__ enc, double* __restrict__ normpart, unsigned long long* __restrict__ numTerms, int numKB) {
   typedef typename VecOf<H>::type V8;
-  typedef MfLds<H, NS> L;
+  typedef GroupRec<NS> R;
   constexpr int Dp = NS * 16;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  L& sm = *reinterpret_cast<L*>(smem_raw);
-  __shared__ double red[256];
+  constexpr int NT = WAVES * 64;
+  constexpr int BC = WAVES * 16 * CT;
+  constexpr int NV = (int)(R::kBytes / 16);  // 16-byte words per record
+  constexpr int PER = (NV + NT - 1) / NT;
+  extern __shared__ __align__(16) unsigned char smem_raw[];  // 2 records
+  __shared__ double red[WAVES];
 
-  int img, kb;
-  fisher_mfma_coords(numKB, numImages, img, kb);
-  if (img >= numImages) return;
+  int li, kb;
+  fisher_mfma_coords(numKB, numImages, li, kb);
+  if (li >= numImages) return;
+  const int img = imgBegin + li;
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int lr = lane & 15, lg = lane >> 4;
-  const int wr = w & 1, wc = w >> 1;  // logits: rows 32wr.., comps 64wc..
-  const int pp = w & 1, ch = w >> 1;  // moments: planes (x'_hi,x'_lo) or (x'^2_hi,x'^2_lo), comps 64ch..
-  const int k0 = kb * MF_COMPS;
-  const long long rBegin = (long long)offsets[img], rEnd = (long long)offsets[img + 1];
-  const long long nImg = rEnd - rBegin;
+  const int kw = kb * BC + 16 * CT * w;  // first component of this wave
+  const long long nImg = (long long)offsets[img + 1] - (long long)offsets[img];
+  const long long gBase = gstart[imgBegin];
+  const long long gB = gstart[img] - gBase, gE = gstart[img + 1] - gBase;  // this image's records
   const bool fast = (flags & 4) != 0;
   const MfmaScales sc = *scp;
 
-  V8 whi[4][NS], wlo[4][NS];
-  float bcomp[4];
+  V8 whi[CT][NS], wlo[CT][NS];
+  float bc[CT];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int comp = k0 + 64 * wc + 16 * c + lr;
-    bcomp[c] = bias[comp];
+  for (int c = 0; c < CT; ++c) {
+    const int comp = kw + 16 * c + lr;
+    bc[c] = bias[comp];
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
       whi[c][j] = *reinterpret_cast<const V8*>(Whi + (size_t)comp * 2 * Dp + 32 * j + 8 * lg);
       wlo[c][j] = *reinterpret_cast<const V8*>(Wlo + (size_t)comp * 2 * Dp + 32 * j + 8 * lg);
     }
   }
-  f32x4 macc[4][NS];  // [comp tile][feature tile]; hi and lo planes accumulate together
+  f32x4 macc[CT][2][NS];  // [comp tile][S1, S2][feature tile]
 #pragma unroll
-  for (int ct = 0; ct < 4; ++ct)
+  for (int c = 0; c < CT; ++c)
 #pragma unroll
-    for (int ft = 0; ft < NS; ++ft) macc[ct][ft] = f32x4{0.f, 0.f, 0.f, 0.f};
-  float s0l[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int ft = 0; ft < NS; ++ft) macc[c][s][ft] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float s0l[CT];
+#pragma unroll
+  for (int c = 0; c < CT; ++c) s0l[c] = 0.f;
   unsigned terms = 0;
   // (double)q < 1e-6 for a float q is q < the next float above float(1e-6)
   const float qmin = __uint_as_float(0x358637beu);  // 1.0000001e-6
 
-  // staging items: (4-row group, dimension); the next chunk's coordinates are
-  // prefetched into registers while the current chunk computes.
-  constexpr int ITEMS = ((MF_ROWS / 4) * Dp + 255) / 256;
-  float xpre[ITEMS][4];
-  auto prefetch = [&](long long c0) {
-    const int n = (int)min((long long)MF_ROWS, rEnd - c0);
-#pragma unroll
-    for (int q = 0; q < ITEMS; ++q) {
-      const int it = tid + 256 * q;
-      const int rq = it / Dp, d = it - rq * Dp;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int r = 4 * rq + i;
-        xpre[q][i] = (it < (MF_ROWS / 4) * Dp && r < n && d < D) ? x[(c0 + r) * D + d] : 0.f;
-      }
-    }
-  };
-  if (rBegin < rEnd) prefetch(rBegin);
+  u32x4 pre[PER];
+  if (gB < gE) {
+    rec_load<PER, NT, NV>(pre, rec + (size_t)gB * R::kBytes);
+    rec_store<PER, NT, NV>(pre, smem_raw);
+    if (gB + 1 < gE) rec_load<PER, NT, NV>(pre, rec + (size_t)(gB + 1) * R::kBytes);
+  }
+  __syncthreads();
 
-  for (long long r0 = rBegin; r0 < rEnd; r0 += MF_ROWS) {
-    const int nr = (int)min((long long)MF_ROWS, rEnd - r0);
-    __syncthreads();
-    // ---- stage the prefetched chunk
-#pragma unroll
-    for (int q = 0; q < ITEMS; ++q) {
-      const int it = tid + 256 * q;
-      if (it < (MF_ROWS / 4) * Dp) {
-        const int rq = it / Dp, d = it - rq * Dp;
-        const float gd = d < D ? g[d] : 0.f;
-        H h1[4], l1[4], h2[4], l2[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int r = 4 * rq + i;
-          const float xc = (r < nr && d < D) ? xpre[q][i] - gd : 0.f;
-          split16<H>(xc * sc.s1, h1[i], l1[i]);
-          split16<H>(xc * xc * sc.s2, h2[i], l2[i]);
-          sm.xr[r][d] = h2[i];
-          sm.xr[r][Dp + d] = h1[i];
-          sm.xr[r][2 * Dp + d] = l2[i];
-          sm.xr[r][3 * Dp + d] = l1[i];
-        }
-        store4<H>(&sm.xct[d][4 * rq], h1[0], h1[1], h1[2], h1[3]);
-        store4<H>(&sm.xct[Dp + d][4 * rq], l1[0], l1[1], l1[2], l1[3]);
-        store4<H>(&sm.xct[2 * Dp + d][4 * rq], h2[0], h2[1], h2[2], h2[3]);
-        store4<H>(&sm.xct[3 * Dp + d][4 * rq], l2[0], l2[1], l2[2], l2[3]);
-      }
+  for (long long gi = gB; gi < gE; ++gi) {
+    const int buf = (int)((gi - gB) & 1);
+    if (gi + 1 < gE) {
+      // every wave finished reading this buffer (group gi - 1) before the last barrier
+      rec_store<PER, NT, NV>(pre, smem_raw + (size_t)(buf ^ 1) * R::kBytes);
+      if (gi + 2 < gE) rec_load<PER, NT, NV>(pre, rec + (size_t)(gi + 2) * R::kBytes);
     }
-    if (tid < MF_ROWS) {
-      const bool ok = tid < nr;
-      sm.rm[tid] = ok ? row_m[r0 + tid] : 0.f;
-      sm.rinv[tid] = ok ? 1.0f / row_s[r0 + tid] : 0.f;
-      sm.ra[tid] = ok ? row_arg[r0 + tid] : -1;
-    }
-    if (r0 + MF_ROWS < rEnd) prefetch(r0 + MF_ROWS);
-    __syncthreads();
+    const unsigned char* cur = smem_raw + (size_t)buf * R::kBytes;
+    const V8* Ap = reinterpret_cast<const V8*>(cur);
+    const V8* Bp = reinterpret_cast<const V8*>(cur + R::kA);
+    const float4* St = reinterpret_cast<const float4*>(cur + R::kA + R::kB);
 
-    // ---- logits of rows 32wr.. x comps 64wc..
-    f32x4 acc[2][4];
+    // ---- logits
+    f32x4 acc[2][CT];
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) acc[t][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int c = 0; c < CT; ++c) acc[t][c] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
-      const int f0 = 32 * j + 8 * lg;
       V8 ah[2], al[2];
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
-        const int rl = 32 * wr + 16 * t + lr;
-        ah[t] = *reinterpret_cast<const V8*>(&sm.xr[rl][f0]);
-        al[t] = *reinterpret_cast<const V8*>(&sm.xr[rl][2 * Dp + f0]);
+        ah[t] = Ap[((t * NS + j) * 2 + 0) * 64 + lane];
+        al[t] = Ap[((t * NS + j) * 2 + 1) * 64 + lane];
       }
 #pragma unroll
       for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[t][c] = mfma16(ah[t], whi[c][j], acc[t][c]);
+        for (int c = 0; c < CT; ++c) acc[t][c] = mfma16(ah[t], whi[c][j], acc[t][c]);
 #pragma unroll
       for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[t][c] = mfma16(al[t], whi[c][j], acc[t][c]);
+        for (int c = 0; c < CT; ++c) acc[t][c] = mfma16(al[t], whi[c][j], acc[t][c]);
 #pragma unroll
       for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[t][c] = mfma16(ah[t], wlo[c][j], acc[t][c]);
+        for (int c = 0; c < CT; ++c) acc[t][c] = mfma16(ah[t], wlo[c][j], acc[t][c]);
     }
-    // ---- posteriors
+
+    // ---- posteriors, straight into A fragments of the moment contraction
+    const long long gr0 = (gi - gB) * GR;  // image-relative first row of the group
+    V8 aq[CT];
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int cl = 64 * wc + 16 * c + lr;
-        const int comp = k0 + cl;
-        H hq[4];
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * t + 4 * lg + r;
+        const float4 st = St[row];
+        const bool ok = gr0 + row < nImg;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int rl = 32 * wr + 16 * t + 4 * lg + r;
+        for (int c = 0; c < CT; ++c) {
           float q = 0.f;
-          if (rl < nr && comp < K && bcomp[c] != -INFINITY) {
+          if (ok) {
             if (fast) {
-              q = (comp == sm.ra[rl]) ? 1.f : 0.f;
+              q = (kw + 16 * c + lr == __float_as_int(st.z) && bc[c] != -INFINITY) ? 1.f : 0.f;
             } else {
-              const float l = acc[t][c][r] + bcomp[c];
-              q = __builtin_amdgcn_exp2f((l - sm.rm[rl]) * kLog2e) * sm.rinv[rl];
+              q = __builtin_amdgcn_exp2f(acc[t][c][r] + bc[c] - st.x) * st.y;
             }
             if (q < qmin) q = 0.f; else terms += 1;
           }
-          hq[r] = H(q * sc.sq);
-          s0l[c] += (float)hq[r];
+          const H h = H(q * sc.sq);
+          s0l[c] += (float)h;
+          aq[c][4 * t + r] = h;
         }
-        store4<H>(&sm.qt[cl][32 * wr + 16 * t + 4 * lg], hq[0], hq[1], hq[2], hq[3]);
+      }
+
+    // ---- moments
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int ft = 0; ft < NS; ++ft) {
+        const V8 bx = Bp[(p * NS + ft) * 64 + lane];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) macc[c][p >> 1][ft] = mfma16(aq[c], bx, macc[c][p >> 1][ft]);
       }
     __syncthreads();
-
-    // ---- moments: planes 2pp, 2pp+1 for components 64ch.., 2 row steps
-#pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
-      V8 aq[4];
-#pragma unroll
-      for (int ct = 0; ct < 4; ++ct)
-        aq[ct] = *reinterpret_cast<const V8*>(&sm.qt[64 * ch + 16 * ct + lr][32 * ks + 8 * lg]);
-#pragma unroll
-      for (int p2 = 0; p2 < 2; ++p2)
-#pragma unroll
-        for (int ft = 0; ft < NS; ++ft) {
-          const V8 bx = *reinterpret_cast<const V8*>(&sm.xct[(2 * pp + p2) * Dp + 16 * ft + lr][32 * ks + 8 * lg]);
-#pragma unroll
-          for (int ct = 0; ct < 4; ++ct) macc[ct][ft] = mfma16(aq[ct], bx, macc[ct][ft]);
-        }
-    }
   }
 
-  // ---- posterior mass per component (lanes lg, then the two row halves)
+  // ---- posterior mass per component: lanes sharing lr hold other rows
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
+  for (int c = 0; c < CT; ++c) {
     float sv = s0l[c];
     sv += __shfl_xor(sv, 16, 64);
     sv += __shfl_xor(sv, 32, 64);
     s0l[c] = sv;
   }
-  __syncthreads();
-  if (lg == 0) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) sm.s0[wr][64 * wc + 16 * c + lr] = s0l[c];
-  }
-  __syncthreads();
 
-  // ---- finalisation: every wave dumps its (plane pair, component half)
-  // moments into LDS as FP32 -- the staging buffers are free now -- and all
-  // threads then finish (component, dimension) pairs.
-  float* F = reinterpret_cast<float*>(&sm.xr[0][0]);  // [2 planes: S1, S2][128 comps][Dp]; spans xr, xct, qt
-#pragma unroll
-  for (int ct = 0; ct < 4; ++ct)
-#pragma unroll
-    for (int ft = 0; ft < NS; ++ft)
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        F[((size_t)pp * MF_COMPS + 64 * ch + 16 * ct + 4 * lg + r) * Dp + 16 * ft + lr] = macc[ct][ft][r];
-  __syncthreads();
+  // ---- finalisation in registers
   const double invq = 1.0 / (double)sc.sq;
+  const double inv1 = invq / (double)sc.s1, inv2 = invq / (double)sc.s2;
   const bool sqrtFlag = (flags & 1) != 0;
   T* encU = enc + (size_t)img * 2 * K * D;
   T* encV = encU + (size_t)K * D;
   double nrm = 0.0;
-#pragma nounroll
-  for (int idx = tid; idx < MF_COMPS * Dp; idx += 256) {
-    const int cl = idx / Dp, d = idx - cl * Dp;
-    const int k = k0 + cl;
-    if (k >= K || d >= D) continue;
-    const double S0 = ((double)sm.s0[0][cl] + (double)sm.s0[1][cl]) * invq;
-    const double S1 = (double)F[(size_t)cl * Dp + d] * invq / (double)sc.s1;
-    const double S2 = (double)F[((size_t)MF_COMPS + cl) * Dp + d] * invq / (double)sc.s2;
-    const double prior = (double)priors[k];
-    T u = T(0), v = T(0);
-    if (prior >= 1e-6) {
-      const double mu = (double)mup[(size_t)k * Dp + d];
-      double uu = (double)isg[(size_t)k * Dp + d] * (S1 - mu * S0);
-      double vv = (double)sinv[(size_t)k * Dp + d] * (S2 - 2.0 * mu * S1 + mu * mu * S0) - S0;
-      if (nImg > 0) {
-        uu *= 1.0 / ((double)nImg * sqrt(prior));
-        vv *= 1.0 / ((double)nImg * sqrt(2.0 * prior));
+#pragma unroll
+  for (int c = 0; c < CT; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      // S0 of component 16c + 4lg + r lives in lane 4lg + r
+      const double S0 = (double)__shfl(s0l[c], 4 * lg + r, 64) * invq;
+      const int k = kw + 16 * c + 4 * lg + r;
+      const double prior = k < K ? (double)priors[k] : 0.0;
+      const bool use = k < K && prior >= 1e-6;
+      double up = 1.0, vp = 1.0;
+      if (nImg > 0 && use) {
+        up = 1.0 / ((double)nImg * sqrt(prior));
+        vp = 1.0 / ((double)nImg * sqrt(2.0 * prior));
       }
-      u = (T)uu;
-      v = (T)vv;
+#pragma unroll
+      for (int ft = 0; ft < NS; ++ft) {
+        const int d = 16 * ft + lr;
+        if (k >= K || d >= D) continue;
+        T u = T(0), v = T(0);
+        if (use) {
+          const double S1 = (double)macc[c][0][ft][r] * inv1;
+          const double S2 = (double)macc[c][1][ft][r] * inv2;
+          const double mu = (double)mup[(size_t)k * Dp + d];
+          const double uu = (double)isg[(size_t)k * Dp + d] * (S1 - mu * S0);
+          const double vv = (double)sinv[(size_t)k * Dp + d] * (S2 - 2.0 * mu * S1 + mu * mu * S0) - S0;
+          u = (T)(uu * up);
+          v = (T)(vv * vp);
+        }
+        if (sqrtFlag) { u = signed_sqrt(u); v = signed_sqrt(v); }
+        encU[(size_t)k * D + d] = u;
+        encV[(size_t)k * D + d] = v;
+        nrm += (double)u * (double)u + (double)v * (double)v;
+      }
     }
-    if (sqrtFlag) { u = signed_sqrt(u); v = signed_sqrt(v); }
-    encU[(size_t)k * D + d] = u;
-    encV[(size_t)k * D + d] = v;
-    nrm += (double)u * (double)u + (double)v * (double)v;
-  }
-  const double bn = block_sum_256(nrm, red);
+  const double bn = block_sum_waves<WAVES>(nrm, red);
   if (tid == 0) normpart[(size_t)img * numKB + kb] = bn;
   if (numTerms) {
-    const double bt = block_sum_256((double)terms, red);
+    const double bt = block_sum_waves<WAVES>((double)terms, red);
     if (tid == 0) atomicAdd(&numTerms[img], (unsigned long long)bt);
   }
 }
 
 // ---------------------------------------------------------------------------
 
-template <typename H, int NS>
+// Bytes of group records materialised at once (batches of whole images).
+constexpr size_t kRecordBudget = size_t(8) << 30;
+
+template <typename H, int NS, int CT, int WAVES>
 static void run_mfma(float* enc, const float* means, const float* covs, const float* priors, size_t D, size_t K,
                      const float* x, const uint64_t* offsets, size_t numImages, int flags,
-                     unsigned long long* numTerms, uint64_t total) {
+                     unsigned long long* numTerms, const std::vector<uint64_t>& off) {
   constexpr int Dp = NS * 16;
-  const int Kp = (int)((K + MF_COMPS - 1) / MF_COMPS) * MF_COMPS;
+  constexpr int BC = WAVES * 16 * CT;
+  typedef GroupRec<NS> R;
+  const int Kp = (int)((K + BC - 1) / BC) * BC;
   const bool isHalf = sizeof(H) == 2 && std::is_same<H, _Float16>::value;
   // tables: [scales][g Dp][bias Kp][mup, isg, sinv Kp*Dp][Whi, Wlo Kp*2Dp]
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
@@ -653,44 +704,67 @@ static void run_mfma(float* enc, const float* means, const float* covs, const fl
   VLHIP_LAUNCH("mfma_tables", (mfma_tables_kernel<float, H>), dim3((Kp + 127) / 128), dim3(128), 0, means, covs,
                priors, (int)D, Dp, (int)K, Kp, g, scp, Whi, Wlo, bias, mup, isg, sinv);
 
-  unsigned char* rsb = (unsigned char*)scratch(SLOT_ROWSTAT, (size_t)(total + 1) * 12 + 512);
-  float* rm = (float*)rsb;
-  float* rs = rm + total + 1;
-  int* ra = (int*)(rs + total + 1);
-  if (total > 0) {
-    const unsigned blocks = (unsigned)((total + ML_ROWS - 1) / ML_ROWS);
-    VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS>), dim3(blocks), dim3(256), 0, x, (long long)total, (int)D,
-                 Kp, Whi, Wlo, bias, scp, g, rm, rs, ra);
-  }
-  const int numKB = Kp / MF_COMPS;
+  const int numKB = Kp / BC;
   double* normpart = (double*)scratch(SLOT_PARTIAL, numImages * (size_t)numKB * sizeof(double));
   if (numTerms) VLHIP_CHECK(hipMemsetAsync(numTerms, 0, numImages * sizeof(unsigned long long), ctx().stream));
-  const size_t shmem = sizeof(MfLds<H, NS>);
-  auto kern = mfma_fisher_kernel<float, H, NS>;
+  const size_t shmem = 2 * R::kBytes;
+  auto kern = mfma_fisher_kernel<float, H, NS, CT, WAVES>;
   VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-  const int padImages = numImages >= 8 ? (int)((numImages + 7) / 8) * 8 : (int)numImages;
-  VLHIP_LAUNCH("mfma_fisher", kern, dim3((unsigned)((size_t)padImages * numKB)), dim3(256), shmem, x, offsets,
-               (int)numImages, (int)D, (int)K, Whi, Wlo, bias, scp, g, mup, isg, sinv, priors, rm, rs, ra, flags, enc,
-               normpart, numTerms, numKB);
+  // image-aligned groups: gstart[i] = sum_{j<i} ceil(n_j / GR)
+  std::vector<long long> gs(numImages + 1);
+  gs[0] = 0;
+  for (size_t i = 0; i < numImages; ++i) gs[i + 1] = gs[i] + (long long)((off[i + 1] - off[i] + GR - 1) / GR);
+  long long* gstart = (long long*)scratch(SLOT_GROUPS, (numImages + 1) * sizeof(long long));
+  upload(gstart, gs.data(), (numImages + 1) * sizeof(long long));
+  sync();  // gs is a local buffer
+  size_t b0 = 0;
+  while (b0 < numImages) {
+    size_t b1 = b0 + 1;
+    while (b1 < numImages && (size_t)(gs[b1 + 1] - gs[b0]) * R::kBytes <= kRecordBudget) ++b1;
+    const uint64_t rowBase = off[b0], rows = off[b1] - off[b0];
+    const size_t nG = (size_t)(gs[b1] - gs[b0]);
+    unsigned char* rec = (unsigned char*)scratch(SLOT_FISHER_WORK, (nG ? nG : 1) * R::kBytes);
+    const int n = (int)(b1 - b0);
+    if (nG > 0) {
+      VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS>), dim3((unsigned)((nG + ML_GROUPS - 1) / ML_GROUPS)),
+                   dim3(256), 0, x + rowBase * D, (long long)rows, (int)D, Kp, Whi, Wlo, bias, scp, g, rec,
+                   (long long)nG, offsets, (int)b0, n, gstart);
+    }
+    const int padImages = n >= 8 ? ((n + 7) / 8) * 8 : n;
+    VLHIP_LAUNCH("mfma_fisher", kern, dim3((unsigned)((size_t)padImages * numKB)), dim3(WAVES * 64), shmem, rec,
+                 offsets, gstart, (int)b0, n, (int)D, (int)K, Whi, Wlo, bias, scp, mup, isg, sinv,
+                 priors, flags, enc, normpart, numTerms, numKB);
+    b0 = b1;
+  }
   if (flags & 2) {
     launch_fisher_normalize(enc, normpart, numKB, numImages, (size_t)2 * K * D);
   }
+}
+
+// Component tiling: 8 waves x 32 components for large mixtures, 4 waves x 32
+// components when K is small (less padding) or D > 80 (registers).
+template <typename H, int NS>
+static void run_mfma_k(float* enc, const float* means, const float* covs, const float* priors, size_t D, size_t K,
+                       const float* x, const uint64_t* offsets, size_t numImages, int flags,
+                       unsigned long long* numTerms, const std::vector<uint64_t>& off) {
+  if (K > 128 && NS <= 5) run_mfma<H, NS, 2, 8>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off);
+  else run_mfma<H, NS, 2, 4>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off);
 }
 
 template <typename H>
 static void mfma_dispatch(float* enc, const float* means, const float* covs, const float* priors, size_t D, size_t K,
                           const float* x, const uint64_t* offsets, size_t numImages, int flags,
                           unsigned long long* numTerms) {
-  uint64_t total = 0;
-  download(&total, offsets + numImages, sizeof(uint64_t));
+  std::vector<uint64_t> off(numImages + 1);
+  download(off.data(), offsets, (numImages + 1) * sizeof(uint64_t));
   const size_t Dp = (D + 15) / 16 * 16;
   switch (Dp) {
-    case 16: run_mfma<H, 1>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, total); break;
-    case 32: run_mfma<H, 2>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, total); break;
-    case 48: run_mfma<H, 3>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, total); break;
-    case 64: run_mfma<H, 4>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, total); break;
-    case 80: run_mfma<H, 5>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, total); break;
-    case 96: run_mfma<H, 6>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, total); break;
+    case 16: run_mfma_k<H, 1>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off); break;
+    case 32: run_mfma_k<H, 2>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off); break;
+    case 48: run_mfma_k<H, 3>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off); break;
+    case 64: run_mfma_k<H, 4>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off); break;
+    case 80: run_mfma_k<H, 5>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off); break;
+    case 96: run_mfma_k<H, 6>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off); break;
     default: fatal("vlhip_fisher_encode_batch", __FILE__, __LINE__,
                    "the matrix-core Fisher encoder supports dimension <= 96");
   }

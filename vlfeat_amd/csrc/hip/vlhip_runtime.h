@@ -44,7 +44,8 @@ enum Slot {
   SLOT_MISC2,
   SLOT_GATE,          // learned-gate activations / candidate lists
   SLOT_GATE2,
-  SLOT_FISHER_Q,      // posteriors of one image group inside the FP32 Fisher encoder
+  SLOT_FISHER_WORK,   // per-batch work buffer of the Fisher encoders (posteriors or operand records)
+  SLOT_GROUPS,        // per-image first group index of the matrix-core Fisher encoder
   SLOT_COUNT
 };
 
