@@ -36,6 +36,7 @@
 namespace vlhip {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
@@ -206,7 +207,9 @@ __device__ __forceinline__ void make_frag(const float (&xc)[8], bool square, flo
 //           (x' s1 hi, x' s1 lo, x'^2 s2 hi, x'^2 s2 lo), lane l holds feature
 //           16ft + (l & 15) of rows rho(l >> 4, v), v = 0..7, where
 //           rho(lg, v) = 16(v >> 2) + 4 lg + (v & 3);
-//   stats   [32] x {max logit, 1 / sum_k 2^(logit_k - max), argmax, 0}.
+//   stats   [3][32]: -max logit, sq / sum_k 2^(logit_k - max) (0 for the
+//           padding rows after an image's last descriptor), argmax (-1 for
+//           padding rows; written only when FAST encodings are requested).
 // Logits are in log2 units (the weight tables carry the factor log2 e).
 // rho is the order in which a lane holds rows of the logit accumulators
 // (C layout: rows 4(l >> 4) + r of each 16-row tile), so the posteriors become
@@ -214,13 +217,13 @@ __device__ __forceinline__ void make_frag(const float (&xc)[8], bool square, flo
 // over rows does not depend on the order, only on A and B agreeing on it.
 constexpr int GR = 32;
 constexpr int MB_COMPS = 64;
-constexpr int ML_GROUPS = 4;  // groups (waves) per first-pass workgroup
+constexpr int ML_GROUPS = 8;  // groups (waves) per first-pass workgroup
 
 template <int NS>
 struct GroupRec {
   static constexpr size_t kA = (size_t)2 * NS * 2 * 64 * 16;
   static constexpr size_t kB = (size_t)4 * NS * 64 * 16;
-  static constexpr size_t kStats = (size_t)GR * 16;
+  static constexpr size_t kStats = (size_t)3 * GR * 4;
   static constexpr size_t kBytes = kA + kB + kStats;
 };
 
@@ -239,16 +242,31 @@ __device__ __forceinline__ double block_sum_waves(double v, double* red) {
   return r;
 }
 
+// Weight staging of the first pass: STG 16-byte words per thread per block.
+template <typename H, int STG>
+__device__ __forceinline__ void w_load(u32x4 (&pre)[STG], const H* Whi, const H* Wlo, int cb, int Dp,
+                                       const bool (&lo)[STG], const int (&off)[STG]) {
+#pragma unroll
+  for (int q = 0; q < STG; ++q) pre[q] = *reinterpret_cast<const u32x4*>((lo[q] ? Wlo : Whi) + (size_t)cb * 2 * Dp + off[q]);
+}
+template <typename H, int STG>
+__device__ __forceinline__ void w_store(const u32x4 (&pre)[STG], H* base, const int (&off)[STG]) {
+#pragma unroll
+  for (int q = 0; q < STG; ++q) *reinterpret_cast<u32x4*>(base + off[q]) = pre[q];
+}
+
 // ---------------------------------------------------------------------------
-// First pass.  Workgroup: 4 waves, wave = one group of 32 rows.  The wave
+// First pass.  Workgroup: 8 waves, wave = one group of 32 rows.  The wave
 // builds its A fragments from the fp32 descriptors, stores them and the B
-// fragments in the group record, then sweeps all components (weights staged
-// in LDS per 64 components; 2 row tiles x 4 component tiles x NS steps x 3
-// products per block) keeping a lane-local online log-sum-exp, merged across
-// the 16 lanes that share rows at the end.
-template <typename H, int NS>
-__global__ __launch_bounds__(256, 2) void mfma_lse_kernel(
-    const float* __restrict__ x, long long N, int D, int Kp,
+// fragments in the group record, then sweeps all components: the split
+// weights of 64 components at a time are double-buffered in LDS (the next
+// block is fetched into registers while the current one is multiplied), and
+// every wave runs 2 row tiles x 4 component tiles x NS steps x 3 products per
+// block, keeping a lane-local online log-sum-exp (and, with ARG, the first
+// index of the maximum) merged across the 16 lanes that share rows at the end.
+template <typename H, int NS, bool ARG>
+__global__ __launch_bounds__(ML_GROUPS * 64, 1) void mfma_lse_kernel(
+    const float* __restrict__ x, int D, int Kp,
     const H* __restrict__ Whi, const H* __restrict__ Wlo, const float* __restrict__ bias,
     const MfmaScales* __restrict__ scp, const float* __restrict__ g,
     unsigned char* __restrict__ rec, long long numGroups, const uint64_t* __restrict__ offsets,
@@ -256,8 +274,12 @@ __global__ __launch_bounds__(256, 2) void mfma_lse_kernel(
   typedef typename VecOf<H>::type V8;
   typedef GroupRec<NS> R;
   constexpr int Dp = NS * 16;
-  constexpr int WS = 2 * Dp + 8;  // LDS row stride (halves) of the staged weights
-  __shared__ __align__(16) H wsh[2][MB_COMPS][WS];
+  constexpr int WS = 2 * Dp + 8;                 // LDS row stride (halves) of the staged weights
+  constexpr int NT = ML_GROUPS * 64;
+  constexpr int PART = MB_COMPS * (2 * Dp / 8);  // 16-byte words of one weight part per block
+  constexpr int STG = 2 * PART / NT;             // words per thread per block (= NS)
+  static_assert(STG * NT == 2 * PART, "staging must divide evenly");
+  __shared__ __align__(16) H wsh[2][2][MB_COMPS][WS];  // [buffer][hi, lo][comp][feature]
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int lr = lane & 15, lg = lane >> 4;
@@ -279,7 +301,22 @@ __global__ __launch_bounds__(256, 2) void mfma_lse_kernel(
   const float s1 = scp->s1, s2 = scp->s2;
   V8* Ap = reinterpret_cast<V8*>(rec + (size_t)grp * R::kBytes);
   V8* Bp = reinterpret_cast<V8*>(rec + (size_t)grp * R::kBytes + R::kA);
-  float4* St = reinterpret_cast<float4*>(rec + (size_t)grp * R::kBytes + R::kA + R::kB);
+  float* Sf = reinterpret_cast<float*>(rec + (size_t)grp * R::kBytes + R::kA + R::kB);
+
+  // staging map of this thread (the same for every block)
+  int srcOff[STG], dstOff[STG];
+  bool srcLo[STG];
+#pragma unroll
+  for (int q = 0; q < STG; ++q) {
+    const int i = tid + NT * q;
+    const int part = i / PART, rem = i - part * PART;
+    const int c = rem / (2 * Dp / 8), q8 = rem - c * (2 * Dp / 8);
+    srcLo[q] = part != 0;
+    srcOff[q] = c * 2 * Dp + 8 * q8;
+    dstOff[q] = (part * MB_COMPS + c) * WS + 8 * q8;
+  }
+  u32x4 wpre[STG];
+  w_load<H, STG>(wpre, Whi, Wlo, 0, Dp, srcLo, srcOff);
 
   // A fragments of this wave's 32 rows, resident for the whole sweep
   V8 ahi[2][NS], alo[2][NS];
@@ -337,17 +374,16 @@ __global__ __launch_bounds__(256, 2) void mfma_lse_kernel(
 #pragma unroll
     for (int r = 0; r < 4; ++r) { mrun[t][r] = -INFINITY; srun[t][r] = 0.f; bval[t][r] = -INFINITY; bidx[t][r] = 0; }
 
+  w_store<H, STG>(wpre, &wsh[0][0][0][0], dstOff);
+  if (MB_COMPS < Kp) w_load<H, STG>(wpre, Whi, Wlo, MB_COMPS, Dp, srcLo, srcOff);
+  __syncthreads();
   for (int cb = 0; cb < Kp; cb += MB_COMPS) {
-    __syncthreads();
-    // stage the block's weights: 64 comps x 2Dp halves, hi and lo
-    for (int i = tid; i < 2 * MB_COMPS * (2 * Dp / 8); i += 256) {
-      const int part = i / (MB_COMPS * (2 * Dp / 8));
-      const int rem = i % (MB_COMPS * (2 * Dp / 8));
-      const int c = rem / (2 * Dp / 8), q8 = rem % (2 * Dp / 8);
-      const H* src = (part ? Wlo : Whi) + (size_t)(cb + c) * 2 * Dp + 8 * q8;
-      *reinterpret_cast<uint4*>(&wsh[part][c][8 * q8]) = *reinterpret_cast<const uint4*>(src);
+    const int buf = (cb / MB_COMPS) & 1;
+    if (cb + MB_COMPS < Kp) {
+      // every wave finished reading this buffer (block cb - 64) before the last barrier
+      w_store<H, STG>(wpre, &wsh[buf ^ 1][0][0][0], dstOff);
+      if (cb + 2 * MB_COMPS < Kp) w_load<H, STG>(wpre, Whi, Wlo, cb + 2 * MB_COMPS, Dp, srcLo, srcOff);
     }
-    __syncthreads();
     f32x4 acc4[4][2];
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc4[c][0] = acc4[c][1] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -356,8 +392,8 @@ __global__ __launch_bounds__(256, 2) void mfma_lse_kernel(
       V8 bh[4], bl[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        bh[c] = *reinterpret_cast<const V8*>(&wsh[0][16 * c + lr][32 * j + 8 * lg]);
-        bl[c] = *reinterpret_cast<const V8*>(&wsh[1][16 * c + lr][32 * j + 8 * lg]);
+        bh[c] = *reinterpret_cast<const V8*>(&wsh[buf][0][16 * c + lr][32 * j + 8 * lg]);
+        bl[c] = *reinterpret_cast<const V8*>(&wsh[buf][1][16 * c + lr][32 * j + 8 * lg]);
       }
       // eight independent accumulators between dependent matrix ops
 #pragma unroll
@@ -384,7 +420,7 @@ __global__ __launch_bounds__(256, 2) void mfma_lse_kernel(
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           lv[c] = acc4[c][t][r] + bc[c];
-          if (lv[c] > bval[t][r]) { bval[t][r] = lv[c]; bidx[t][r] = cb + 16 * c + lr; }
+          if (ARG && lv[c] > bval[t][r]) { bval[t][r] = lv[c]; bidx[t][r] = cb + 16 * c + lr; }
         }
         const float mx = fmaxf(fmaxf(lv[0], lv[1]), fmaxf(lv[2], lv[3]));
         const float mn = fmaxf(mrun[t][r], mx);
@@ -394,9 +430,11 @@ __global__ __launch_bounds__(256, 2) void mfma_lse_kernel(
                    + (__builtin_amdgcn_exp2f(lv[2] - ms) + __builtin_amdgcn_exp2f(lv[3] - ms)));
         mrun[t][r] = mn;
       }
+    __syncthreads();
   }
 
   // merge the 16 lanes that hold the same rows (lanes lg*16 .. lg*16+15)
+  const float sq = scp->sq;
 #pragma unroll
   for (int t = 0; t < 2; ++t)
 #pragma unroll
@@ -408,15 +446,23 @@ __global__ __launch_bounds__(256, 2) void mfma_lse_kernel(
       float s = srun[t][r] * __builtin_amdgcn_exp2f(mrun[t][r] - ms);
 #pragma unroll
       for (int o = 1; o < 16; o <<= 1) s += __shfl_xor(s, o, 64);
-      float bv = bval[t][r];
       int bi = bidx[t][r];
+      if (ARG) {
+        float bv = bval[t][r];
 #pragma unroll
-      for (int o = 1; o < 16; o <<= 1) {
-        const float ov = __shfl_xor(bv, o, 64);
-        const int oi = __shfl_xor(bi, o, 64);
-        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+        for (int o = 1; o < 16; o <<= 1) {
+          const float ov = __shfl_xor(bv, o, 64);
+          const int oi = __shfl_xor(bi, o, 64);
+          if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+        }
       }
-      if (live && lr == 0) St[16 * t + 4 * lg + r] = make_float4(m, 1.0f / s, __int_as_float(bi), 0.f);
+      const int row = 16 * t + 4 * lg + r;
+      if (live && lr == 0) {
+        const bool pad = row0 + row >= rowEnd;
+        Sf[row] = pad ? 0.f : -m;
+        Sf[GR + row] = pad ? 0.f : sq / s;
+        if (ARG) Sf[2 * GR + row] = __int_as_float(pad ? -1 : bi);
+      }
     }
 }
 
@@ -436,7 +482,6 @@ __device__ __forceinline__ void fisher_mfma_coords(int numKB, int numImages, int
 }
 
 // Copy of one group record between HBM, registers and LDS (NT threads, 16 bytes each).
-typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 template <int PER, int NT, int NV>
 __device__ __forceinline__ void rec_load(u32x4 (&pre)[PER], const unsigned char* src) {
   const u32x4* s = reinterpret_cast<const u32x4*>(src);
@@ -519,12 +564,16 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
     for (int s = 0; s < 2; ++s)
 #pragma unroll
       for (int ft = 0; ft < NS; ++ft) macc[c][s][ft] = f32x4{0.f, 0.f, 0.f, 0.f};
-  float s0l[CT];
+  f32x4 macc0[CT];  // posterior mass S0 (every column)
 #pragma unroll
-  for (int c = 0; c < CT; ++c) s0l[c] = 0.f;
+  for (int c = 0; c < CT; ++c) macc0[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  V8 ones;
+#pragma unroll
+  for (int v = 0; v < 8; ++v) ones[v] = H(1.f);
   unsigned terms = 0;
-  // (double)q < 1e-6 for a float q is q < the next float above float(1e-6)
-  const float qmin = __uint_as_float(0x358637beu);  // 1.0000001e-6
+  // (double)q < 1e-6 for a float q is q < the next float above float(1e-6);
+  // the posteriors are compared after the exact power-of-two scaling by sq
+  const float qminS = __uint_as_float(0x358637beu) * sc.sq;  // 1.0000001e-6 * sq
 
   u32x4 pre[PER];
   if (gB < gE) {
@@ -544,14 +593,22 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
     const unsigned char* cur = smem_raw + (size_t)buf * R::kBytes;
     const V8* Ap = reinterpret_cast<const V8*>(cur);
     const V8* Bp = reinterpret_cast<const V8*>(cur + R::kA);
-    const float4* St = reinterpret_cast<const float4*>(cur + R::kA + R::kB);
+    const float* Sf = reinterpret_cast<const float*>(cur + R::kA + R::kB);
+    // rows 16t + 4lg + r of this lane: -max, sq / sum
+    f32x4 nm4[2], rs4[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      nm4[t] = *reinterpret_cast<const f32x4*>(&Sf[16 * t + 4 * lg]);
+      rs4[t] = *reinterpret_cast<const f32x4*>(&Sf[GR + 16 * t + 4 * lg]);
+    }
 
-    // ---- logits
+    // ---- logits - max (the accumulators start at bias - max)
     f32x4 acc[2][CT];
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
-      for (int c = 0; c < CT; ++c) acc[t][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int c = 0; c < CT; ++c)
+        acc[t][c] = f32x4{bc[c] + nm4[t][0], bc[c] + nm4[t][1], bc[c] + nm4[t][2], bc[c] + nm4[t][3]};
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
       V8 ah[2], al[2];
@@ -574,32 +631,39 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
         for (int c = 0; c < CT; ++c) acc[t][c] = mfma16(ah[t], wlo[c][j], acc[t][c]);
     }
 
-    // ---- posteriors, straight into A fragments of the moment contraction
-    const long long gr0 = (gi - gB) * GR;  // image-relative first row of the group
+    // ---- scaled posteriors q * sq, straight into A fragments of the moment
+    // contraction (padding rows have sq / sum = 0 and FAST argmax -1)
     V8 aq[CT];
+    if (fast) {
+      f32x4 ar4[2];
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+      for (int t = 0; t < 2; ++t) ar4[t] = *reinterpret_cast<const f32x4*>(&Sf[2 * GR + 16 * t + 4 * lg]);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = 16 * t + 4 * lg + r;
-        const float4 st = St[row];
-        const bool ok = gr0 + row < nImg;
+      for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int c = 0; c < CT; ++c) {
-          float q = 0.f;
-          if (ok) {
-            if (fast) {
-              q = (kw + 16 * c + lr == __float_as_int(st.z) && bc[c] != -INFINITY) ? 1.f : 0.f;
-            } else {
-              q = __builtin_amdgcn_exp2f(acc[t][c][r] + bc[c] - st.x) * st.y;
-            }
-            if (q < qmin) q = 0.f; else terms += 1;
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < CT; ++c) {
+            const bool hit = kw + 16 * c + lr == __float_as_int(ar4[t][r]) && bc[c] != -INFINITY;
+            terms += hit ? 1u : 0u;
+            aq[c][4 * t + r] = hit ? H(sc.sq) : H(0.f);
           }
-          const H h = H(q * sc.sq);
-          s0l[c] += (float)h;
-          aq[c][4 * t + r] = h;
-        }
-      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < CT; ++c) {
+            const float v = __builtin_amdgcn_exp2f(acc[t][c][r]) * rs4[t][r];
+            const bool keep = v >= qminS;
+            terms += keep ? 1u : 0u;
+            aq[c][4 * t + r] = H(keep ? v : 0.f);
+          }
+    }
+    // posterior mass: every column of q x ones is the row sum
+#pragma unroll
+    for (int c = 0; c < CT; ++c) macc0[c] = mfma16(aq[c], ones, macc0[c]);
 
     // ---- moments
 #pragma unroll
@@ -613,15 +677,6 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
     __syncthreads();
   }
 
-  // ---- posterior mass per component: lanes sharing lr hold other rows
-#pragma unroll
-  for (int c = 0; c < CT; ++c) {
-    float sv = s0l[c];
-    sv += __shfl_xor(sv, 16, 64);
-    sv += __shfl_xor(sv, 32, 64);
-    s0l[c] = sv;
-  }
-
   // ---- finalisation in registers
   const double invq = 1.0 / (double)sc.sq;
   const double inv1 = invq / (double)sc.s1, inv2 = invq / (double)sc.s2;
@@ -633,8 +688,7 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
   for (int c = 0; c < CT; ++c)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      // S0 of component 16c + 4lg + r lives in lane 4lg + r
-      const double S0 = (double)__shfl(s0l[c], 4 * lg + r, 64) * invq;
+      const double S0 = (double)macc0[c][r] * invq;
       const int k = kw + 16 * c + 4 * lg + r;
       const double prior = k < K ? (double)priors[k] : 0.0;
       const bool use = k < K && prior >= 1e-6;
@@ -726,9 +780,16 @@ static void run_mfma(float* enc, const float* means, const float* covs, const fl
     unsigned char* rec = (unsigned char*)scratch(SLOT_FISHER_WORK, (nG ? nG : 1) * R::kBytes);
     const int n = (int)(b1 - b0);
     if (nG > 0) {
-      VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS>), dim3((unsigned)((nG + ML_GROUPS - 1) / ML_GROUPS)),
-                   dim3(256), 0, x + rowBase * D, (long long)rows, (int)D, Kp, Whi, Wlo, bias, scp, g, rec,
-                   (long long)nG, offsets, (int)b0, n, gstart);
+      const unsigned lb = (unsigned)((nG + ML_GROUPS - 1) / ML_GROUPS);
+      if (flags & 4) {
+        VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, true>), dim3(lb), dim3(ML_GROUPS * 64), 0,
+                     x + rowBase * D, (int)D, Kp, Whi, Wlo, bias, scp, g, rec, (long long)nG, offsets, (int)b0, n,
+                     gstart);
+      } else {
+        VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, false>), dim3(lb), dim3(ML_GROUPS * 64), 0,
+                     x + rowBase * D, (int)D, Kp, Whi, Wlo, bias, scp, g, rec, (long long)nG, offsets, (int)b0, n,
+                     gstart);
+      }
     }
     const int padImages = n >= 8 ? ((n + 7) / 8) * 8 : n;
     VLHIP_LAUNCH("mfma_fisher", kern, dim3((unsigned)((size_t)padImages * numKB)), dim3(WAVES * 64), shmem, rec,
