@@ -255,6 +255,158 @@ __device__ __forceinline__ void w_store(const u32x4 (&pre)[STG], H* base, const 
   for (int q = 0; q < STG; ++q) *reinterpret_cast<u32x4*>(base + off[q]) = pre[q];
 }
 
+// Group table of a batch: group gstart[i] - gstart[imgBegin] + k of image i
+// starts at batch-relative row offsets[i] - offsets[imgBegin] + 32k, and its
+// image ends at offsets[i+1] - offsets[imgBegin].  One thread per image.
+__global__ void mfma_group_table_kernel(const uint64_t* __restrict__ offsets, const long long* __restrict__ gstart,
+                                        int imgBegin, int numImages, int2* __restrict__ tab) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= numImages) return;
+  const int img = imgBegin + i;
+  const long long rb = (long long)offsets[imgBegin];
+  const int r0 = (int)((long long)offsets[img] - rb), r1 = (int)((long long)offsets[img + 1] - rb);
+  const long long g0 = gstart[img] - gstart[imgBegin];
+  const int ng = (int)(gstart[img + 1] - gstart[img]);
+  for (int k = 0; k < ng; ++k) tab[g0 + k] = make_int2(r0 + GR * k, r1);
+}
+
+// Running log-sum-exp (log2 units) and first maximum of the lane's 8 rows.
+struct LseState {
+  float m[2][4], s[2][4], bv[2][4];
+  int bi[2][4];
+};
+
+// Logits of 32 rows x 64 components (one block of staged weights, layout
+// [hi, lo][comp][2Dp + 8]): 2 row tiles x 4 component tiles x NS steps x 3
+// products, eight independent accumulators between dependent matrix ops.
+template <typename H, int NS>
+__device__ __forceinline__ void lse_mfma(f32x4 (&acc)[4][2], const typename VecOf<H>::type (&ahi)[2][NS],
+                                         const typename VecOf<H>::type (&alo)[2][NS], const H* w, int lr, int lg) {
+  typedef typename VecOf<H>::type V8;
+  constexpr int WS = 2 * NS * 16 + 8;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) acc[c][0] = acc[c][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    V8 bh[4], bl[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      bh[c] = *reinterpret_cast<const V8*>(w + (16 * c + lr) * WS + 32 * j + 8 * lg);
+      bl[c] = *reinterpret_cast<const V8*>(w + (MB_COMPS + 16 * c + lr) * WS + 32 * j + 8 * lg);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[c][t] = mfma16(ahi[t][j], bh[c], acc[c][t]);
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[c][t] = mfma16(alo[t][j], bh[c], acc[c][t]);
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[c][t] = mfma16(ahi[t][j], bl[c], acc[c][t]);
+  }
+}
+
+// Online log-sum-exp update with the logits of components cb .. cb + 63
+// (lane holds components cb + 16c + lr); with ARG also the first maximum.
+template <bool ARG>
+__device__ __forceinline__ void lse_update(const f32x4 (&acc)[4][2], const float* __restrict__ bias, int cb, int lr,
+                                           LseState& st) {
+  float bc[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) bc[c] = bias[cb + 16 * c + lr];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float lv[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        lv[c] = acc[c][t][r] + bc[c];
+        if (ARG && lv[c] > st.bv[t][r]) { st.bv[t][r] = lv[c]; st.bi[t][r] = cb + 16 * c + lr; }
+      }
+      const float mx = fmaxf(fmaxf(lv[0], lv[1]), fmaxf(lv[2], lv[3]));
+      const float mn = fmaxf(st.m[t][r], mx);
+      const float ms = (mn == -INFINITY) ? 0.f : mn;
+      st.s[t][r] = st.s[t][r] * __builtin_amdgcn_exp2f(st.m[t][r] - ms)
+                 + ((__builtin_amdgcn_exp2f(lv[0] - ms) + __builtin_amdgcn_exp2f(lv[1] - ms))
+                 + (__builtin_amdgcn_exp2f(lv[2] - ms) + __builtin_amdgcn_exp2f(lv[3] - ms)));
+      st.m[t][r] = mn;
+    }
+}
+
+// Log-sum-exp update of one row (t, r) with the logits acc of components
+// cb + 16c + lr (bias bc already loaded).
+template <bool ARG>
+__device__ __forceinline__ void lse_row(const f32x4 (&acc)[4][2], const float (&bc)[4], int t, int r, int cb, int lr,
+                                        LseState& st) {
+  float lv[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    lv[c] = acc[c][t][r] + bc[c];
+    if (ARG && lv[c] > st.bv[t][r]) { st.bv[t][r] = lv[c]; st.bi[t][r] = cb + 16 * c + lr; }
+  }
+  const float mx = fmaxf(fmaxf(lv[0], lv[1]), fmaxf(lv[2], lv[3]));
+  const float mn = fmaxf(st.m[t][r], mx);
+  const float ms = (mn == -INFINITY) ? 0.f : mn;
+  st.s[t][r] = st.s[t][r] * __builtin_amdgcn_exp2f(st.m[t][r] - ms)
+             + ((__builtin_amdgcn_exp2f(lv[0] - ms) + __builtin_amdgcn_exp2f(lv[1] - ms))
+             + (__builtin_amdgcn_exp2f(lv[2] - ms) + __builtin_amdgcn_exp2f(lv[3] - ms)));
+  st.m[t][r] = mn;
+}
+
+// One pipelined step: the products of a block into accNew and the
+// log-sum-exp update with the previous block's accOld, cut into NS chunks
+// (one feature step of products plus a share of the 8 row updates each) whose
+// matrix and vector instructions are interleaved one to two.
+template <typename H, int NS, bool ARG>
+__device__ __forceinline__ void lse_fused(f32x4 (&accNew)[4][2], const f32x4 (&accOld)[4][2],
+                                          const typename VecOf<H>::type (&ahi)[2][NS],
+                                          const typename VecOf<H>::type (&alo)[2][NS], const H* w, int lr, int lg,
+                                          const float* __restrict__ bias, int cbOld, LseState& st) {
+  typedef typename VecOf<H>::type V8;
+  constexpr int WS = 2 * NS * 16 + 8;
+  float bc[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) bc[c] = bias[cbOld + 16 * c + lr];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) accNew[c][0] = accNew[c][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    V8 bh[4], bl[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      bh[c] = *reinterpret_cast<const V8*>(w + (16 * c + lr) * WS + 32 * j + 8 * lg);
+      bl[c] = *reinterpret_cast<const V8*>(w + (MB_COMPS + 16 * c + lr) * WS + 32 * j + 8 * lg);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) accNew[c][t] = mfma16(ahi[t][j], bh[c], accNew[c][t]);
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) accNew[c][t] = mfma16(alo[t][j], bh[c], accNew[c][t]);
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) accNew[c][t] = mfma16(ahi[t][j], bl[c], accNew[c][t]);
+    // rows [8j/NS, 8(j+1)/NS) of the previous block
+#pragma unroll
+    for (int q = (8 * j) / NS; q < (8 * (j + 1)) / NS; ++q) lse_row<ARG>(accOld, bc, q >> 2, q & 3, cbOld, lr, st);
+    __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);  // the step's 8 LDS fragment reads
+    __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);  // vector work under their latency
+#pragma unroll
+    for (int i = 0; i < 24; ++i) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one matrix op
+      __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // two vector ops
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // First pass.  Workgroup: 8 waves, wave = one group of 32 rows.  The wave
 // builds its A fragments from the fp32 descriptors, stores them and the B
@@ -264,13 +416,13 @@ __device__ __forceinline__ void w_store(const u32x4 (&pre)[STG], H* base, const 
 // every wave runs 2 row tiles x 4 component tiles x NS steps x 3 products per
 // block, keeping a lane-local online log-sum-exp (and, with ARG, the first
 // index of the maximum) merged across the 16 lanes that share rows at the end.
+// The log-sum-exp update of a block is issued with the products of the next.
 template <typename H, int NS, bool ARG>
 __global__ __launch_bounds__(ML_GROUPS * 64, 1) void mfma_lse_kernel(
     const float* __restrict__ x, int D, int Kp,
     const H* __restrict__ Whi, const H* __restrict__ Wlo, const float* __restrict__ bias,
     const MfmaScales* __restrict__ scp, const float* __restrict__ g,
-    unsigned char* __restrict__ rec, long long numGroups, const uint64_t* __restrict__ offsets,
-    int imgBegin, int numImages, const long long* __restrict__ gstart) {
+    unsigned char* __restrict__ rec, long long numGroups, const int2* __restrict__ gtab) {
   typedef typename VecOf<H>::type V8;
   typedef GroupRec<NS> R;
   constexpr int Dp = NS * 16;
@@ -285,18 +437,12 @@ __global__ __launch_bounds__(ML_GROUPS * 64, 1) void mfma_lse_kernel(
   const int lr = lane & 15, lg = lane >> 4;
   const long long grp = (long long)blockIdx.x * ML_GROUPS + w;  // relative to the batch
   const bool live = grp < numGroups;  // uniform per wave; dead waves still take part in barriers
-  // owning image: the last i with gstart[i] <= group (empty images share their start)
-  long long row0 = 0, rowEnd = 0;  // batch-relative rows of this group
+  // batch-relative first row of the group and end of its image
+  long long row0 = 0, rowEnd = 0;
   if (live) {
-    const long long gg = gstart[imgBegin] + grp;
-    int lo = imgBegin, hi = imgBegin + numImages;  // gstart[lo] <= gg < gstart[hi]
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (gstart[mid] <= gg) lo = mid; else hi = mid;
-    }
-    const long long rowBase = (long long)offsets[imgBegin];
-    row0 = (long long)offsets[lo] - rowBase + (gg - gstart[lo]) * GR;
-    rowEnd = (long long)offsets[lo + 1] - rowBase;
+    const int2 gr = gtab[grp];
+    row0 = gr.x;
+    rowEnd = gr.y;
   }
   const float s1 = scp->s1, s2 = scp->s2;
   V8* Ap = reinterpret_cast<V8*>(rec + (size_t)grp * R::kBytes);
@@ -318,21 +464,32 @@ __global__ __launch_bounds__(ML_GROUPS * 64, 1) void mfma_lse_kernel(
   u32x4 wpre[STG];
   w_load<H, STG>(wpre, Whi, Wlo, 0, Dp, srcLo, srcOff);
 
-  // A fragments of this wave's 32 rows, resident for the whole sweep
+  // A fragments of this wave's 32 rows, resident for the whole sweep (16-byte
+  // loads where the row is whole and aligned; g holds Dp entries, zero past D)
+  const bool vecx = (D & 3) == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0;
   V8 ahi[2][NS], alo[2][NS];
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const long long row = row0 + 16 * t + lr;
+  for (int j = 0; j < NS; ++j) {
+    const int f0 = 32 * j + 8 * lg;
+    const bool sq = f0 < Dp;
+    const int d0 = sq ? f0 : f0 - Dp;
+    const f32x4 g0 = *reinterpret_cast<const f32x4*>(g + d0);
+    const f32x4 g1 = *reinterpret_cast<const f32x4*>(g + d0 + 4);
 #pragma unroll
-    for (int j = 0; j < NS; ++j) {
-      const int f0 = 32 * j + 8 * lg;
-      const bool sq = f0 < Dp;
-      const int d0 = sq ? f0 : f0 - Dp;
+    for (int t = 0; t < 2; ++t) {
+      const long long row = row0 + 16 * t + lr;
       float xc[8];
+      if (vecx && row < rowEnd && d0 + 8 <= D) {
+        const f32x4 a = *reinterpret_cast<const f32x4*>(x + row * D + d0);
+        const f32x4 b = *reinterpret_cast<const f32x4*>(x + row * D + d0 + 4);
 #pragma unroll
-      for (int v = 0; v < 8; ++v) {
-        const int d = d0 + v;
-        xc[v] = (row < rowEnd && d < D) ? x[row * D + d] - g[d] : 0.f;
+        for (int v = 0; v < 4; ++v) { xc[v] = a[v] - g0[v]; xc[4 + v] = b[v] - g1[v]; }
+      } else {
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+          const int d = d0 + v;
+          xc[v] = (row < rowEnd && d < D) ? x[row * D + d] - g[d] : 0.f;
+        }
       }
       make_frag<H>(xc, sq, s1, s2, ahi[t][j], alo[t][j]);
       if (live) {
@@ -367,71 +524,51 @@ __global__ __launch_bounds__(ML_GROUPS * 64, 1) void mfma_lse_kernel(
     }
   }
 
-  float mrun[2][4], srun[2][4], bval[2][4];
-  int bidx[2][4];
+  LseState st;
 #pragma unroll
   for (int t = 0; t < 2; ++t)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) { mrun[t][r] = -INFINITY; srun[t][r] = 0.f; bval[t][r] = -INFINITY; bidx[t][r] = 0; }
+    for (int r = 0; r < 4; ++r) { st.m[t][r] = -INFINITY; st.s[t][r] = 0.f; st.bv[t][r] = -INFINITY; st.bi[t][r] = 0; }
 
+  // Software pipeline over the component blocks: the matrix products of block
+  // b are issued together with the log-sum-exp update of block b - 1, so the
+  // vector work overlaps the matrix cores; one barrier per block.
+  const int nb = Kp / MB_COMPS;
+  f32x4 accA[4][2], accB[4][2];
   w_store<H, STG>(wpre, &wsh[0][0][0][0], dstOff);
-  if (MB_COMPS < Kp) w_load<H, STG>(wpre, Whi, Wlo, MB_COMPS, Dp, srcLo, srcOff);
+  if (nb > 1) w_load<H, STG>(wpre, Whi, Wlo, MB_COMPS, Dp, srcLo, srcOff);
   __syncthreads();
-  for (int cb = 0; cb < Kp; cb += MB_COMPS) {
-    const int buf = (cb / MB_COMPS) & 1;
-    if (cb + MB_COMPS < Kp) {
-      // every wave finished reading this buffer (block cb - 64) before the last barrier
-      w_store<H, STG>(wpre, &wsh[buf ^ 1][0][0][0], dstOff);
-      if (cb + 2 * MB_COMPS < Kp) w_load<H, STG>(wpre, Whi, Wlo, cb + 2 * MB_COMPS, Dp, srcLo, srcOff);
-    }
-    f32x4 acc4[4][2];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) acc4[c][0] = acc4[c][1] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int j = 0; j < NS; ++j) {
-      V8 bh[4], bl[4];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        bh[c] = *reinterpret_cast<const V8*>(&wsh[buf][0][16 * c + lr][32 * j + 8 * lg]);
-        bl[c] = *reinterpret_cast<const V8*>(&wsh[buf][1][16 * c + lr][32 * j + 8 * lg]);
-      }
-      // eight independent accumulators between dependent matrix ops
-#pragma unroll
-      for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc4[c][t] = mfma16(ahi[t][j], bh[c], acc4[c][t]);
-#pragma unroll
-      for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc4[c][t] = mfma16(alo[t][j], bh[c], acc4[c][t]);
-#pragma unroll
-      for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc4[c][t] = mfma16(ahi[t][j], bl[c], acc4[c][t]);
-    }
-    float bc[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) bc[c] = bias[cb + 16 * c + lr];
-#pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        float lv[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          lv[c] = acc4[c][t][r] + bc[c];
-          if (ARG && lv[c] > bval[t][r]) { bval[t][r] = lv[c]; bidx[t][r] = cb + 16 * c + lr; }
-        }
-        const float mx = fmaxf(fmaxf(lv[0], lv[1]), fmaxf(lv[2], lv[3]));
-        const float mn = fmaxf(mrun[t][r], mx);
-        const float ms = (mn == -INFINITY) ? 0.f : mn;
-        srun[t][r] = srun[t][r] * __builtin_amdgcn_exp2f(mrun[t][r] - ms)
-                   + ((__builtin_amdgcn_exp2f(lv[0] - ms) + __builtin_amdgcn_exp2f(lv[1] - ms))
-                   + (__builtin_amdgcn_exp2f(lv[2] - ms) + __builtin_amdgcn_exp2f(lv[3] - ms)));
-        mrun[t][r] = mn;
-      }
-    __syncthreads();
+  lse_mfma<H, NS>(accA, ahi, alo, &wsh[0][0][0][0], lr, lg);
+  if (nb > 1) {
+    w_store<H, STG>(wpre, &wsh[1][0][0][0], dstOff);
+    if (nb > 2) w_load<H, STG>(wpre, Whi, Wlo, 2 * MB_COMPS, Dp, srcLo, srcOff);
   }
+  __syncthreads();
+  // step b: stage block b + 1 (its buffer held block b - 1, read before the
+  // last barrier), fetch block b + 2, multiply block b, update with block b - 1
+#define LSE_STEP(ACC_NEW, ACC_OLD, B)                                                                 \
+  {                                                                                                   \
+    const int b_ = (B);                                                                               \
+    if (b_ + 1 < nb) {                                                                                \
+      w_store<H, STG>(wpre, &wsh[(b_ + 1) & 1][0][0][0], dstOff);                                     \
+      if (b_ + 2 < nb) w_load<H, STG>(wpre, Whi, Wlo, (b_ + 2) * MB_COMPS, Dp, srcLo, srcOff);         \
+    }                                                                                                 \
+    lse_fused<H, NS, ARG>(ACC_NEW, ACC_OLD, ahi, alo, &wsh[b_ & 1][0][0][0], lr, lg, bias,            \
+                          (b_ - 1) * MB_COMPS, st);                                                   \
+    __syncthreads();                                                                                  \
+  }
+  int b = 1;
+  for (; b + 1 < nb; b += 2) {
+    LSE_STEP(accB, accA, b)
+    LSE_STEP(accA, accB, b + 1)
+  }
+  if (b < nb) {
+    LSE_STEP(accB, accA, b)
+    lse_update<ARG>(accB, bias, (nb - 1) * MB_COMPS, lr, st);
+  } else {
+    lse_update<ARG>(accA, bias, (nb - 1) * MB_COMPS, lr, st);
+  }
+#undef LSE_STEP
 
   // merge the 16 lanes that hold the same rows (lanes lg*16 .. lg*16+15)
   const float sq = scp->sq;
@@ -439,16 +576,16 @@ __global__ __launch_bounds__(ML_GROUPS * 64, 1) void mfma_lse_kernel(
   for (int t = 0; t < 2; ++t)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      float m = mrun[t][r];
+      float m = st.m[t][r];
 #pragma unroll
       for (int o = 1; o < 16; o <<= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
       const float ms = (m == -INFINITY) ? 0.f : m;
-      float s = srun[t][r] * __builtin_amdgcn_exp2f(mrun[t][r] - ms);
+      float s = st.s[t][r] * __builtin_amdgcn_exp2f(st.m[t][r] - ms);
 #pragma unroll
       for (int o = 1; o < 16; o <<= 1) s += __shfl_xor(s, o, 64);
-      int bi = bidx[t][r];
+      int bi = st.bi[t][r];
       if (ARG) {
-        float bv = bval[t][r];
+        float bv = st.bv[t][r];
 #pragma unroll
         for (int o = 1; o < 16; o <<= 1) {
           const float ov = __shfl_xor(bv, o, 64);
@@ -776,19 +913,21 @@ static void run_mfma(float* enc, const float* means, const float* covs, const fl
     size_t b1 = b0 + 1;
     while (b1 < numImages && (size_t)(gs[b1 + 1] - gs[b0]) * R::kBytes <= kRecordBudget) ++b1;
     const uint64_t rowBase = off[b0], rows = off[b1] - off[b0];
+    VLHIP_REQUIRE(rows < (uint64_t(1) << 31), "an image batch of the matrix-core encoder exceeds 2^31 descriptors");
     const size_t nG = (size_t)(gs[b1] - gs[b0]);
     unsigned char* rec = (unsigned char*)scratch(SLOT_FISHER_WORK, (nG ? nG : 1) * R::kBytes);
     const int n = (int)(b1 - b0);
     if (nG > 0) {
+      int2* gtab = (int2*)scratch(SLOT_GROUPTAB, nG * sizeof(int2));
+      VLHIP_LAUNCH("mfma_group_table", mfma_group_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                   offsets, gstart, (int)b0, n, gtab);
       const unsigned lb = (unsigned)((nG + ML_GROUPS - 1) / ML_GROUPS);
       if (flags & 4) {
         VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, true>), dim3(lb), dim3(ML_GROUPS * 64), 0,
-                     x + rowBase * D, (int)D, Kp, Whi, Wlo, bias, scp, g, rec, (long long)nG, offsets, (int)b0, n,
-                     gstart);
+                     x + rowBase * D, (int)D, Kp, Whi, Wlo, bias, scp, g, rec, (long long)nG, gtab);
       } else {
         VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, false>), dim3(lb), dim3(ML_GROUPS * 64), 0,
-                     x + rowBase * D, (int)D, Kp, Whi, Wlo, bias, scp, g, rec, (long long)nG, offsets, (int)b0, n,
-                     gstart);
+                     x + rowBase * D, (int)D, Kp, Whi, Wlo, bias, scp, g, rec, (long long)nG, gtab);
       }
     }
     const int padImages = n >= 8 ? ((n + 7) / 8) * 8 : n;

@@ -46,6 +46,7 @@ enum Slot {
   SLOT_GATE2,
   SLOT_FISHER_WORK,   // per-batch work buffer of the Fisher encoders (posteriors or operand records)
   SLOT_GROUPS,        // per-image first group index of the matrix-core Fisher encoder
+  SLOT_GROUPTAB,      // per-group first row and image end of the matrix-core Fisher encoder
   SLOT_COUNT
 };
 
