@@ -1,22 +1,29 @@
 """Benchmark: Fisher-vector encoding throughput at K=1024, D=80 on one MI355X.
 
-Workload (the metric's configuration): a K=1024 diagonal GMM fitted to SIFT-like
-80-d descriptors, then improved Fisher vectors (signed square root + L2) of
-4,096 images x 3,000 descriptors (12.29 M descriptors, FV dimension 163,840),
-all inputs resident in HBM when the timed region starts.  A step is one
-encode of the whole batch.  value = descriptors encoded per second.
+Workload (the metric's configuration): a K=1024 diagonal GMM fitted on the GPU
+to SIFT-like 80-d descriptors, then improved Fisher vectors (signed square root
++ L2) of 4,096 images x 3,000 descriptors (12.29 M descriptors, FV dimension
+163,840), all inputs resident in HBM when the timed region starts.  A step is
+one encode of the whole batch through the library's batched device entry point
+(vlhip_fisher_encode_batch).  value = descriptors encoded per second by the
+selected encoder (--mode; default: the compensated FP16 matrix-core encoder).
 
-Beside it: the reference's own OpenMP CPU path (VLFeat built from source,
-oracle/_ref) timed on a bounded sample of the same images on this host, and
-the parity of the GPU result against it on that sample.
+Beside it, in the same run:
+  * the plain FP32 GPU encoder (the reference-parity path) on the same batch,
+    and the speed-up of the selected encoder over it;
+  * the reference's own OpenMP CPU path (VLFeat built from source,
+    oracle/_ref) on a bounded sample of the same images, and the parity of the
+    GPU result against it;
+  * the error of the selected encoder against the FP64 oracle on a few images.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode fp32|bf16x3|gated]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode f16x3|bf16x3|fp32|gated]
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -31,8 +38,22 @@ NUM_IMAGES = 4096
 PER_IMAGE = 3000
 FLAGS = 3  # VL_FISHER_FLAG_IMPROVED
 PEAK_FP32_TFLOPS = 157.3
-PEAK_BF16_TFLOPS = 2500.0
+PEAK_F16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
+
+# Algorithmic flop per descriptor of each kernel (K components, D dimensions):
+# the E-step contraction [x'^2 | x'] . [w2 | w1] is 4DK, the posterior-weighted
+# accumulation Q^T [x' | x'^2] is 4DK.
+KERNEL_FLOP_PER_DESC = {
+    "mfma_fisher": 8 * DIM * K_COMP,   # logits + moments of its components
+    "mfma_lse": 4 * DIM * K_COMP,      # logits for the row statistics
+    "fisher_accum": 4 * DIM * K_COMP,  # FP32 accumulation
+    "fisher_estep": 4 * DIM * K_COMP,  # FP32 E-step
+}
+# Matrix-core flop actually issued per descriptor and component by the
+# compensated kernels (three products for the logits, two for the moments,
+# one for the posterior mass), relative to the algorithmic count.
+EXECUTED_PER_ALGORITHMIC = {"mfma_fisher": (3 * 2 * DIM + 2 * 2 * DIM + 32) / (8 * DIM), "mfma_lse": 3.0}
 
 
 def log(msg: str) -> None:
@@ -51,31 +72,61 @@ def fit_gmm(x_host: np.ndarray, K: int, iters: int, seed: int):
     return m, c, p, ll
 
 
-def cpu_baseline(x_host, offsets, m, c, p, budget_s: float):
-    """Reference VLFeat (oracle/_ref, OpenMP over all host threads) on whole
-    images of the workload until the time budget is used."""
-    from oracle import reference
+def cpu_legs(x_host, offsets, m, c, p, budget_s: float, gpu_enc, gpu_fp32_enc, n_oracle: int):
+    """The reference VLFeat (oracle/_ref, OpenMP over all host threads) on whole
+    images until the time budget is used; parity of the GPU FP32 encoder
+    against it; error of the selected GPU encoder against the FP64 oracle."""
+    from oracle import oracle, reference
+    out = {}
     R = reference()
-    if R is None:
-        return None, None
-    R.set_num_threads(0)
-    threads = int(R.vl_get_max_threads())
-    t0 = time.perf_counter()
-    done_desc = 0
-    done_img = 0
-    encs = []
-    max_img = min(len(offsets) - 1, x_host.shape[0] // PER_IMAGE)
-    while time.perf_counter() - t0 < budget_s and done_img < max_img:
-        xs = x_host[offsets[done_img]:offsets[done_img + 1]]
-        e, _ = R.fisher(xs, m, c, p, FLAGS)
-        encs.append(e)
-        done_desc += xs.shape[0]
-        done_img += 1
-    dt = time.perf_counter() - t0
-    return ({"value": done_desc / dt, "unit": "descriptors/s", "cores": threads, "kind": "reference",
-             "sample": f"{done_img} images x {PER_IMAGE} descriptors (K={K_COMP}, D={DIM}, improved FV), "
-                       f"vl_fisher_encode per image, OpenMP inside the image, {dt:.1f} s"},
-            encs)
+    if R is not None:
+        R.set_num_threads(0)
+        threads = int(R.vl_get_max_threads())
+        t0 = time.perf_counter()
+        done_desc = 0
+        done_img = 0
+        encs = []
+        max_img = min(len(offsets) - 1, x_host.shape[0] // PER_IMAGE)
+        while time.perf_counter() - t0 < budget_s and done_img < max_img:
+            xs = x_host[offsets[done_img]:offsets[done_img + 1]]
+            e, _ = R.fisher(xs, m, c, p, FLAGS)
+            encs.append(e)
+            done_desc += xs.shape[0]
+            done_img += 1
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": done_desc / dt, "unit": "descriptors/s", "cores": threads, "kind": "reference",
+            "sample": f"{done_img} images x {PER_IMAGE} descriptors (K={K_COMP}, D={DIM}, improved FV), "
+                      f"vl_fisher_encode per image, OpenMP inside the image, {dt:.1f} s"}
+        nc = min(len(encs), gpu_fp32_enc.shape[0])
+        r = np.stack(encs[:nc]).astype(np.float64)
+        g = gpu_fp32_enc[:nc].astype(np.float64)
+        err = np.linalg.norm(g - r, axis=1) / np.linalg.norm(r, axis=1)
+        out["parity_fp32_vs_reference"] = {"images": nc, "max_rel_l2": float(err.max())}
+    O = oracle()
+    errs, errs32, errref = [], [], []
+    for i in range(min(n_oracle, len(offsets) - 1)):
+        xs = x_host[offsets[i]:offsets[i + 1]]
+        eo, _ = O.fisher(xs, m, c, p, FLAGS)
+        nrm = np.linalg.norm(eo)
+        errs.append(np.linalg.norm(gpu_enc[i].astype(np.float64) - eo) / nrm)
+        errs32.append(np.linalg.norm(gpu_fp32_enc[i].astype(np.float64) - eo) / nrm)
+        if R is not None and i < r.shape[0]:
+            errref.append(np.linalg.norm(r[i] - eo) / nrm)
+    out["error_vs_fp64_oracle"] = {
+        "images": len(errs), "selected_max_rel_l2": float(max(errs)), "fp32_max_rel_l2": float(max(errs32)),
+        "reference_float_max_rel_l2": float(max(errref)) if errref else None}
+    return out
+
+
+def traffic_per_descriptor(kernel: str):
+    """HBM bytes per descriptor of `kernel` from the committed PMC summary
+    (FETCH_SIZE and WRITE_SIZE passes scaled by a copy-kernel calibration), if any."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_k{K_COMP}_d{DIM}.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^hbm_bytes_per_descriptor:\s*([0-9.eE+]+)", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
 
 
 def main() -> int:
@@ -83,17 +134,20 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", default="fp32", choices=["fp32", "bf16x3", "f16x3", "gated"])
+    ap.add_argument("--mode", default="f16x3", choices=["fp32", "bf16x3", "f16x3", "gated"])
     ap.add_argument("--images", type=int, default=NUM_IMAGES)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--oracle-images", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 comparison run")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 1 GiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding (images split across "
-                          "GPUs, one all-reduce of EM statistics) is described in DESIGN.md but not built"}))
+        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU layout (images split across GPUs, "
+                          "one all-reduce of EM statistics per iteration) is described in DESIGN.md, not built"}))
         return 2
 
-    import ctypes as C
     import torch
     import vlfeat_amd as va
     from vlfeat_amd import synth
@@ -117,45 +171,56 @@ def main() -> int:
     pd = torch.as_tensor(p, device=dev)
     offsets = np.arange(n_img + 1, dtype=np.uint64) * PER_IMAGE
     od = torch.as_tensor(offsets.astype(np.int64), device=dev)
-    enc = torch.empty((n_img, 2 * K_COMP * DIM), dtype=torch.float32, device=dev)
-    terms = torch.zeros(n_img, dtype=torch.int64, device=dev)
-    mode = {"fp32": va.MODE_FP32, "bf16x3": va.MODE_BF16X3, "f16x3": va.MODE_F16X3, "gated": va.MODE_GATED}[args.mode]
+    modes = {"fp32": va.MODE_FP32, "bf16x3": va.MODE_BF16X3, "f16x3": va.MODE_F16X3, "gated": va.MODE_GATED}
     torch.cuda.synchronize()
 
-    def step():
-        L.vlhip_fisher_encode_batch(1, enc.data_ptr(), md.data_ptr(), cd.data_ptr(), pd.data_ptr(), DIM, K_COMP,
-                                    x.data_ptr(), od.data_ptr(), n_img, FLAGS, mode, terms.data_ptr())
+    def run(mode_name: str, steps: int, warmup: int):
+        enc = torch.empty((n_img, 2 * K_COMP * DIM), dtype=torch.float32, device=dev)
+        terms = torch.zeros(n_img, dtype=torch.int64, device=dev)
 
-    for i in range(args.warmup):
-        step()
+        def step():
+            L.vlhip_fisher_encode_batch(1, enc.data_ptr(), md.data_ptr(), cd.data_ptr(), pd.data_ptr(), DIM, K_COMP,
+                                        x.data_ptr(), od.data_ptr(), n_img, FLAGS, modes[mode_name],
+                                        terms.data_ptr())
+        for i in range(warmup):
+            step()
+            torch.cuda.synchronize()
+            log(f"{mode_name}: warmup step {i} done")
+        L.vlhip_timing_reset()
+        L.vlhip_timing_enable(1)
         torch.cuda.synchronize()
-        log(f"warmup step {i} done")
-    L.vlhip_timing_enable(1)
-    L.vlhip_timing_reset()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    L.vlhip_timing_enable(0)
-    timings = va.kernel_timings()
-    log(f"timed {args.steps} steps in {dt:.3f} s; kernels: " +
-        ", ".join(f"{k}={v[0] / max(v[1], 1):.3f}ms x{v[1]}" for k, v in timings.items()))
-    ms_per_step = dt * 1000 / args.steps
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        L.vlhip_timing_enable(0)
+        timings = va.kernel_timings()
+        log(f"{mode_name}: timed {steps} steps in {dt:.3f} s; kernels: " +
+            ", ".join(f"{k}={v[0] / max(v[1], 1):.3f}ms x{v[1]}" for k, v in timings.items()))
+        return dt * 1000 / steps, timings, enc, terms
+
+    ms_per_step, timings, enc, terms = run(args.mode, args.steps, args.warmup)
     value = N / (ms_per_step / 1000)
 
-    # roofline of the dominant kernel: algorithmic 4*D*K flop per descriptor for
-    # each of the E-step contraction and the posterior-weighted accumulation
-    kern = max(timings.items(), key=lambda kv: kv[1][0])
-    kname, (kms, kcnt) = kern
+    # roofline of the dominant kernel: algorithmic flop per launch / average launch time
+    kname, (kms, kcnt) = max(timings.items(), key=lambda kv: kv[1][0])
     avg_ms = kms / max(kcnt, 1)
-    flop_per_launch = 4.0 * DIM * K_COMP * N
+    launches_per_step = kcnt / args.steps
+    desc_per_launch = N / launches_per_step
+    flop_per_launch = KERNEL_FLOP_PER_DESC.get(kname, 8 * DIM * K_COMP) * desc_per_launch
     achieved = flop_per_launch / (avg_ms / 1000) / 1e12
-    peak = PEAK_FP32_TFLOPS if args.mode == "fp32" else PEAK_BF16_TFLOPS
+    peak = PEAK_FP32_TFLOPS if args.mode == "fp32" else PEAK_F16_TFLOPS
+    bpd = traffic_per_descriptor(kname)
     roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                "traffic": None, "kernel": kname, "kernel_avg_ms": avg_ms,
-                "kernels_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in timings.items()}}
+                "traffic": bpd * desc_per_launch if bpd else None, "kernel": kname, "kernel_avg_ms": avg_ms,
+                "launches_per_step": launches_per_step, "descriptors_per_launch": desc_per_launch,
+                "flop_per_descriptor": KERNEL_FLOP_PER_DESC.get(kname, 8 * DIM * K_COMP),
+                "kernels_ms_per_step": {k: v[0] / args.steps for k, v in timings.items() if v[1]}}
+    if kname in EXECUTED_PER_ALGORITHMIC:
+        ex = achieved * EXECUTED_PER_ALGORITHMIC[kname]
+        roofline["executed_matrix_tflops"] = ex
+        roofline["executed_frac"] = ex / peak
 
     t_host = terms.cpu().numpy()
     result = {
@@ -170,16 +235,31 @@ def main() -> int:
         "roofline": roofline,
         "terms_per_descriptor": float(t_host.sum()) / N,
     }
+    n_check = max(args.oracle_images, 64 if not args.no_cpu else 0)
+    enc_sel = enc[: min(n_img, n_check)].cpu().numpy()
+    del enc
+    enc32 = None
+    if args.mode != "fp32" and not args.no_fp32:
+        ms32, t32, e32, _ = run("fp32", 1, 1)
+        result["fp32_gpu"] = {"value": N / (ms32 / 1000), "ms_per_step": ms32,
+                              "kernels_ms_per_step": {k: v[0] for k, v in t32.items() if v[1]}}
+        result["speedup_vs_fp32_gpu"] = ms32 / ms_per_step
+        enc32 = e32[: min(n_img, n_check)].cpu().numpy()
+        del e32
+    elif args.mode == "fp32":
+        enc32 = enc_sel
 
-    if not args.no_cpu:
-        x_host = x[: min(N, 64 * PER_IMAGE)].cpu().numpy()
-        cb, encs = cpu_baseline(x_host, offsets, m, c, p, args.cpu_budget)
-        result["cpu_baseline"] = cb
-        if encs:
-            g = enc[: len(encs)].cpu().numpy().astype(np.float64)
-            r = np.stack(encs).astype(np.float64)
-            err = np.linalg.norm(g - r, axis=1) / np.linalg.norm(r, axis=1)
-            result["parity_vs_reference"] = {"images": len(encs), "max_rel_l2": float(err.max())}
+    if not args.no_cpu and enc32 is not None:
+        x_host = x[: min(N, 1024 * PER_IMAGE)].cpu().numpy()
+        result.update(cpu_legs(x_host, offsets, m, c, p, args.cpu_budget, enc_sel, enc32, args.oracle_images))
+        result["cpu_baseline"] = result.get("cpu_baseline")
+
+    if args.copy_probe:
+        a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy probe: 1 GiB read + 1 GiB written by one copy kernel")
     print(json.dumps(result), flush=True)
     return 0
 
