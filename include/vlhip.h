@@ -135,4 +135,37 @@ VLHIP_API void vlhip_kmeans_quantize (int dtype, uint32_t * assignments, void * 
 VLHIP_API void vlhip_l2_to_point (int dtype, void * distances, void const * data, void const * point,
                                   size_t numData, size_t dimension, int lanes) ;
 
+
+/* ---- learned gate of the Fisher encoder (no counterpart in the reference) --
+ * A perceptron dimension -> hidden1 -> hidden2 -> numClusters with GELU
+ * activations (BF16 matrix products, FP32 master weights) trained on the GPU
+ * to predict the exact posteriors of a GMM (soft-target cross-entropy, Adam).
+ * Parameters are laid out [W1 (D x H1) | b1 | W2 (H1 x H2) | b2 | W3 (H2 x K)
+ * | b3], weights row-major with the input index first; the input is
+ * standardised per dimension, (x - mean) * istd. */
+VLHIP_API void * vlhip_gate_new (size_t dimension, size_t numClusters, size_t hidden1, size_t hidden2,
+                                 unsigned long long seed) ;
+VLHIP_API void   vlhip_gate_delete (void * gate) ;
+VLHIP_API size_t vlhip_gate_num_params (void * gate) ;
+VLHIP_API void   vlhip_gate_get_params (void * gate, float * params_host, float * mean_host, float * istd_host) ;
+VLHIP_API void   vlhip_gate_set_params (void * gate, float const * params_host, float const * mean_host,
+                                        float const * istd_host) ;
+/* mean and 1 / (unbiased standard deviation) of numData FP32 descriptors */
+VLHIP_API void   vlhip_gate_set_standardization_from_data (void * gate, void const * data, size_t numData) ;
+/* mean loss of one batch (FP32 data rows x D, FP32 targets rows x K) and, when
+ * grads_host is not NULL, its gradient (vlhip_gate_num_params floats) */
+VLHIP_API double vlhip_gate_gradients (void * gate, void const * data, void const * targets, size_t rows,
+                                       float * grads_host) ;
+/* steps Adam steps (lr, betas 0.9 / 0.999) on consecutive batches of numData
+ * FP32 descriptors against the GMM's exact posteriors (gmm.c:711-798);
+ * returns the last batch's loss, loss_trace_host (optional) every step's */
+VLHIP_API double vlhip_gate_train (void * gate, void const * data, size_t numData, void const * means,
+                                   void const * covariances, void const * priors, size_t steps, size_t batch,
+                                   float lr, float * loss_trace_host) ;
+/* per descriptor: the M most probable components of the gate (idx, rows x M,
+ * -1 when screened out), their predicted mass (mass), and conf = mass >= need */
+VLHIP_API void   vlhip_gate_select (void * gate, void const * data, size_t rows, int M, float need, int * idx,
+                                    float * mass, unsigned char * conf) ;
+VLHIP_API void   vlhip_gate_logits (void * gate, void const * data, size_t rows, float * logits_host) ;
+
 #endif /* VLHIP_H */

@@ -20,6 +20,8 @@ Pinning of each oracle routine (what checks it beyond its own formula):
   or_fisher_encode    vl_test_fisher.m's five closed-form variants
   or_vlad_encode      vl_test_vlad.m's known answers and closed forms
   or_kmeans_quantize  brute force (numpy argmin)
+  or_mlp_loss_grads   central finite differences of its own loss; softmax
+                      cross-entropy identities (uniform logits, one-hot targets)
 """
 from __future__ import annotations
 
@@ -63,6 +65,7 @@ class Oracle:
             "or_fisher_encode": (_sz, [_p, _p, _sz, _sz, _p, _p, _p, _sz, C.c_int]),
             "or_vlad_encode": (None, [_p, _p, _sz, _sz, _p, _sz, _p, C.c_int]),
             "or_kmeans_quantize": (None, [_p, _p, _p, _p, _sz, _sz, _sz]),
+            "or_mlp_loss_grads": (C.c_double, [_p, _sz, _sz, _sz, _sz, _p, _p, _sz, _p]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -112,6 +115,14 @@ class Oracle:
         self.L.or_vlad_encode(enc.ctypes.data, means.ctypes.data, D, K, x.ctypes.data, x.shape[0],
                               assign.ctypes.data, flags)
         return enc
+
+    def mlp_loss_grads(self, params, D, H1, H2, K, xin, q, want_grads=True):
+        """Mean soft-target cross-entropy of the gate perceptron and its gradient (FP64)."""
+        params, xin, q = _f64(params), _f64(xin), _f64(q)
+        g = np.empty(params.shape[0]) if want_grads else None
+        loss = self.L.or_mlp_loss_grads(params.ctypes.data, D, H1, H2, K, xin.ctypes.data, q.ctypes.data,
+                                        xin.shape[0], None if g is None else g.ctypes.data)
+        return loss, g
 
     def quantize(self, x, centers):
         x, centers = _f64(x), _f64(centers)

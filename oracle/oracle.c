@@ -310,3 +310,95 @@ or_kmeans_quantize (uint32_t * assign, double * dist, double const * x, double c
     if (dist) dist[i] = best ;
   }
 }
+
+/* ------------------------------------------------------------------------- */
+/* Learned gate (no counterpart in the reference): the perceptron
+ *   a1 = x W1 + b1, h1 = gelu(a1), a2 = h1 W2 + b2, h2 = gelu(a2), z = h2 W3 + b3,
+ *   gelu(a) = a Phi(a) = a (1 + erf(a / sqrt 2)) / 2,
+ * loss = mean over rows of -sum_k q_k log softmax(z)_k, and its gradient with
+ * respect to the parameters laid out [W1 (D x H1) | b1 | W2 (H1 x H2) | b2 |
+ * W3 (H2 x K) | b3] (weights row-major, input index first).  FP64 throughout;
+ * xin is the already standardised input.  Returns the mean loss; grads may be
+ * NULL. */
+static double or_gelu (double a) { return 0.5 * a * (1.0 + erf(a / sqrt(2.0))) ; }
+static double or_gelu_grad (double a)
+{
+  return 0.5 * (1.0 + erf(a / sqrt(2.0))) + a * exp(-0.5 * a * a) * 0.39894228040143267794 ; /* 1 / sqrt(2 pi) */
+}
+
+OR_API double
+or_mlp_loss_grads (double const * params, size_t D, size_t H1, size_t H2, size_t K,
+                   double const * xin, double const * q, size_t rows, double * grads)
+{
+  double const * W1 = params ;
+  double const * b1 = W1 + D * H1 ;
+  double const * W2 = b1 + H1 ;
+  double const * b2 = W2 + H1 * H2 ;
+  double const * W3 = b2 + H2 ;
+  double const * b3 = W3 + H2 * K ;
+  size_t nP = D * H1 + H1 + H1 * H2 + H2 + H2 * K + K ;
+  double * a1 = malloc(sizeof(double) * H1) ;
+  double * h1 = malloc(sizeof(double) * H1) ;
+  double * a2 = malloc(sizeof(double) * H2) ;
+  double * h2 = malloc(sizeof(double) * H2) ;
+  double * z = malloc(sizeof(double) * K) ;
+  double * dz = malloc(sizeof(double) * K) ;
+  double * dh2 = malloc(sizeof(double) * H2) ;
+  double * dh1 = malloc(sizeof(double) * H1) ;
+  double loss = 0 ;
+  size_t i, j, k, d ;
+  if (grads) memset(grads, 0, sizeof(double) * nP) ;
+  for (i = 0 ; i < rows ; ++i) {
+    double const * x = xin + i * D ;
+    double const * qi = q + i * K ;
+    double m = -INFINITY, s = 0, ls ;
+    for (j = 0 ; j < H1 ; ++j) {
+      double t = b1[j] ;
+      for (d = 0 ; d < D ; ++d) t += x[d] * W1[d * H1 + j] ;
+      a1[j] = t ; h1[j] = or_gelu(t) ;
+    }
+    for (j = 0 ; j < H2 ; ++j) {
+      double t = b2[j] ;
+      for (d = 0 ; d < H1 ; ++d) t += h1[d] * W2[d * H2 + j] ;
+      a2[j] = t ; h2[j] = or_gelu(t) ;
+    }
+    for (k = 0 ; k < K ; ++k) {
+      double t = b3[k] ;
+      for (d = 0 ; d < H2 ; ++d) t += h2[d] * W3[d * K + k] ;
+      z[k] = t ; if (t > m) m = t ;
+    }
+    for (k = 0 ; k < K ; ++k) s += exp(z[k] - m) ;
+    ls = log(s) ;
+    for (k = 0 ; k < K ; ++k) {
+      double lp = z[k] - m - ls ;
+      if (qi[k] > 0) loss -= qi[k] * lp ;
+      dz[k] = (exp(lp) - qi[k]) / (double)rows ;
+    }
+    if (!grads) continue ;
+    {
+      double * gW1 = grads ;
+      double * gb1 = gW1 + D * H1 ;
+      double * gW2 = gb1 + H1 ;
+      double * gb2 = gW2 + H1 * H2 ;
+      double * gW3 = gb2 + H2 ;
+      double * gb3 = gW3 + H2 * K ;
+      for (k = 0 ; k < K ; ++k) gb3[k] += dz[k] ;
+      for (d = 0 ; d < H2 ; ++d) {
+        double t = 0 ;
+        for (k = 0 ; k < K ; ++k) { gW3[d * K + k] += h2[d] * dz[k] ; t += dz[k] * W3[d * K + k] ; }
+        dh2[d] = t * or_gelu_grad(a2[d]) ;
+      }
+      for (j = 0 ; j < H2 ; ++j) gb2[j] += dh2[j] ;
+      for (d = 0 ; d < H1 ; ++d) {
+        double t = 0 ;
+        for (j = 0 ; j < H2 ; ++j) { gW2[d * H2 + j] += h1[d] * dh2[j] ; t += dh2[j] * W2[d * H2 + j] ; }
+        dh1[d] = t * or_gelu_grad(a1[d]) ;
+      }
+      for (j = 0 ; j < H1 ; ++j) gb1[j] += dh1[j] ;
+      for (d = 0 ; d < D ; ++d)
+        for (j = 0 ; j < H1 ; ++j) gW1[d * H1 + j] += x[d] * dh1[j] ;
+    }
+  }
+  free(a1) ; free(h1) ; free(a2) ; free(h2) ; free(z) ; free(dz) ; free(dh2) ; free(dh1) ;
+  return rows ? loss / (double)rows : 0.0 ;
+}

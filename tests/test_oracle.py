@@ -209,3 +209,50 @@ def test_planted_sign_error_is_caught(fisher_case):
     q, _ = oracle().posteriors(x.T, mu.T, s2.T, pri)
     bad[K * D:] += (q.sum(0) / x.shape[1] / np.sqrt(2 * pri)).repeat(D)
     assert not np.allclose(bad, simple_fisher(x, mu, s2, pri), atol=1e-6)
+
+
+# ---- learned-gate perceptron (or_mlp_loss_grads) -------------------------------
+
+def _mlp_case(D=5, H1=7, H2=6, K=9, rows=4, seed=0):
+    rng = np.random.default_rng(seed)
+    n = D * H1 + H1 + H1 * H2 + H2 + H2 * K + K
+    params = rng.normal(0, 0.5, n)
+    xin = rng.normal(0, 1, (rows, D))
+    q = rng.dirichlet(np.ones(K), rows)
+    return params, xin, q, (D, H1, H2, K)
+
+
+def test_mlp_oracle_matches_finite_differences():
+    params, xin, q, (D, H1, H2, K) = _mlp_case()
+    O = oracle()
+    loss, g = O.mlp_loss_grads(params, D, H1, H2, K, xin, q)
+    eps = 1e-6
+    fd = np.empty_like(params)
+    for i in range(params.size):
+        p1 = params.copy(); p1[i] += eps
+        p2 = params.copy(); p2[i] -= eps
+        fd[i] = (O.mlp_loss_grads(p1, D, H1, H2, K, xin, q, False)[0]
+                 - O.mlp_loss_grads(p2, D, H1, H2, K, xin, q, False)[0]) / (2 * eps)
+    assert np.linalg.norm(g - fd) / np.linalg.norm(fd) < 1e-7
+    # a plausible slip (GELU derivative without the a*phi(a) term) would not pass
+    assert np.abs(g - fd).max() < 1e-7
+
+
+def test_mlp_oracle_softmax_identities():
+    D, H1, H2, K, rows = 4, 3, 5, 8, 6
+    n = D * H1 + H1 + H1 * H2 + H2 + H2 * K + K
+    rng = np.random.default_rng(1)
+    xin = rng.normal(0, 1, (rows, D))
+    q = rng.dirichlet(np.ones(K), rows)
+    O = oracle()
+    # zero network: uniform prediction, loss = log K, dL/db3 = 1/K - mean q
+    loss, g = O.mlp_loss_grads(np.zeros(n), D, H1, H2, K, xin, q)
+    assert abs(loss - np.log(K)) < 1e-12
+    np.testing.assert_allclose(g[-K:], 1.0 / K - q.mean(0), atol=1e-14)
+    # a large output bias on the target class of one-hot rows drives the loss to ~0
+    p = np.zeros(n)
+    p[-K:] = 0.0
+    p[-K + 3] = 40.0
+    oh = np.zeros((rows, K)); oh[:, 3] = 1
+    loss, _ = O.mlp_loss_grads(p, D, H1, H2, K, xin, oh, False)
+    assert loss < 1e-15

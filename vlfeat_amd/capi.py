@@ -132,6 +132,16 @@ _EXT_SIGS = {
     "vlhip_vlad_encode_batch": (None, [C.c_int, _p, _p, C.c_size_t, C.c_size_t, _p, _p, C.c_size_t,
                                        _p, _p, C.c_int]),
     "vlhip_kmeans_quantize": (None, [C.c_int, _p, _p, _p, _p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_int]),
+    "vlhip_gate_new": (_p, [_sz, _sz, _sz, _sz, C.c_ulonglong]),
+    "vlhip_gate_delete": (None, [_p]),
+    "vlhip_gate_num_params": (_sz, [_p]),
+    "vlhip_gate_get_params": (None, [_p, _p, _p, _p]),
+    "vlhip_gate_set_params": (None, [_p, _p, _p, _p]),
+    "vlhip_gate_set_standardization_from_data": (None, [_p, _p, _sz]),
+    "vlhip_gate_gradients": (C.c_double, [_p, _p, _p, _sz, _p]),
+    "vlhip_gate_train": (C.c_double, [_p, _p, _sz, _p, _p, _p, _sz, _sz, C.c_float, _p]),
+    "vlhip_gate_select": (None, [_p, _p, _sz, C.c_int, C.c_float, _p, _p, _p]),
+    "vlhip_gate_logits": (None, [_p, _p, _sz, _p]),
 }
 
 
