@@ -47,8 +47,18 @@ VL_EXPORT void vl_fisher_encode_batch
  * order 1e-5 for FP16, 1e-4 for BF16).  Double data always uses FP64. */
 #define VL_FISHER_PRECISION_FP32     0
 #define VL_FISHER_PRECISION_BF16X3   1
+#define VL_FISHER_PRECISION_GATED    2
 #define VL_FISHER_PRECISION_F16X3    3
 VL_EXPORT void vl_fisher_set_precision (int precision) ;
 VL_EXPORT int vl_fisher_get_precision (void) ;
+
+/* Learned gate of VL_FISHER_PRECISION_GATED (no counterpart in the
+ * reference): a handle from vlhip_gate_new (vlhip.h), trained on the GMM's
+ * posteriors.  Descriptors are taken in groups of 32 per image; a group whose
+ * every descriptor has predicted mass >= need on the gate's `candidates`
+ * (<= 16) most probable components gets exact posteriors over those
+ * candidates only; all other groups use the dense F16X3 E-step.  The gate is
+ * shared by all threads of the process; NULL clears it. */
+VL_EXPORT void vl_fisher_set_gate (void * gate, int candidates, float need) ;
 
 #endif /* VL_FISHER_H */

@@ -109,6 +109,17 @@ VLHIP_API void vlhip_fisher_encode_batch (int dtype, void * enc,
                                           void const * data, uint64_t const * offsets, size_t numImages,
                                           int flags, int mode, unsigned long long * numTerms) ;
 
+/* Gate used by VLHIP_MODE_GATED (a vlhip_gate_new handle, NULL to clear):
+ * per descriptor its `candidates` (<= 16) most probable components; groups of
+ * 32 consecutive descriptors of an image whose predicted mass on those
+ * candidates reaches `need` for every descriptor get exact posteriors over the
+ * candidates only (normalised over them), all others go through the dense
+ * compensated FP16 E-step.  No counterpart in the reference. */
+VLHIP_API void vlhip_fisher_set_gate (void * gate, int candidates, float need) ;
+/* Groups of 32 descriptors seen by VLHIP_MODE_GATED encodings since the last
+ * reset, and how many of them the gate served (host values). */
+VLHIP_API void vlhip_fisher_gate_stats (unsigned long long * groups, unsigned long long * served, int reset) ;
+
 /* ---- VLAD (vl/vlad.c) ---------------------------------------------------- */
 
 /* VLAD of numImages images, vlad.c:163-242 per image.  Exactly one of

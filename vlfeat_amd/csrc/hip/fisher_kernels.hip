@@ -274,6 +274,23 @@ void fisher_fp_t(T* enc, const T* means, const T* covs, const T* priors, size_t 
 
 using namespace vlhip;
 
+extern "C" void vlhip_fisher_set_gate(void* gate, int candidates, float need) {
+  ensure_init();
+  std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+  VLHIP_REQUIRE(gate == nullptr || (candidates >= 1 && candidates <= 16), "gate candidates must be in [1, 16]");
+  ctx().gate = gate;
+  ctx().gate_m = candidates;
+  ctx().gate_need = need;
+}
+
+extern "C" void vlhip_fisher_gate_stats(unsigned long long* groups, unsigned long long* served, int reset) {
+  ensure_init();
+  std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+  if (groups) *groups = ctx().gate_groups;
+  if (served) *served = ctx().gate_served;
+  if (reset) ctx().gate_groups = ctx().gate_served = 0;
+}
+
 extern "C" void vlhip_fisher_encode_batch(int dtype, void* enc, void const* means, void const* covs,
                                           void const* priors, size_t D, size_t K, void const* data,
                                           uint64_t const* offsets, size_t numImages, int flags, int mode,
@@ -292,7 +309,12 @@ extern "C" void vlhip_fisher_encode_batch(int dtype, void* enc, void const* mean
                        (const float*)data, offsets, numImages, flags, numTerms);
   } else if (mode == VLHIP_MODE_BF16X3 || mode == VLHIP_MODE_F16X3) {
     fisher_split16((float*)enc, (const float*)means, (const float*)covs, (const float*)priors, D, K,
-                   (const float*)data, offsets, numImages, flags, numTerms, mode == VLHIP_MODE_F16X3);
+                   (const float*)data, offsets, numImages, flags, numTerms, mode == VLHIP_MODE_F16X3, nullptr);
+  } else if (mode == VLHIP_MODE_GATED) {
+    VLHIP_REQUIRE(ctx().gate != nullptr, "VLHIP_MODE_GATED needs a gate (vlhip_fisher_set_gate)");
+    const GateUse gu{ctx().gate, ctx().gate_m, ctx().gate_need};
+    fisher_split16((float*)enc, (const float*)means, (const float*)covs, (const float*)priors, D, K,
+                   (const float*)data, offsets, numImages, flags, numTerms, true, &gu);
   } else {
     fatal("vlhip_fisher_encode_batch", __FILE__, __LINE__, "unknown mode");
   }

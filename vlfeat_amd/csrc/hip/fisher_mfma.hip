@@ -245,9 +245,10 @@ __device__ __forceinline__ double block_sum_waves(double v, double* red) {
 // Weight staging of the first pass: STG 16-byte words per thread per block.
 template <typename H, int STG>
 __device__ __forceinline__ void w_load(u32x4 (&pre)[STG], const H* Whi, const H* Wlo, int cb, int Dp,
-                                       const bool (&lo)[STG], const int (&off)[STG]) {
+                                       unsigned loMask, const int (&off)[STG]) {
 #pragma unroll
-  for (int q = 0; q < STG; ++q) pre[q] = *reinterpret_cast<const u32x4*>((lo[q] ? Wlo : Whi) + (size_t)cb * 2 * Dp + off[q]);
+  for (int q = 0; q < STG; ++q)
+    pre[q] = *reinterpret_cast<const u32x4*>((((loMask >> q) & 1) ? Wlo : Whi) + (size_t)cb * 2 * Dp + off[q]);
 }
 template <typename H, int STG>
 __device__ __forceinline__ void w_store(const u32x4 (&pre)[STG], H* base, const int (&off)[STG]) {
@@ -256,10 +257,11 @@ __device__ __forceinline__ void w_store(const u32x4 (&pre)[STG], H* base, const 
 }
 
 // Group table of a batch: group gstart[i] - gstart[imgBegin] + k of image i
-// starts at batch-relative row offsets[i] - offsets[imgBegin] + 32k, and its
-// image ends at offsets[i+1] - offsets[imgBegin].  One thread per image.
+// starts at batch-relative row offsets[i] - offsets[imgBegin] + 32k, its image
+// ends at offsets[i+1] - offsets[imgBegin]; entries {first row, image end,
+// image index, 0}.  One thread per image.
 __global__ void mfma_group_table_kernel(const uint64_t* __restrict__ offsets, const long long* __restrict__ gstart,
-                                        int imgBegin, int numImages, int2* __restrict__ tab) {
+                                        int imgBegin, int numImages, int4* __restrict__ tab) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= numImages) return;
   const int img = imgBegin + i;
@@ -267,7 +269,7 @@ __global__ void mfma_group_table_kernel(const uint64_t* __restrict__ offsets, co
   const int r0 = (int)((long long)offsets[img] - rb), r1 = (int)((long long)offsets[img + 1] - rb);
   const long long g0 = gstart[img] - gstart[imgBegin];
   const int ng = (int)(gstart[img + 1] - gstart[img]);
-  for (int k = 0; k < ng; ++k) tab[g0 + k] = make_int2(r0 + GR * k, r1);
+  for (int k = 0; k < ng; ++k) tab[g0 + k] = make_int4(r0 + GR * k, r1, img, 0);
 }
 
 // Running log-sum-exp (log2 units) and first maximum of the lane's 8 rows.
@@ -422,7 +424,8 @@ __global__ __launch_bounds__(ML_GROUPS * 64, 1) void mfma_lse_kernel(
     const float* __restrict__ x, int D, int Kp,
     const H* __restrict__ Whi, const H* __restrict__ Wlo, const float* __restrict__ bias,
     const MfmaScales* __restrict__ scp, const float* __restrict__ g,
-    unsigned char* __restrict__ rec, long long numGroups, const int2* __restrict__ gtab) {
+    unsigned char* __restrict__ rec, long long numGroups, const int4* __restrict__ gtab,
+    const unsigned char* __restrict__ gflag) {
   typedef typename VecOf<H>::type V8;
   typedef GroupRec<NS> R;
   constexpr int Dp = NS * 16;
@@ -440,7 +443,7 @@ __global__ __launch_bounds__(ML_GROUPS * 64, 1) void mfma_lse_kernel(
   // batch-relative first row of the group and end of its image
   long long row0 = 0, rowEnd = 0;
   if (live) {
-    const int2 gr = gtab[grp];
+    const int4 gr = gtab[grp];
     row0 = gr.x;
     rowEnd = gr.y;
   }
@@ -451,13 +454,13 @@ __global__ __launch_bounds__(ML_GROUPS * 64, 1) void mfma_lse_kernel(
 
   // staging map of this thread (the same for every block)
   int srcOff[STG], dstOff[STG];
-  bool srcLo[STG];
+  unsigned srcLo = 0;  // bit q: word q comes from the lo part
 #pragma unroll
   for (int q = 0; q < STG; ++q) {
     const int i = tid + NT * q;
     const int part = i / PART, rem = i - part * PART;
     const int c = rem / (2 * Dp / 8), q8 = rem - c * (2 * Dp / 8);
-    srcLo[q] = part != 0;
+    srcLo |= (part != 0 ? 1u : 0u) << q;
     srcOff[q] = c * 2 * Dp + 8 * q8;
     dstOff[q] = (part * MB_COMPS + c) * WS + 8 * q8;
   }
@@ -533,12 +536,16 @@ __global__ __launch_bounds__(ML_GROUPS * 64, 1) void mfma_lse_kernel(
   // Software pipeline over the component blocks: the matrix products of block
   // b are issued together with the log-sum-exp update of block b - 1, so the
   // vector work overlaps the matrix cores; one barrier per block.
+  // groups the gate serves sparsely need no row statistics; a workgroup whose
+  // groups are all sparse skips the sweep
+  const bool need = live && !(gflag != nullptr && gflag[grp] != 0);
+  if (!__syncthreads_or(need)) return;
   const int nb = Kp / MB_COMPS;
   f32x4 accA[4][2], accB[4][2];
   w_store<H, STG>(wpre, &wsh[0][0][0][0], dstOff);
   if (nb > 1) w_load<H, STG>(wpre, Whi, Wlo, MB_COMPS, Dp, srcLo, srcOff);
   __syncthreads();
-  lse_mfma<H, NS>(accA, ahi, alo, &wsh[0][0][0][0], lr, lg);
+  if (need) lse_mfma<H, NS>(accA, ahi, alo, &wsh[0][0][0][0], lr, lg);
   if (nb > 1) {
     w_store<H, STG>(wpre, &wsh[1][0][0][0], dstOff);
     if (nb > 2) w_load<H, STG>(wpre, Whi, Wlo, 2 * MB_COMPS, Dp, srcLo, srcOff);
@@ -553,8 +560,9 @@ __global__ __launch_bounds__(ML_GROUPS * 64, 1) void mfma_lse_kernel(
       w_store<H, STG>(wpre, &wsh[(b_ + 1) & 1][0][0][0], dstOff);                                     \
       if (b_ + 2 < nb) w_load<H, STG>(wpre, Whi, Wlo, (b_ + 2) * MB_COMPS, Dp, srcLo, srcOff);         \
     }                                                                                                 \
-    lse_fused<H, NS, ARG>(ACC_NEW, ACC_OLD, ahi, alo, &wsh[b_ & 1][0][0][0], lr, lg, bias,            \
-                          (b_ - 1) * MB_COMPS, st);                                                   \
+    if (need)                                                                                         \
+      lse_fused<H, NS, ARG>(ACC_NEW, ACC_OLD, ahi, alo, &wsh[b_ & 1][0][0][0], lr, lg, bias,          \
+                            (b_ - 1) * MB_COMPS, st);                                                 \
     __syncthreads();                                                                                  \
   }
   int b = 1;
@@ -564,9 +572,9 @@ __global__ __launch_bounds__(ML_GROUPS * 64, 1) void mfma_lse_kernel(
   }
   if (b < nb) {
     LSE_STEP(accB, accA, b)
-    lse_update<ARG>(accB, bias, (nb - 1) * MB_COMPS, lr, st);
+    if (need) lse_update<ARG>(accB, bias, (nb - 1) * MB_COMPS, lr, st);
   } else {
-    lse_update<ARG>(accA, bias, (nb - 1) * MB_COMPS, lr, st);
+    if (need) lse_update<ARG>(accA, bias, (nb - 1) * MB_COMPS, lr, st);
   }
 #undef LSE_STEP
 
@@ -594,13 +602,91 @@ __global__ __launch_bounds__(ML_GROUPS * 64, 1) void mfma_lse_kernel(
         }
       }
       const int row = 16 * t + 4 * lg + r;
-      if (live && lr == 0) {
+      if (need && lr == 0) {
         const bool pad = row0 + row >= rowEnd;
         Sf[row] = pad ? 0.f : -m;
         Sf[GR + row] = pad ? 0.f : sq / s;
         if (ARG) Sf[2 * GR + row] = __int_as_float(pad ? -1 : bi);
       }
     }
+}
+
+// Gated groups.  One wave per group: the group is served sparsely when the
+// gate is confident for every one of its rows; then, for each row, the exact
+// logits of its M <= 16 candidate components (the same FP32 tables as the
+// dense path: bias + log2 e * sum_d sigma^-2 (mu' x' - x'^2 / 2)) are
+// normalised over the candidates, thresholded at 1e-6 (fisher.c:436-441; FAST:
+// one-hot on the first largest), scaled by sq and listed as
+// {component, q * sq} in list[group][32][M] (padding rows: {-1, 0}).  The
+// terms are counted into numTerms of the group's image.
+template <int NS>
+__global__ __launch_bounds__(256) void mfma_sparse_kernel(
+    const float* __restrict__ x, int D, long long numGroups, const int4* __restrict__ gtab,
+    const unsigned char* __restrict__ conf, const int* __restrict__ cand, int M, const float* __restrict__ bias,
+    const float* __restrict__ mup, const float* __restrict__ sinv, const float* __restrict__ g,
+    const MfmaScales* __restrict__ scp, int flags, unsigned char* __restrict__ gflag, int2* __restrict__ list,
+    unsigned long long* __restrict__ numTerms) {
+  constexpr int Dp = NS * 16;
+  const int lane = threadIdx.x & 63;
+  const long long grp = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (grp >= numGroups) return;
+  const int4 gr = gtab[grp];
+  const long long row0 = gr.x;
+  const int nrow = min(GR, gr.y - gr.x);
+  const bool rowOk = lane >= nrow || conf[row0 + lane] != 0;
+  const bool sparse = __all(rowOk) != 0;
+  if (lane == 0) gflag[grp] = sparse ? 1 : 0;
+  if (!sparse) return;
+  const bool fast = (flags & 4) != 0;
+  const float sq = scp->sq;
+  const float qmin = __uint_as_float(0x358637beu);  // next float above float(1e-6)
+  const float kL2e = 1.4426950408889634f;
+  const int j = lane & 15, sub = lane >> 4;
+  unsigned terms = 0;
+  for (int it = 0; it < GR / 4; ++it) {
+    const int r = 4 * it + sub;
+    const bool valid = r < nrow && j < M;
+    const int comp = valid ? cand[(row0 + r) * M + j] : -1;
+    float l = -INFINITY;
+    if (valid && comp >= 0) {
+      const float* xr = x + (row0 + r) * D;
+      const float* mu = mup + (size_t)comp * Dp;
+      const float* si = sinv + (size_t)comp * Dp;
+      float acc = 0.f;
+      for (int d = 0; d < D; ++d) {
+        const float xc = xr[d] - g[d];
+        acc += si[d] * (mu[d] * xc - 0.5f * xc * xc);
+      }
+      l = bias[comp] + kL2e * acc;
+    }
+    // normalise over the candidates of the row (lanes 16 sub .. 16 sub + 15)
+    float m = l;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    const float e = (l == -INFINITY) ? 0.f : __builtin_amdgcn_exp2f(l - m);
+    float s = e;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) s += __shfl_xor(s, o, 64);
+    float q = s > 0.f ? e / s : 0.f;
+    if (fast) {
+      int best = (l == m && l != -INFINITY) ? comp : 0x7fffffff;
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) best = min(best, __shfl_xor(best, o, 64));
+      q = (comp == best) ? 1.f : 0.f;
+    }
+    float v = 0.f;
+    if (valid && comp >= 0 && q >= qmin) {
+      v = q * sq;
+      terms += 1;
+    }
+    if (j < M) list[((size_t)grp * GR + r) * M + j] = make_int2(valid ? comp : -1, __float_as_int(v));
+  }
+  if (numTerms) {
+    unsigned t = terms;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if (lane == 0 && t) atomicAdd(&numTerms[gr.z], (unsigned long long)t);
+  }
 }
 
 // XCD-aware block mapping: workgroups are dealt to the 8 XCDs round robin, so
@@ -657,7 +743,8 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
     int imgBegin, int numImages, int D, int K, const H* __restrict__ Whi, const H* __restrict__ Wlo,
     const float* __restrict__ bias, const MfmaScales* __restrict__ scp, const float* __restrict__ mup,
     const float* __restrict__ isg, const float* __restrict__ sinv, const T* __restrict__ priors, int flags,
-    T* __restrict__ enc, double* __restrict__ normpart, unsigned long long* __restrict__ numTerms, int numKB) {
+    T* __restrict__ enc, double* __restrict__ normpart, unsigned long long* __restrict__ numTerms, int numKB,
+    const unsigned char* __restrict__ gflag, const int2* __restrict__ list, int M) {
   typedef typename VecOf<H>::type V8;
   typedef GroupRec<NS> R;
   constexpr int Dp = NS * 16;
@@ -730,73 +817,96 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
     const unsigned char* cur = smem_raw + (size_t)buf * R::kBytes;
     const V8* Ap = reinterpret_cast<const V8*>(cur);
     const V8* Bp = reinterpret_cast<const V8*>(cur + R::kA);
-    const float* Sf = reinterpret_cast<const float*>(cur + R::kA + R::kB);
-    // rows 16t + 4lg + r of this lane: -max, sq / sum
-    f32x4 nm4[2], rs4[2];
+    V8 aq[CT];
+    if (gflag != nullptr && gflag[gi] != 0) {
+      // gate-served group: scatter its listed posteriors into a rows x BC tile
+      float* tile = reinterpret_cast<float*>(smem_raw + 2 * R::kBytes);
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      nm4[t] = *reinterpret_cast<const f32x4*>(&Sf[16 * t + 4 * lg]);
-      rs4[t] = *reinterpret_cast<const f32x4*>(&Sf[GR + 16 * t + 4 * lg]);
-    }
-
-    // ---- logits - max (the accumulators start at bias - max)
-    f32x4 acc[2][CT];
+      for (int q = 0; q < (GR * BC / 4) / NT; ++q)
+        reinterpret_cast<f32x4*>(tile)[tid + NT * q] = f32x4{0.f, 0.f, 0.f, 0.f};
+      __syncthreads();
+      const int2* L = list + (size_t)gi * GR * M;
+      for (int e = tid; e < GR * M; e += NT) {
+        const int2 en = L[e];
+        const int c = en.x - kb * BC;
+        if (c >= 0 && c < BC) tile[(e / M) * BC + c] = __int_as_float(en.y);
+      }
+      __syncthreads();
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+      for (int t = 0; t < 2; ++t)
 #pragma unroll
-      for (int c = 0; c < CT; ++c)
-        acc[t][c] = f32x4{bc[c] + nm4[t][0], bc[c] + nm4[t][1], bc[c] + nm4[t][2], bc[c] + nm4[t][3]};
+        for (int r = 0; r < 4; ++r)
 #pragma unroll
-    for (int j = 0; j < NS; ++j) {
-      V8 ah[2], al[2];
+          for (int c = 0; c < CT; ++c)
+            aq[c][4 * t + r] = H(tile[(16 * t + 4 * lg + r) * BC + 16 * CT * w + 16 * c + lr]);
+    } else {
+      const float* Sf = reinterpret_cast<const float*>(cur + R::kA + R::kB);
+      // rows 16t + 4lg + r of this lane: -max, sq / sum
+      f32x4 nm4[2], rs4[2];
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
-        ah[t] = Ap[((t * NS + j) * 2 + 0) * 64 + lane];
-        al[t] = Ap[((t * NS + j) * 2 + 1) * 64 + lane];
+        nm4[t] = *reinterpret_cast<const f32x4*>(&Sf[16 * t + 4 * lg]);
+        rs4[t] = *reinterpret_cast<const f32x4*>(&Sf[GR + 16 * t + 4 * lg]);
       }
-#pragma unroll
-      for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int c = 0; c < CT; ++c) acc[t][c] = mfma16(ah[t], whi[c][j], acc[t][c]);
-#pragma unroll
-      for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int c = 0; c < CT; ++c) acc[t][c] = mfma16(al[t], whi[c][j], acc[t][c]);
-#pragma unroll
-      for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int c = 0; c < CT; ++c) acc[t][c] = mfma16(ah[t], wlo[c][j], acc[t][c]);
-    }
 
-    // ---- scaled posteriors q * sq, straight into A fragments of the moment
-    // contraction (padding rows have sq / sum = 0 and FAST argmax -1)
-    V8 aq[CT];
-    if (fast) {
-      f32x4 ar4[2];
-#pragma unroll
-      for (int t = 0; t < 2; ++t) ar4[t] = *reinterpret_cast<const f32x4*>(&Sf[2 * GR + 16 * t + 4 * lg]);
+      // ---- logits - max (the accumulators start at bias - max)
+      f32x4 acc[2][CT];
 #pragma unroll
       for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < CT; ++c)
+          acc[t][c] = f32x4{bc[c] + nm4[t][0], bc[c] + nm4[t][1], bc[c] + nm4[t][2], bc[c] + nm4[t][3]};
 #pragma unroll
-          for (int c = 0; c < CT; ++c) {
-            const bool hit = kw + 16 * c + lr == __float_as_int(ar4[t][r]) && bc[c] != -INFINITY;
-            terms += hit ? 1u : 0u;
-            aq[c][4 * t + r] = hit ? H(sc.sq) : H(0.f);
-          }
-    } else {
+      for (int j = 0; j < NS; ++j) {
+        V8 ah[2], al[2];
 #pragma unroll
-      for (int t = 0; t < 2; ++t)
+        for (int t = 0; t < 2; ++t) {
+          ah[t] = Ap[((t * NS + j) * 2 + 0) * 64 + lane];
+          al[t] = Ap[((t * NS + j) * 2 + 1) * 64 + lane];
+        }
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+        for (int t = 0; t < 2; ++t)
 #pragma unroll
-          for (int c = 0; c < CT; ++c) {
-            const float v = __builtin_amdgcn_exp2f(acc[t][c][r]) * rs4[t][r];
-            const bool keep = v >= qminS;
-            terms += keep ? 1u : 0u;
-            aq[c][4 * t + r] = H(keep ? v : 0.f);
-          }
+          for (int c = 0; c < CT; ++c) acc[t][c] = mfma16(ah[t], whi[c][j], acc[t][c]);
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+          for (int c = 0; c < CT; ++c) acc[t][c] = mfma16(al[t], whi[c][j], acc[t][c]);
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+          for (int c = 0; c < CT; ++c) acc[t][c] = mfma16(ah[t], wlo[c][j], acc[t][c]);
+      }
+
+      // ---- scaled posteriors q * sq, straight into A fragments of the moment
+      // contraction (padding rows have sq / sum = 0 and FAST argmax -1)
+      if (fast) {
+        f32x4 ar4[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) ar4[t] = *reinterpret_cast<const f32x4*>(&Sf[2 * GR + 16 * t + 4 * lg]);
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < CT; ++c) {
+              const bool hit = kw + 16 * c + lr == __float_as_int(ar4[t][r]) && bc[c] != -INFINITY;
+              terms += hit ? 1u : 0u;
+              aq[c][4 * t + r] = hit ? H(sc.sq) : H(0.f);
+            }
+      } else {
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < CT; ++c) {
+              const float v = __builtin_amdgcn_exp2f(acc[t][c][r]) * rs4[t][r];
+              const bool keep = v >= qminS;
+              terms += keep ? 1u : 0u;
+              aq[c][4 * t + r] = H(keep ? v : 0.f);
+            }
+      }
     }
     // posterior mass: every column of q x ones is the row sum
 #pragma unroll
@@ -870,7 +980,7 @@ constexpr size_t kRecordBudget = size_t(8) << 30;
 template <typename H, int NS, int CT, int WAVES>
 static void run_mfma(float* enc, const float* means, const float* covs, const float* priors, size_t D, size_t K,
                      const float* x, const uint64_t* offsets, size_t numImages, int flags,
-                     unsigned long long* numTerms, const std::vector<uint64_t>& off) {
+                     unsigned long long* numTerms, const std::vector<uint64_t>& off, const GateUse* gu) {
   constexpr int Dp = NS * 16;
   constexpr int BC = WAVES * 16 * CT;
   typedef GroupRec<NS> R;
@@ -898,7 +1008,8 @@ static void run_mfma(float* enc, const float* means, const float* covs, const fl
   const int numKB = Kp / BC;
   double* normpart = (double*)scratch(SLOT_PARTIAL, numImages * (size_t)numKB * sizeof(double));
   if (numTerms) VLHIP_CHECK(hipMemsetAsync(numTerms, 0, numImages * sizeof(unsigned long long), ctx().stream));
-  const size_t shmem = 2 * R::kBytes;
+  // two group records, plus the posterior tile of gate-served groups
+  const size_t shmem = 2 * R::kBytes + (gu ? (size_t)GR * BC * sizeof(float) : 0);
   auto kern = mfma_fisher_kernel<float, H, NS, CT, WAVES>;
   VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
   // image-aligned groups: gstart[i] = sum_{j<i} ceil(n_j / GR)
@@ -917,23 +1028,45 @@ static void run_mfma(float* enc, const float* means, const float* covs, const fl
     const size_t nG = (size_t)(gs[b1] - gs[b0]);
     unsigned char* rec = (unsigned char*)scratch(SLOT_FISHER_WORK, (nG ? nG : 1) * R::kBytes);
     const int n = (int)(b1 - b0);
+    unsigned char* gflag = nullptr;
+    int2* list = nullptr;
     if (nG > 0) {
-      int2* gtab = (int2*)scratch(SLOT_GROUPTAB, nG * sizeof(int2));
+      int4* gtab = (int4*)scratch(SLOT_GROUPTAB, nG * sizeof(int4));
       VLHIP_LAUNCH("mfma_group_table", mfma_group_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                    offsets, gstart, (int)b0, n, gtab);
+      if (gu) {
+        // gate: candidates and confidence per row, then the groups it serves
+        auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+        int* cand = (int*)scratch(SLOT_GATE, (size_t)rows * gu->M * sizeof(int));
+        unsigned char* gb = (unsigned char*)scratch(SLOT_GATE2, al(rows * sizeof(float)) + al(rows) + al(nG));
+        float* mass = (float*)gb;
+        unsigned char* conf = gb + al(rows * sizeof(float));
+        gflag = conf + al(rows);
+        vlhip_gate_select(gu->gate, x + rowBase * D, rows, gu->M, gu->need, cand, mass, conf);
+        list = (int2*)scratch(SLOT_GATELIST, nG * GR * gu->M * sizeof(int2));
+        VLHIP_LAUNCH("mfma_sparse", mfma_sparse_kernel<NS>, dim3((unsigned)((nG + 3) / 4)), dim3(256), 0,
+                     x + rowBase * D, (int)D, (long long)nG, gtab, conf, cand, gu->M, bias, mup, sinv, g, scp, flags,
+                     gflag, list, numTerms);
+        std::vector<unsigned char> hf(nG);
+        download(hf.data(), gflag, nG);
+        size_t served = 0;
+        for (size_t i = 0; i < nG; ++i) served += hf[i];
+        ctx().gate_groups += nG;
+        ctx().gate_served += served;
+      }
       const unsigned lb = (unsigned)((nG + ML_GROUPS - 1) / ML_GROUPS);
       if (flags & 4) {
         VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, true>), dim3(lb), dim3(ML_GROUPS * 64), 0,
-                     x + rowBase * D, (int)D, Kp, Whi, Wlo, bias, scp, g, rec, (long long)nG, gtab);
+                     x + rowBase * D, (int)D, Kp, Whi, Wlo, bias, scp, g, rec, (long long)nG, gtab, gflag);
       } else {
         VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, false>), dim3(lb), dim3(ML_GROUPS * 64), 0,
-                     x + rowBase * D, (int)D, Kp, Whi, Wlo, bias, scp, g, rec, (long long)nG, gtab);
+                     x + rowBase * D, (int)D, Kp, Whi, Wlo, bias, scp, g, rec, (long long)nG, gtab, gflag);
       }
     }
     const int padImages = n >= 8 ? ((n + 7) / 8) * 8 : n;
     VLHIP_LAUNCH("mfma_fisher", kern, dim3((unsigned)((size_t)padImages * numKB)), dim3(WAVES * 64), shmem, rec,
                  offsets, gstart, (int)b0, n, (int)D, (int)K, Whi, Wlo, bias, scp, mup, isg, sinv,
-                 priors, flags, enc, normpart, numTerms, numKB);
+                 priors, flags, enc, normpart, numTerms, numKB, gflag, list, gu ? gu->M : 0);
     b0 = b1;
   }
   if (flags & 2) {
@@ -946,25 +1079,27 @@ static void run_mfma(float* enc, const float* means, const float* covs, const fl
 template <typename H, int NS>
 static void run_mfma_k(float* enc, const float* means, const float* covs, const float* priors, size_t D, size_t K,
                        const float* x, const uint64_t* offsets, size_t numImages, int flags,
-                       unsigned long long* numTerms, const std::vector<uint64_t>& off) {
-  if (K > 128 && NS <= 5) run_mfma<H, NS, 2, 8>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off);
-  else run_mfma<H, NS, 2, 4>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off);
+                       unsigned long long* numTerms, const std::vector<uint64_t>& off, const GateUse* gu) {
+  if (K > 128 && NS <= 5)
+    run_mfma<H, NS, 2, 8>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off, gu);
+  else
+    run_mfma<H, NS, 2, 4>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off, gu);
 }
 
 template <typename H>
 static void mfma_dispatch(float* enc, const float* means, const float* covs, const float* priors, size_t D, size_t K,
                           const float* x, const uint64_t* offsets, size_t numImages, int flags,
-                          unsigned long long* numTerms) {
+                          unsigned long long* numTerms, const GateUse* gu) {
   std::vector<uint64_t> off(numImages + 1);
   download(off.data(), offsets, (numImages + 1) * sizeof(uint64_t));
   const size_t Dp = (D + 15) / 16 * 16;
   switch (Dp) {
-    case 16: run_mfma_k<H, 1>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off); break;
-    case 32: run_mfma_k<H, 2>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off); break;
-    case 48: run_mfma_k<H, 3>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off); break;
-    case 64: run_mfma_k<H, 4>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off); break;
-    case 80: run_mfma_k<H, 5>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off); break;
-    case 96: run_mfma_k<H, 6>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off); break;
+    case 16: run_mfma_k<H, 1>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off, gu); break;
+    case 32: run_mfma_k<H, 2>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off, gu); break;
+    case 48: run_mfma_k<H, 3>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off, gu); break;
+    case 64: run_mfma_k<H, 4>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off, gu); break;
+    case 80: run_mfma_k<H, 5>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off, gu); break;
+    case 96: run_mfma_k<H, 6>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off, gu); break;
     default: fatal("vlhip_fisher_encode_batch", __FILE__, __LINE__,
                    "the matrix-core Fisher encoder supports dimension <= 96");
   }
@@ -972,11 +1107,12 @@ static void mfma_dispatch(float* enc, const float* means, const float* covs, con
 
 void fisher_split16(float* enc, const float* means, const float* covs, const float* priors, size_t D, size_t K,
                     const float* x, const uint64_t* offsets, size_t numImages, int flags,
-                    unsigned long long* numTerms, bool fp16) {
+                    unsigned long long* numTerms, bool fp16, const GateUse* gu) {
   VLHIP_REQUIRE(K >= 1 && D >= 1, "numClusters and dimension must be positive");
   if (numImages == 0) return;
-  if (fp16) mfma_dispatch<_Float16>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms);
-  else mfma_dispatch<__bf16>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms);
+  if (gu) VLHIP_REQUIRE(gu->M >= 1 && gu->M <= 16 && (size_t)gu->M <= K, "gate candidates must be in [1, min(16, K)]");
+  if (fp16) mfma_dispatch<_Float16>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, gu);
+  else mfma_dispatch<__bf16>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, gu);
 }
 
 }  // namespace vlhip

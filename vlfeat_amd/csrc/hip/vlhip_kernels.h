@@ -38,8 +38,15 @@ void launch_fisher_normalize(float* enc, const double* normpart, int numKB, size
 
 // Split-16 matrix-core Fisher encoder (fisher_mfma.hip); fp16 selects fp16
 // operands, otherwise bf16.
+// Learned-gate setting of the matrix-core encoder (nullptr: dense).
+struct GateUse {
+  void* gate;
+  int M;       // candidates per descriptor (<= 16)
+  float need;  // predicted mass a descriptor needs to skip the dense E-step
+};
 void fisher_split16(float* enc, const float* means, const float* covs, const float* priors, size_t D, size_t K,
                     const float* x, const uint64_t* offsets, size_t numImages, int flags,
-                    unsigned long long* numTerms, bool fp16);
+                    unsigned long long* numTerms, bool fp16,
+                    const GateUse* gate);
 
 }  // namespace vlhip

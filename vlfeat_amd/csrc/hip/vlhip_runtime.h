@@ -47,6 +47,7 @@ enum Slot {
   SLOT_FISHER_WORK,   // per-batch work buffer of the Fisher encoders (posteriors or operand records)
   SLOT_GROUPS,        // per-image first group index of the matrix-core Fisher encoder
   SLOT_GROUPTAB,      // per-group first row and image end of the matrix-core Fisher encoder
+  SLOT_GATELIST,      // posteriors of gate-served groups (component, q) of the matrix-core Fisher encoder
   SLOT_COUNT
 };
 
@@ -73,6 +74,11 @@ struct Context {
   std::vector<PendingEvent> pending;
   std::vector<hipEvent_t> event_pool;
   int num_cus = 0;
+  void* gate = nullptr;  // learned gate of VLHIP_MODE_GATED
+  int gate_m = 16;
+  float gate_need = 0.99f;
+  unsigned long long gate_groups = 0;  // descriptor groups seen by gated encodings
+  unsigned long long gate_served = 0;  // of which served by the gate
 };
 
 Context& ctx();

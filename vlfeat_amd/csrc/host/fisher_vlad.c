@@ -22,7 +22,7 @@ VL_EXPORT void
 vl_fisher_set_precision (int precision)
 {
   if (precision != VL_FISHER_PRECISION_FP32 && precision != VL_FISHER_PRECISION_BF16X3 &&
-      precision != VL_FISHER_PRECISION_F16X3) {
+      precision != VL_FISHER_PRECISION_F16X3 && precision != VL_FISHER_PRECISION_GATED) {
     VL_PRINTF("vl_fisher_set_precision: unknown precision %d\n", precision) ;
     abort() ;
   }
@@ -30,6 +30,12 @@ vl_fisher_set_precision (int precision)
 }
 
 VL_EXPORT int vl_fisher_get_precision (void) { return g_precision ; }
+
+VL_EXPORT void
+vl_fisher_set_gate (void * gate, int candidates, float need)
+{
+  vlhip_fisher_set_gate(gate, candidates, need) ;
+}
 
 static void
 check_type (vl_type t, char const * who)
