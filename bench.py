@@ -21,6 +21,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode f16x3|bf16x3|
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import re
@@ -72,7 +73,7 @@ def fit_gmm(x_host: np.ndarray, K: int, iters: int, seed: int):
     return m, c, p, ll
 
 
-def cpu_legs(x_host, offsets, m, c, p, budget_s: float, gpu_enc, gpu_fp32_enc, n_oracle: int):
+def cpu_legs(x_host, offsets, m, c, p, budget_s: float, gpu_enc, gpu_fp32_enc, n_oracle: int, gpu_gated_enc=None):
     """The reference VLFeat (oracle/_ref, OpenMP over all host threads) on whole
     images until the time budget is used; parity of the GPU FP32 encoder
     against it; error of the selected GPU encoder against the FP64 oracle."""
@@ -104,18 +105,21 @@ def cpu_legs(x_host, offsets, m, c, p, budget_s: float, gpu_enc, gpu_fp32_enc, n
         err = np.linalg.norm(g - r, axis=1) / np.linalg.norm(r, axis=1)
         out["parity_fp32_vs_reference"] = {"images": nc, "max_rel_l2": float(err.max())}
     O = oracle()
-    errs, errs32, errref = [], [], []
+    errs, errs32, errref, errg = [], [], [], []
     for i in range(min(n_oracle, len(offsets) - 1)):
         xs = x_host[offsets[i]:offsets[i + 1]]
         eo, _ = O.fisher(xs, m, c, p, FLAGS)
         nrm = np.linalg.norm(eo)
         errs.append(np.linalg.norm(gpu_enc[i].astype(np.float64) - eo) / nrm)
         errs32.append(np.linalg.norm(gpu_fp32_enc[i].astype(np.float64) - eo) / nrm)
+        if gpu_gated_enc is not None:
+            errg.append(np.linalg.norm(gpu_gated_enc[i].astype(np.float64) - eo) / nrm)
         if R is not None and i < r.shape[0]:
             errref.append(np.linalg.norm(r[i] - eo) / nrm)
     out["error_vs_fp64_oracle"] = {
         "images": len(errs), "selected_max_rel_l2": float(max(errs)), "fp32_max_rel_l2": float(max(errs32)),
-        "reference_float_max_rel_l2": float(max(errref)) if errref else None}
+        "reference_float_max_rel_l2": float(max(errref)) if errref else None,
+        "gated_max_rel_l2": float(max(errg)) if errg else None}
     return out
 
 
@@ -140,6 +144,9 @@ def main() -> int:
     ap.add_argument("--oracle-images", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 comparison run")
+    ap.add_argument("--no-gated", action="store_true", help="skip the learned-gate leg")
+    ap.add_argument("--gate-train", type=int, default=2_000_000, help="descriptors the gate is trained on")
+    ap.add_argument("--gate-need", type=float, default=0.999)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -173,6 +180,28 @@ def main() -> int:
     od = torch.as_tensor(offsets.astype(np.int64), device=dev)
     modes = {"fp32": va.MODE_FP32, "bf16x3": va.MODE_BF16X3, "f16x3": va.MODE_F16X3, "gated": va.MODE_GATED}
     torch.cuda.synchronize()
+
+    gate_info = None
+    gate = None
+    if args.mode == "gated" or not args.no_gated:
+        # learned gate (80 -> 256 -> 256 -> 1024, GELU, BF16 products) trained on the
+        # GPU against the exact posteriors of the fitted GMM, outside the timed region
+        n_train = min(args.gate_train, N)
+        batch, epochs = 4096, 2
+        steps = epochs * (n_train // batch)
+        gate = L.vlhip_gate_new(DIM, K_COMP, 256, 256, 1)
+        L.vlhip_gate_set_standardization_from_data(gate, x.data_ptr(), n_train)
+        trace = np.empty(steps, np.float32)
+        t0 = time.perf_counter()
+        L.vlhip_gate_train(gate, x.data_ptr(), n_train, md.data_ptr(), cd.data_ptr(), pd.data_ptr(), steps, batch,
+                           C.c_float(1e-3), trace.ctypes.data)
+        torch.cuda.synchronize()
+        gate_info = {"train_descriptors": n_train, "steps": steps, "batch": batch,
+                     "train_s": time.perf_counter() - t0, "loss_first": float(trace[:20].mean()),
+                     "loss_last": float(trace[-20:].mean()), "candidates": 16, "need": args.gate_need}
+        log(f"gate trained: {steps} steps in {gate_info['train_s']:.1f} s, loss "
+            f"{gate_info['loss_first']:.3f} -> {gate_info['loss_last']:.3f}")
+        L.vlhip_fisher_set_gate(gate, 16, C.c_float(args.gate_need))
 
     def run(mode_name: str, steps: int, warmup: int):
         enc = torch.empty((n_img, 2 * K_COMP * DIM), dtype=torch.float32, device=dev)
@@ -248,10 +277,26 @@ def main() -> int:
         del e32
     elif args.mode == "fp32":
         enc32 = enc_sel
+    enc_gated = None
+    if gate_info is not None:
+        if args.mode == "gated":
+            msg, enc_gated = ms_per_step, enc_sel
+        else:
+            va.fisher_gate_stats(reset=True)
+            msg, tg, eg, _ = run("gated", 1, 1)
+            enc_gated = eg[: min(n_img, n_check)].cpu().numpy()
+            del eg
+        groups, served = va.fisher_gate_stats(reset=True)
+        gate_info.update({"value": N / (msg / 1000), "ms_per_step": msg,
+                          "served_group_fraction": served / max(groups, 1),
+                          "speedup_vs_fp32_gpu": (result["fp32_gpu"]["ms_per_step"] / msg) if "fp32_gpu" in result
+                          else None})
+        result["gated_gpu"] = gate_info
 
     if not args.no_cpu and enc32 is not None:
         x_host = x[: min(N, 1024 * PER_IMAGE)].cpu().numpy()
-        result.update(cpu_legs(x_host, offsets, m, c, p, args.cpu_budget, enc_sel, enc32, args.oracle_images))
+        result.update(cpu_legs(x_host, offsets, m, c, p, args.cpu_budget, enc_sel, enc32, args.oracle_images,
+                               enc_gated))
         result["cpu_baseline"] = result.get("cpu_baseline")
 
     if args.copy_probe:
@@ -260,6 +305,9 @@ def main() -> int:
         b.copy_(a)
         torch.cuda.synchronize()
         log("copy probe: 1 GiB read + 1 GiB written by one copy kernel")
+    if gate is not None:
+        L.vlhip_fisher_set_gate(None, 16, C.c_float(args.gate_need))
+        L.vlhip_gate_delete(gate)
     print(json.dumps(result), flush=True)
     return 0
 

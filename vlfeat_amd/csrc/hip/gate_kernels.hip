@@ -18,7 +18,10 @@
 // 64 x 64 x 32 MFMA GEMM with optional transposed operands, bias and GELU
 // epilogue), gate_xent (softmax, loss and its gradient per row), gate_gelu_bwd,
 // gate_colsum (bias gradients, fixed-order two-stage reduction), gate_adam,
-// gate_to_bf16, gate_select (softmax mass screen + exact top-M per row).
+// gate_to_bf16, and for inference gate_transpose (weights to [out][in]),
+// gate_input_pad, gate_gemm_nt (128 x 128 double-buffered MFMA GEMM with
+// 16-byte loads; the gate's logits are rounded to BF16) and gate_select
+// (softmax mass screen + exact top-M per row).
 #include "vlhip_runtime.h"
 #include "vlhip_device.h"
 #include "vlhip_kernels.h"
@@ -205,7 +208,123 @@ __global__ void gate_to_bf16_kernel(const float* __restrict__ in, long long n, _
   if (i < n) out[i] = (__bf16)in[i];
 }
 
-// Candidate selection, one wave per row of logits z (K <= 64 * GS_VPL):
+// ---- inference path --------------------------------------------------------
+// Weights transposed to [out][in] (input index contiguous, zero-padded to a
+// multiple of 8), so that both operands of every product are read along the
+// reduction index with 16-byte loads.
+__global__ void gate_transpose_kernel(const float* __restrict__ W, int in, int out, int inPad,
+                                      __bf16* __restrict__ WT) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)out * inPad) return;
+  const int o = (int)(i / inPad), k = (int)(i % inPad);
+  WT[i] = (__bf16)(k < in ? W[(size_t)k * out + o] : 0.f);
+}
+
+// x_in = bf16((x - mean) * istd) with rows padded to Dq (zeros).
+__global__ void gate_input_pad_kernel(const float* __restrict__ x, long long n, int D, int Dq,
+                                      const float* __restrict__ mean, const float* __restrict__ istd,
+                                      __bf16* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * Dq) return;
+  const long long r = i / Dq;
+  const int d = (int)(i % Dq);
+  out[i] = (__bf16)(d < D ? (x[r * D + d] - mean[d]) * istd[d] : 0.f);
+}
+
+typedef unsigned g_u32x4 __attribute__((ext_vector_type(4)));
+constexpr int NT_M = 128, NT_N = 128, NT_K = 32, NT_LD = NT_K + 8;
+
+// One 128 x NT_K operand tile (rows r0.., reduction k0..) into registers as
+// 16-byte words, NT_W per thread; rows or k past the end read as zero (K is a
+// multiple of 8).
+constexpr int NT_W = NT_M * NT_K / 8 / 256;
+__device__ __forceinline__ void nt_load(g_u32x4 (&r)[NT_W], const __bf16* __restrict__ P, int ld, int rows, int r0,
+                                        int K, int k0) {
+#pragma unroll
+  for (int q = 0; q < NT_W; ++q) {
+    const int v = threadIdx.x + 256 * q;
+    const int row = v / (NT_K / 8), kk = (v % (NT_K / 8)) * 8;
+    const int gr = r0 + row, gk = k0 + kk;
+    r[q] = (gr < rows && gk < K) ? *reinterpret_cast<const g_u32x4*>(P + (size_t)gr * ld + gk)
+                                 : g_u32x4{0u, 0u, 0u, 0u};
+  }
+}
+__device__ __forceinline__ void nt_store(const g_u32x4 (&r)[NT_W], __bf16 (*S)[NT_LD]) {
+#pragma unroll
+  for (int q = 0; q < NT_W; ++q) {
+    const int v = threadIdx.x + 256 * q;
+    *reinterpret_cast<g_u32x4*>(&S[v / (NT_K / 8)][(v % (NT_K / 8)) * 8]) = r[q];
+  }
+}
+
+// C[M x N] = bf16(epilogue(A B^T + bias)), A [M x K] and B [N x K] row-major
+// BF16 (K a multiple of 8, rows 16-byte aligned).  128 x 128 tile per
+// workgroup, 4 waves of 64 x 64 (4 x 4 MFMA tiles), reduction steps of NT_K
+// double-buffered in LDS with the next step fetched into registers.
+__global__ __launch_bounds__(256) void gate_gemm_nt_kernel(int M, int N, int K, const __bf16* __restrict__ A, int lda,
+                                                           const __bf16* __restrict__ B, int ldb,
+                                                           const float* __restrict__ bias, int epi,
+                                                           __bf16* __restrict__ C, int ldc) {
+  __shared__ __align__(16) __bf16 As[2][NT_M][NT_LD];
+  __shared__ __align__(16) __bf16 Bs[2][NT_N][NT_LD];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wm = w & 1, wn = w >> 1, lr = lane & 15, lg = lane >> 4;
+  const int m0 = blockIdx.y * NT_M, n0 = blockIdx.x * NT_N;
+  g_f32x4 acc[4][4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[t][u] = g_f32x4{0.f, 0.f, 0.f, 0.f};
+  g_u32x4 ra[NT_W], rb[NT_W];
+  nt_load(ra, A, lda, M, m0, K, 0);
+  nt_load(rb, B, ldb, N, n0, K, 0);
+  nt_store(ra, As[0]);
+  nt_store(rb, Bs[0]);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = 0; k0 < K; k0 += NT_K) {
+    const bool more = k0 + NT_K < K;
+    if (more) {
+      nt_load(ra, A, lda, M, m0, K, k0 + NT_K);
+      nt_load(rb, B, ldb, N, n0, K, k0 + NT_K);
+    }
+#pragma unroll
+    for (int kk = 0; kk < NT_K; kk += 32) {
+      g_bf16x8 a[4], b[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        a[t] = *reinterpret_cast<const g_bf16x8*>(&As[buf][64 * wm + 16 * t + lr][kk + 8 * lg]);
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        b[u] = *reinterpret_cast<const g_bf16x8*>(&Bs[buf][64 * wn + 16 * u + lr][kk + 8 * lg]);
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], b[u], acc[t][u], 0, 0, 0);
+    }
+    if (more) {
+      // the other buffer was last read before the previous barrier
+      nt_store(ra, As[buf ^ 1]);
+      nt_store(rb, Bs[buf ^ 1]);
+    }
+    __syncthreads();
+    buf ^= 1;
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + 64 * wm + 16 * t + 4 * lg + r, n = n0 + 64 * wn + 16 * u + lr;
+        if (m >= M || n >= N) continue;
+        const float v = acc[t][u][r] + bias[n];
+        C[(size_t)m * ldc + n] = (__bf16)(epi == EPI_GELU ? gelu_f(v) : v);
+      }
+}
+
+// Candidate selection, one wave per row of BF16 logits z (K <= 64 * GS_VPL):
 // p = softmax(z).  Screen: with c = #{p >= t} and S = sum_{p >= t} p, the mass
 // of the M most probable components is at most S when c >= M and at most
 // S + (M - c) t otherwise; rows whose bound is below `need` are marked
@@ -213,21 +332,30 @@ __global__ void gate_to_bf16_kernel(const float* __restrict__ in, long long n, _
 // rounds of a wave-wide argmax, ties to the lower index), and are confident
 // when the selected mass reaches `need`.  idx receives M component indices
 // (-1 for dense rows), mass the selected mass (or the bound).
-constexpr int GS_VPL = 32;
-__global__ __launch_bounds__(256) void gate_select_kernel(const float* __restrict__ Z, long long rows, int K, int M,
-                                                          float t, float need, int* __restrict__ idx,
+constexpr int GS_MAXV = 32;  // components per lane at most (K <= 2048)
+// Lane l holds components 8(l + 64q) + e, e < 8, q < GS_VPL / 8 (16-byte loads
+// of the row's BF16 logits; the row stride ldz is a multiple of 8).
+template <int GS_VPL>
+__global__ __launch_bounds__(256) void gate_select_kernel(const __bf16* __restrict__ Z, int ldz, long long rows, int K,
+                                                          int M, float t, float need, int* __restrict__ idx,
                                                           float* __restrict__ mass, unsigned char* __restrict__ conf) {
   const int lane = threadIdx.x & 63;
   const long long row = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= rows) return;
-  const float* z = Z + row * K;
+  const __bf16* z = Z + row * ldz;
   float v[GS_VPL];
   float mx = -INFINITY;
 #pragma unroll
-  for (int j = 0; j < GS_VPL; ++j) {
-    const int k = lane + 64 * j;
-    v[j] = k < K ? z[k] : -INFINITY;
-    mx = fmaxf(mx, v[j]);
+  for (int q = 0; q < GS_VPL / 8; ++q) {
+    const int k0 = 8 * (lane + 64 * q);
+    g_bf16x8 zz = g_bf16x8{};
+    if (k0 < K) zz = *reinterpret_cast<const g_bf16x8*>(z + k0);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int j = 8 * q + e;
+      v[j] = (k0 + e < K) ? (float)zz[e] : -INFINITY;
+      mx = fmaxf(mx, v[j]);
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
@@ -266,7 +394,7 @@ __global__ __launch_bounds__(256) void gate_select_kernel(const float* __restric
 #pragma unroll
     for (int j = 0; j < GS_VPL; ++j)
       if (v[j] > bv) { bv = v[j]; bj = j; }
-    int bk = lane + 64 * bj;
+    int bk = 8 * (lane + 64 * (bj >> 3)) + (bj & 7);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       const float ov = __shfl_xor(bv, o, 64);
@@ -275,10 +403,10 @@ __global__ __launch_bounds__(256) void gate_select_kernel(const float* __restric
     }
     sel += bv;
     if (lane == 0) out[it] = bk;
-    if ((bk & 63) == lane) {
+    if (((bk >> 3) & 63) == lane) {
 #pragma unroll
       for (int j = 0; j < GS_VPL; ++j)
-        if (lane + 64 * j == bk) v[j] = -1.f;
+        if (8 * (lane + 64 * (j >> 3)) + (j & 7) == bk) v[j] = -1.f;
     }
   }
   if (lane == 0) {
@@ -303,6 +431,11 @@ struct Gate {
   float* Am = nullptr;  // Adam first moments
   float* Av = nullptr;  // Adam second moments
   __bf16* Wb = nullptr; // BF16 copies of W1, W2, W3 (contiguous)
+  int Dq = 0;           // D rounded up to 8
+  int Kq = 0;           // K rounded up to 8 (row stride of the BF16 logits)
+  __bf16* WT = nullptr; // inference copies [W1^T (H1 x Dq) | W2^T (H2 x H1) | W3^T (K x H2)]
+  long long icap = 0;   // inference buffers for up to icap rows
+  __bf16 *Xi = nullptr, *H1i = nullptr, *H2i = nullptr, *Zi = nullptr;
   float* mean = nullptr;
   float* istd = nullptr;
   long long step = 0;
@@ -318,10 +451,15 @@ struct Gate {
   float* b2() { return P + nW1 + H1 + nW2; }
   float* W3() { return P + nW1 + H1 + nW2 + H2; }
   float* b3() { return P + nW1 + H1 + nW2 + H2 + nW3; }
+  __bf16* W1T() { return WT; }
+  __bf16* W2T() { return WT + (size_t)H1 * Dq; }
+  __bf16* W3T() { return WT + (size_t)H1 * Dq + (size_t)H2 * H1; }
   __bf16* W1b() { return Wb; }
   __bf16* W2b() { return Wb + nW1; }
   __bf16* W3b() { return Wb + nW1 + nW2; }
 };
+
+static void free_infer(Gate& g);
 
 static void free_work(Gate& g) {
   void* bufs[] = {g.Xb, g.H1b, g.H2b, g.dZb, g.dA2b, g.dA1b, g.A1, g.A2, g.Z, g.Q, g.dZf, g.dHf, g.dAf, g.loss, g.part};
@@ -417,6 +555,12 @@ static void backward(Gate& g, long long rows) {
 }
 
 static void refresh_bf16(Gate& g) {
+  VLHIP_LAUNCH("gate_transpose", gate_transpose_kernel, dim3(grid1((long long)g.H1 * g.Dq)), dim3(256), 0, g.W1(), g.D,
+               g.H1, g.Dq, g.W1T());
+  VLHIP_LAUNCH("gate_transpose", gate_transpose_kernel, dim3(grid1((long long)g.H2 * g.H1)), dim3(256), 0, g.W2(), g.H1,
+               g.H2, g.H1, g.W2T());
+  VLHIP_LAUNCH("gate_transpose", gate_transpose_kernel, dim3(grid1((long long)g.K * g.H2)), dim3(256), 0, g.W3(), g.H2,
+               g.K, g.H2, g.W3T());
   VLHIP_LAUNCH("gate_to_bf16", gate_to_bf16_kernel, dim3(grid1(g.nW1)), dim3(256), 0, g.W1(), (long long)g.nW1,
                g.W1b());
   VLHIP_LAUNCH("gate_to_bf16", gate_to_bf16_kernel, dim3(grid1(g.nW2)), dim3(256), 0, g.W2(), (long long)g.nW2,
@@ -463,7 +607,8 @@ extern "C" {
 
 void* vlhip_gate_new(size_t dimension, size_t numClusters, size_t hidden1, size_t hidden2, unsigned long long seed) {
   VLHIP_REQUIRE(dimension >= 1 && numClusters >= 1 && hidden1 >= 1 && hidden2 >= 1, "gate sizes must be positive");
-  VLHIP_REQUIRE(numClusters <= (size_t)64 * GS_VPL, "the gate supports at most 2048 components");
+  VLHIP_REQUIRE(numClusters <= (size_t)64 * GS_MAXV, "the gate supports at most 2048 components");
+  VLHIP_REQUIRE(hidden1 % 8 == 0 && hidden2 % 8 == 0, "hidden widths must be multiples of 8");
   ensure_init();
   std::lock_guard<std::recursive_mutex> lk(ctx().mu);
   Gate* g = new Gate();
@@ -480,6 +625,9 @@ void* vlhip_gate_new(size_t dimension, size_t numClusters, size_t hidden1, size_
   g->Am = (float*)dev_alloc(sizeof(float) * g->nParams);
   g->Av = (float*)dev_alloc(sizeof(float) * g->nParams);
   g->Wb = (__bf16*)dev_alloc(sizeof(__bf16) * (g->nW1 + g->nW2 + g->nW3));
+  g->Dq = (g->D + 7) / 8 * 8;
+  g->Kq = (g->K + 7) / 8 * 8;
+  g->WT = (__bf16*)dev_alloc(sizeof(__bf16) * ((size_t)g->H1 * g->Dq + (size_t)g->H2 * g->H1 + (size_t)g->K * g->H2));
   g->mean = (float*)dev_alloc(sizeof(float) * g->D);
   g->istd = (float*)dev_alloc(sizeof(float) * g->D);
   VLHIP_CHECK(hipMemsetAsync(g->Am, 0, sizeof(float) * g->nParams, ctx().stream));
@@ -500,7 +648,8 @@ void vlhip_gate_delete(void* gate) {
   Gate* g = (Gate*)gate;
   sync();
   free_work(*g);
-  void* bufs[] = {g->P, g->G, g->Am, g->Av, g->Wb, g->mean, g->istd};
+  free_infer(*g);
+  void* bufs[] = {g->P, g->G, g->Am, g->Av, g->Wb, g->WT, g->mean, g->istd};
   for (void* b : bufs)
     if (b) VLHIP_CHECK(hipFree(b));
   delete g;
@@ -602,6 +751,55 @@ double vlhip_gate_train(void* gate, void const* data, size_t numData, void const
   return last;
 }
 
+}  // extern "C"
+
+namespace vlhip {
+
+static void free_infer(Gate& g) {
+  void* bufs[] = {g.Xi, g.H1i, g.H2i, g.Zi};
+  for (void* b : bufs)
+    if (b) VLHIP_CHECK(hipFree(b));
+  g.Xi = g.H1i = g.H2i = g.Zi = nullptr;
+  g.icap = 0;
+}
+
+static void ensure_infer(Gate& g, long long rows) {
+  if (rows <= g.icap) return;
+  sync();
+  free_infer(g);
+  g.Xi = (__bf16*)dev_alloc(sizeof(__bf16) * rows * g.Dq);
+  g.H1i = (__bf16*)dev_alloc(sizeof(__bf16) * rows * g.H1);
+  g.H2i = (__bf16*)dev_alloc(sizeof(__bf16) * rows * g.H2);
+  g.Zi = (__bf16*)dev_alloc(sizeof(__bf16) * rows * g.Kq);
+  g.icap = rows;
+}
+
+static void gemm_nt(int M, int N, int K, const __bf16* A, int lda, const __bf16* B, int ldb, const float* bias,
+                    int epi, __bf16* C, int ldc) {
+  dim3 grid((N + NT_N - 1) / NT_N, (M + NT_M - 1) / NT_M);
+  VLHIP_LAUNCH("gate_gemm_nt", gate_gemm_nt_kernel, grid, dim3(256), 0, M, N, K, A, lda, B, ldb, bias, epi, C, ldc);
+}
+
+// Inference forward of rows descriptors into g.Zi (BF16 logits: the gate's output).
+static void infer(Gate& g, const float* x, long long rows) {
+  const int r = (int)rows;
+  VLHIP_LAUNCH("gate_input", gate_input_pad_kernel, dim3(grid1(rows * g.Dq)), dim3(256), 0, x, rows, g.D, g.Dq,
+               g.mean, g.istd, g.Xi);
+  gemm_nt(r, g.H1, g.Dq, g.Xi, g.Dq, g.W1T(), g.Dq, g.b1(), EPI_GELU, g.H1i, g.H1);
+  gemm_nt(r, g.H2, g.H1, g.H1i, g.H1, g.W2T(), g.H1, g.b2(), EPI_GELU, g.H2i, g.H2);
+  gemm_nt(r, g.K, g.H2, g.H2i, g.H2, g.W3T(), g.H2, g.b3(), EPI_NONE, g.Zi, g.Kq);
+}
+
+__global__ void gate_bf16_to_f32_kernel(const __bf16* __restrict__ in, long long rows, int K, int ld,
+                                        float* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < rows * K) out[i] = (float)in[(i / K) * ld + i % K];
+}
+
+}  // namespace vlhip
+
+extern "C" {
+
 // Candidates for rows descriptors (device FP32): idx (device, rows x M int32),
 // mass (device, rows FP32) and conf (device, rows bytes) as gate_select_kernel
 // defines them; processed in chunks of at most 65536 rows.
@@ -610,15 +808,24 @@ void vlhip_gate_select(void* gate, void const* data, size_t rows, int M, float n
   std::lock_guard<std::recursive_mutex> lk(ctx().mu);
   Gate* g = (Gate*)gate;
   VLHIP_REQUIRE(M >= 1 && M <= 64 && M <= g->K, "candidate count must be in [1, min(64, K)]");
-  const long long chunk = 65536;
-  ensure_work(*g, rows < (size_t)chunk ? (long long)rows : chunk);
+  VLHIP_REQUIRE(g->Kq / 8 <= 64 * (GS_MAXV / 8), "the gate supports at most 2048 components");
+  const long long chunk = 262144;
+  ensure_infer(*g, rows < (size_t)chunk ? (long long)rows : chunk);
   const float* x = (const float*)data;
   const float t = need / (float)(4 * M);  // screen threshold
+  const int vpl = (g->Kq / 8 + 63) / 64 * 8;  // values per lane, a multiple of 8
   for (size_t r0 = 0; r0 < rows; r0 += (size_t)chunk) {
     const long long n = (long long)((rows - r0) < (size_t)chunk ? (rows - r0) : (size_t)chunk);
-    forward(*g, x + r0 * g->D, n);
-    VLHIP_LAUNCH("gate_select", gate_select_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, g->Z, n, g->K, M, t,
-                 need, idx + r0 * M, mass + r0, conf + r0);
+    infer(*g, x + r0 * g->D, n);
+    const dim3 grid((unsigned)((n + 3) / 4)), block(256);
+#define GATE_SELECT(V)                                                                                        \
+    VLHIP_LAUNCH("gate_select", gate_select_kernel<V>, grid, block, 0, g->Zi, g->Kq, n, g->K, M, t, need,       \
+                 idx + r0 * M,                                                                                \
+                 mass + r0, conf + r0)
+    if (vpl <= 8) GATE_SELECT(8);
+    else if (vpl <= 16) GATE_SELECT(16);
+    else GATE_SELECT(32);
+#undef GATE_SELECT
   }
 }
 
@@ -626,8 +833,11 @@ void vlhip_gate_select(void* gate, void const* data, size_t rows, int M, float n
 void vlhip_gate_logits(void* gate, void const* data, size_t rows, float* logits_host) {
   std::lock_guard<std::recursive_mutex> lk(ctx().mu);
   Gate* g = (Gate*)gate;
+  ensure_infer(*g, (long long)rows);
   ensure_work(*g, (long long)rows);
-  forward(*g, (const float*)data, (long long)rows);
+  infer(*g, (const float*)data, (long long)rows);
+  VLHIP_LAUNCH("gate_bf16_to_f32", gate_bf16_to_f32_kernel, dim3(grid1((long long)rows * g->K)), dim3(256), 0, g->Zi,
+               (long long)rows, g->K, g->Kq, g->Z);
   download(logits_host, g->Z, rows * g->K * sizeof(float));
 }
 
