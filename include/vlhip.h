@@ -120,6 +120,28 @@ VLHIP_API void vlhip_fisher_set_gate (void * gate, int candidates, float need) ;
  * reset, and how many of them the gate served (host values). */
 VLHIP_API void vlhip_fisher_gate_stats (unsigned long long * groups, unsigned long long * served, int reset) ;
 
+/* Encode plans for batches of uneven images (no counterpart in the
+ * reference, whose callers loop over images).  A plan fixes the dimension, the
+ * number of components, the images' descriptor counts (offsets_host, HOST
+ * array of numImages+1 values starting at 0) and the flags; it prepares the
+ * image-aligned group tables and device buffers once and picks the
+ * matrix-core component tiling by timing the candidates on the first run
+ * (cached per shape).  vlhip_encode_plan_run computes the compensated-FP16
+ * Fisher vectors (fisher.c:363-488 per image) and, when vladFlags >= 0, the
+ * VLAD of every image with each descriptor assigned to its nearest mean
+ * (kmeans.c:655-702, vlad.c:163-242); its kernel sequence is captured into a
+ * hipGraph on the first run and replayed while the pointers stay the same.
+ * Float data, dimension <= 96.  Runs are synchronous. */
+VLHIP_API void * vlhip_encode_plan_new (size_t dimension, size_t numClusters, uint64_t const * offsets_host,
+                                        size_t numImages, int fisherFlags, int vladFlags) ;
+VLHIP_API void   vlhip_encode_plan_run (void * plan, void const * data, void const * means,
+                                        void const * covariances, void const * priors, void * fisherEnc,
+                                        void * vladEnc, unsigned long long * numTerms) ;
+/* chosen tiling (0: 8 waves x 32 components per workgroup, 1: 4 waves x 32),
+ * the measured milliseconds of each candidate (-1 if not tried), batches */
+VLHIP_API void   vlhip_encode_plan_info (void * plan, int * variant, double * variantMs, size_t * numBatches) ;
+VLHIP_API void   vlhip_encode_plan_delete (void * plan) ;
+
 /* ---- VLAD (vl/vlad.c) ---------------------------------------------------- */
 
 /* VLAD of numImages images, vlad.c:163-242 per image.  Exactly one of

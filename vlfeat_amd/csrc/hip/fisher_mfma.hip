@@ -977,6 +977,89 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
 // Bytes of group records materialised at once (batches of whole images).
 constexpr size_t kRecordBudget = size_t(8) << 30;
 
+// Device tables of one encode, carved from one buffer:
+// [scales][g Dp][bias Kp][mup, isg, sinv Kp*Dp][Whi, Wlo Kp*2Dp].
+template <typename H>
+struct MfmaTables {
+  MfmaScales* scp;
+  float *g, *bias, *mup, *isg, *sinv;
+  H *Whi, *Wlo;
+};
+
+static inline size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
+
+template <typename H>
+static size_t mfma_tables_bytes(int Dp, int Kp) {
+  return al256(sizeof(MfmaScales)) + al256(sizeof(float) * Dp) + al256(sizeof(float) * Kp) +
+         3 * al256(sizeof(float) * (size_t)Kp * Dp) + 2 * al256(sizeof(H) * (size_t)Kp * 2 * Dp);
+}
+
+template <typename H>
+static MfmaTables<H> mfma_tables_carve(unsigned char* base, int Dp, int Kp) {
+  const size_t bS = al256(sizeof(MfmaScales)), bG = al256(sizeof(float) * Dp), bB = al256(sizeof(float) * Kp),
+               bT = al256(sizeof(float) * (size_t)Kp * Dp), bW = al256(sizeof(H) * (size_t)Kp * 2 * Dp);
+  MfmaTables<H> t;
+  t.scp = (MfmaScales*)base;
+  t.g = (float*)(base + bS);
+  t.bias = (float*)(base + bS + bG);
+  t.mup = (float*)(base + bS + bG + bB);
+  t.isg = t.mup + bT / sizeof(float);
+  t.sinv = t.isg + bT / sizeof(float);
+  t.Whi = (H*)(base + bS + bG + bB + 3 * bT);
+  t.Wlo = (H*)(base + bS + bG + bB + 3 * bT + bW);
+  return t;
+}
+
+// Zero fill by a kernel (capturable in a graph, unlike a memset node).
+__global__ void mfma_zero_kernel(unsigned* __restrict__ p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+static void zero_words(void* p, size_t words) {
+  if (words == 0) return;
+  VLHIP_LAUNCH("mfma_zero", mfma_zero_kernel, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, (unsigned*)p, words);
+}
+
+template <typename H>
+static void mfma_tables_launch(const MfmaTables<H>& t, const float* means, const float* covs, const float* priors,
+                               int D, int Dp, int K, int Kp) {
+  const bool isHalf = std::is_same<H, _Float16>::value;
+  zero_words(t.g, (size_t)Dp);
+  VLHIP_LAUNCH("mfma_center", (mfma_center_kernel<float, H>), dim3(1), dim3(256), 0, means, covs, priors, D, K, t.g,
+               t.scp, isHalf ? 1 : 0);
+  VLHIP_LAUNCH("mfma_tables", (mfma_tables_kernel<float, H>), dim3((Kp + 127) / 128), dim3(128), 0, means, covs,
+               priors, D, Dp, K, Kp, t.g, t.scp, t.Whi, t.Wlo, t.bias, t.mup, t.isg, t.sinv);
+}
+
+// Both passes over one batch of images [b0, b0 + n) whose descriptors start at
+// xb (nG groups, group table gtab with batch-relative rows).
+template <typename H, int NS, int CT, int WAVES>
+static void mfma_batch_launch(const MfmaTables<H>& t, const float* xb, int D, int K, int Kp, int flags,
+                              unsigned char* rec, size_t nG, const int4* gtab, const uint64_t* offsets,
+                              const long long* gstart, int b0, int n, const float* priors, float* enc,
+                              double* normpart, unsigned long long* numTerms, int numKB,
+                              const unsigned char* gflag, const int2* list, int M, size_t shmem) {
+  if (nG > 0) {
+    const unsigned lb = (unsigned)((nG + ML_GROUPS - 1) / ML_GROUPS);
+    if (flags & 4) {
+      VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, true>), dim3(lb), dim3(ML_GROUPS * 64), 0, xb, D, Kp, t.Whi,
+                   t.Wlo, t.bias, t.scp, t.g, rec, (long long)nG, gtab, gflag);
+    } else {
+      VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, false>), dim3(lb), dim3(ML_GROUPS * 64), 0, xb, D, Kp, t.Whi,
+                   t.Wlo, t.bias, t.scp, t.g, rec, (long long)nG, gtab, gflag);
+    }
+  }
+  const int padImages = n >= 8 ? ((n + 7) / 8) * 8 : n;
+  VLHIP_LAUNCH("mfma_fisher", (mfma_fisher_kernel<float, H, NS, CT, WAVES>), dim3((unsigned)((size_t)padImages * numKB)),
+               dim3(WAVES * 64), shmem, rec, offsets, gstart, b0, n, D, K, t.Whi, t.Wlo, t.bias, t.scp, t.mup,
+               t.isg, t.sinv, priors, flags, enc, normpart, numTerms, numKB, gflag, list, M);
+}
+
+template <typename H, int NS, int CT, int WAVES>
+static size_t mfma_shmem(bool gated) {
+  return 2 * GroupRec<NS>::kBytes + (gated ? (size_t)GR * (WAVES * 16 * CT) * sizeof(float) : 0);
+}
+
 template <typename H, int NS, int CT, int WAVES>
 static void run_mfma(float* enc, const float* means, const float* covs, const float* priors, size_t D, size_t K,
                      const float* x, const uint64_t* offsets, size_t numImages, int flags,
@@ -985,31 +1068,15 @@ static void run_mfma(float* enc, const float* means, const float* covs, const fl
   constexpr int BC = WAVES * 16 * CT;
   typedef GroupRec<NS> R;
   const int Kp = (int)((K + BC - 1) / BC) * BC;
-  const bool isHalf = sizeof(H) == 2 && std::is_same<H, _Float16>::value;
-  // tables: [scales][g Dp][bias Kp][mup, isg, sinv Kp*Dp][Whi, Wlo Kp*2Dp]
-  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-  const size_t bS = al(sizeof(MfmaScales)), bG = al(sizeof(float) * Dp), bB = al(sizeof(float) * Kp),
-               bT = al(sizeof(float) * (size_t)Kp * Dp), bW = al(sizeof(H) * (size_t)Kp * 2 * Dp);
-  unsigned char* base = (unsigned char*)scratch(SLOT_PREP, bS + bG + bB + 3 * bT + 2 * bW);
-  MfmaScales* scp = (MfmaScales*)base;
-  float* g = (float*)(base + bS);
-  float* bias = (float*)(base + bS + bG);
-  float* mup = (float*)(base + bS + bG + bB);
-  float* isg = mup + bT / sizeof(float);
-  float* sinv = isg + bT / sizeof(float);
-  H* Whi = (H*)(base + bS + bG + bB + 3 * bT);
-  H* Wlo = (H*)(base + bS + bG + bB + 3 * bT + bW);
-  VLHIP_CHECK(hipMemsetAsync(g, 0, bG, ctx().stream));
-  VLHIP_LAUNCH("mfma_center", (mfma_center_kernel<float, H>), dim3(1), dim3(256), 0, means, covs, priors, (int)D,
-               (int)K, g, scp, isHalf ? 1 : 0);
-  VLHIP_LAUNCH("mfma_tables", (mfma_tables_kernel<float, H>), dim3((Kp + 127) / 128), dim3(128), 0, means, covs,
-               priors, (int)D, Dp, (int)K, Kp, g, scp, Whi, Wlo, bias, mup, isg, sinv);
+  unsigned char* base = (unsigned char*)scratch(SLOT_PREP, mfma_tables_bytes<H>(Dp, Kp));
+  const MfmaTables<H> t = mfma_tables_carve<H>(base, Dp, Kp);
+  mfma_tables_launch<H>(t, means, covs, priors, (int)D, Dp, (int)K, Kp);
 
   const int numKB = Kp / BC;
   double* normpart = (double*)scratch(SLOT_PARTIAL, numImages * (size_t)numKB * sizeof(double));
-  if (numTerms) VLHIP_CHECK(hipMemsetAsync(numTerms, 0, numImages * sizeof(unsigned long long), ctx().stream));
+  if (numTerms) zero_words(numTerms, 2 * numImages);
   // two group records, plus the posterior tile of gate-served groups
-  const size_t shmem = 2 * R::kBytes + (gu ? (size_t)GR * BC * sizeof(float) : 0);
+  const size_t shmem = mfma_shmem<H, NS, CT, WAVES>(gu != nullptr);
   auto kern = mfma_fisher_kernel<float, H, NS, CT, WAVES>;
   VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
   // image-aligned groups: gstart[i] = sum_{j<i} ceil(n_j / GR)
@@ -1030,23 +1097,23 @@ static void run_mfma(float* enc, const float* means, const float* covs, const fl
     const int n = (int)(b1 - b0);
     unsigned char* gflag = nullptr;
     int2* list = nullptr;
+    int4* gtab = nullptr;
     if (nG > 0) {
-      int4* gtab = (int4*)scratch(SLOT_GROUPTAB, nG * sizeof(int4));
+      gtab = (int4*)scratch(SLOT_GROUPTAB, nG * sizeof(int4));
       VLHIP_LAUNCH("mfma_group_table", mfma_group_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                    offsets, gstart, (int)b0, n, gtab);
       if (gu) {
         // gate: candidates and confidence per row, then the groups it serves
-        auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
         int* cand = (int*)scratch(SLOT_GATE, (size_t)rows * gu->M * sizeof(int));
-        unsigned char* gb = (unsigned char*)scratch(SLOT_GATE2, al(rows * sizeof(float)) + al(rows) + al(nG));
+        unsigned char* gb = (unsigned char*)scratch(SLOT_GATE2, al256(rows * sizeof(float)) + al256(rows) + al256(nG));
         float* mass = (float*)gb;
-        unsigned char* conf = gb + al(rows * sizeof(float));
-        gflag = conf + al(rows);
+        unsigned char* conf = gb + al256(rows * sizeof(float));
+        gflag = conf + al256(rows);
         vlhip_gate_select(gu->gate, x + rowBase * D, rows, gu->M, gu->need, cand, mass, conf);
         list = (int2*)scratch(SLOT_GATELIST, nG * GR * gu->M * sizeof(int2));
         VLHIP_LAUNCH("mfma_sparse", mfma_sparse_kernel<NS>, dim3((unsigned)((nG + 3) / 4)), dim3(256), 0,
-                     x + rowBase * D, (int)D, (long long)nG, gtab, conf, cand, gu->M, bias, mup, sinv, g, scp, flags,
-                     gflag, list, numTerms);
+                     x + rowBase * D, (int)D, (long long)nG, gtab, conf, cand, gu->M, t.bias, t.mup, t.sinv, t.g,
+                     t.scp, flags, gflag, list, numTerms);
         std::vector<unsigned char> hf(nG);
         download(hf.data(), gflag, nG);
         size_t served = 0;
@@ -1054,19 +1121,10 @@ static void run_mfma(float* enc, const float* means, const float* covs, const fl
         ctx().gate_groups += nG;
         ctx().gate_served += served;
       }
-      const unsigned lb = (unsigned)((nG + ML_GROUPS - 1) / ML_GROUPS);
-      if (flags & 4) {
-        VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, true>), dim3(lb), dim3(ML_GROUPS * 64), 0,
-                     x + rowBase * D, (int)D, Kp, Whi, Wlo, bias, scp, g, rec, (long long)nG, gtab, gflag);
-      } else {
-        VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, false>), dim3(lb), dim3(ML_GROUPS * 64), 0,
-                     x + rowBase * D, (int)D, Kp, Whi, Wlo, bias, scp, g, rec, (long long)nG, gtab, gflag);
-      }
     }
-    const int padImages = n >= 8 ? ((n + 7) / 8) * 8 : n;
-    VLHIP_LAUNCH("mfma_fisher", kern, dim3((unsigned)((size_t)padImages * numKB)), dim3(WAVES * 64), shmem, rec,
-                 offsets, gstart, (int)b0, n, (int)D, (int)K, Whi, Wlo, bias, scp, mup, isg, sinv,
-                 priors, flags, enc, normpart, numTerms, numKB, gflag, list, gu ? gu->M : 0);
+    mfma_batch_launch<H, NS, CT, WAVES>(t, x + rowBase * D, (int)D, (int)K, Kp, flags, rec, nG, gtab, offsets, gstart,
+                                        (int)b0, n, priors, enc, normpart, numTerms, numKB, gflag, list,
+                                        gu ? gu->M : 0, shmem);
     b0 = b1;
   }
   if (flags & 2) {
@@ -1105,6 +1163,121 @@ static void mfma_dispatch(float* enc, const float* means, const float* covs, con
   }
 }
 
+// ---------------------------------------------------------------------------
+// Encode plans (batched uneven images, graph replay).  Everything that
+// depends only on the image sizes is prepared once: the image-aligned group
+// table of every batch, the batches (<= 8 GiB of records), device buffers,
+// and the component tiling, chosen by timing the candidate tilings on the
+// real shapes (autotuner, cached per shape).  A run is then a pure sequence of
+// kernel launches -- zero fills, tables, both passes per batch, the
+// normalisation and, optionally, the nearest-mean quantizer and the hard
+// VLAD -- captured into a hipGraph on the first run and replayed while the
+// argument pointers stay the same.
+
+struct PlanBatch {
+  size_t b0, n;       // images [b0, b0 + n)
+  uint64_t rowBase;   // first descriptor of the batch
+  size_t nG;          // groups of the batch
+  long long gOff;     // first group of the batch in the plan's group table
+};
+
+struct EncodePlan {
+  size_t D = 0, K = 0, numImages = 0, totalRows = 0;
+  int flags = 0, vladFlags = -1, NS = 0;
+  int variant = -1;            // component tiling: 0 = 8 waves x 32, 1 = 4 waves x 32
+  double variantMs[2] = {-1.0, -1.0};
+  std::vector<PlanBatch> batches;
+  size_t maxGroups = 0;
+  uint64_t* dOff = nullptr;
+  long long* dGstart = nullptr;
+  int4* dGtab = nullptr;
+  unsigned char* dRec = nullptr;
+  unsigned char* dTables = nullptr;
+  double* dNorm = nullptr;
+  uint32_t* dAssign = nullptr;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void* key[7] = {};
+};
+
+// autotuner cache: (D, K, flags, rows bucket) -> variant
+static std::vector<std::pair<std::vector<long long>, int>>& tuning_cache() {
+  static std::vector<std::pair<std::vector<long long>, int>> c;
+  return c;
+}
+
+template <typename H, int NS, int CT, int WAVES>
+static void plan_enqueue(EncodePlan& P, const float* x, const float* means, const float* covs, const float* priors,
+                         float* enc, float* vladEnc, unsigned long long* numTerms) {
+  constexpr int Dp = NS * 16;
+  constexpr int BC = WAVES * 16 * CT;
+  const int Kp = (int)((P.K + BC - 1) / BC) * BC;
+  const int numKB = Kp / BC;
+  const MfmaTables<H> t = mfma_tables_carve<H>(P.dTables, Dp, Kp);
+  mfma_tables_launch<H>(t, means, covs, priors, (int)P.D, Dp, (int)P.K, Kp);
+  if (numTerms) zero_words(numTerms, 2 * P.numImages);
+  const size_t shmem = mfma_shmem<H, NS, CT, WAVES>(false);
+  for (const PlanBatch& b : P.batches)
+    mfma_batch_launch<H, NS, CT, WAVES>(t, x + b.rowBase * P.D, (int)P.D, (int)P.K, Kp, P.flags, P.dRec, b.nG,
+                                        P.dGtab + b.gOff, P.dOff, P.dGstart, (int)b.b0, (int)b.n, priors, enc,
+                                        P.dNorm, numTerms, numKB, nullptr, nullptr, 0, shmem);
+  if (P.flags & 2) launch_fisher_normalize(enc, P.dNorm, numKB, P.numImages, (size_t)2 * P.K * P.D);
+  if (P.vladFlags >= 0 && vladEnc) {
+    quantize_t<float>(P.dAssign, nullptr, x, means, P.totalRows, P.D, P.K, 8);
+    vlad_t<float>(vladEnc, means, P.D, P.K, x, P.dOff, P.numImages, nullptr, P.dAssign, P.vladFlags);
+  }
+}
+
+template <typename H, int NS>
+static void plan_enqueue_variant(EncodePlan& P, int variant, const float* x, const float* means, const float* covs,
+                                 const float* priors, float* enc, float* vladEnc, unsigned long long* numTerms) {
+  if (variant == 0 && NS <= 5)
+    plan_enqueue<H, NS, 2, 8>(P, x, means, covs, priors, enc, vladEnc, numTerms);
+  else
+    plan_enqueue<H, NS, 2, 4>(P, x, means, covs, priors, enc, vladEnc, numTerms);
+}
+
+static void plan_enqueue_any(EncodePlan& P, int variant, const float* x, const float* means, const float* covs,
+                             const float* priors, float* enc, float* vladEnc, unsigned long long* numTerms) {
+  switch (P.NS) {
+    case 1: plan_enqueue_variant<_Float16, 1>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
+    case 2: plan_enqueue_variant<_Float16, 2>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
+    case 3: plan_enqueue_variant<_Float16, 3>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
+    case 4: plan_enqueue_variant<_Float16, 4>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
+    case 5: plan_enqueue_variant<_Float16, 5>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
+    default: plan_enqueue_variant<_Float16, 6>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
+  }
+}
+
+template <int NS, int CT, int WAVES>
+static void plan_set_attr() {
+  auto kern = mfma_fisher_kernel<float, _Float16, NS, CT, WAVES>;
+  VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)mfma_shmem<_Float16, NS, CT, WAVES>(false)));
+}
+static void plan_set_attrs(int NS) {
+  switch (NS) {
+    case 1: plan_set_attr<1, 2, 8>(); plan_set_attr<1, 2, 4>(); break;
+    case 2: plan_set_attr<2, 2, 8>(); plan_set_attr<2, 2, 4>(); break;
+    case 3: plan_set_attr<3, 2, 8>(); plan_set_attr<3, 2, 4>(); break;
+    case 4: plan_set_attr<4, 2, 8>(); plan_set_attr<4, 2, 4>(); break;
+    case 5: plan_set_attr<5, 2, 8>(); plan_set_attr<5, 2, 4>(); break;
+    default: plan_set_attr<6, 2, 4>(); break;
+  }
+}
+
+static size_t plan_record_bytes(int NS) {
+  switch (NS) {
+    case 1: return GroupRec<1>::kBytes;
+    case 2: return GroupRec<2>::kBytes;
+    case 3: return GroupRec<3>::kBytes;
+    case 4: return GroupRec<4>::kBytes;
+    case 5: return GroupRec<5>::kBytes;
+    default: return GroupRec<6>::kBytes;
+  }
+}
+
 void fisher_split16(float* enc, const float* means, const float* covs, const float* priors, size_t D, size_t K,
                     const float* x, const uint64_t* offsets, size_t numImages, int flags,
                     unsigned long long* numTerms, bool fp16, const GateUse* gu) {
@@ -1116,3 +1289,154 @@ void fisher_split16(float* enc, const float* means, const float* covs, const flo
 }
 
 }  // namespace vlhip
+
+using namespace vlhip;
+
+extern "C" {
+
+void* vlhip_encode_plan_new(size_t D, size_t K, uint64_t const* offsets_host, size_t numImages, int fisherFlags,
+                            int vladFlags) {
+  ensure_init();
+  std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+  VLHIP_REQUIRE(D >= 1 && D <= 96 && K >= 1, "encode plans need 1 <= dimension <= 96 and numClusters >= 1");
+  VLHIP_REQUIRE(numImages >= 1, "encode plans need at least one image");
+  EncodePlan* P = new EncodePlan();
+  P->D = D;
+  P->K = K;
+  P->numImages = numImages;
+  P->flags = fisherFlags;
+  P->vladFlags = vladFlags;
+  P->NS = (int)((D + 15) / 16);
+  P->totalRows = offsets_host[numImages] - offsets_host[0];
+  VLHIP_REQUIRE(offsets_host[0] == 0, "offsets must start at 0");
+  const size_t recBytes = plan_record_bytes(P->NS);
+  // image-aligned groups and batches
+  std::vector<long long> gs(numImages + 1);
+  gs[0] = 0;
+  for (size_t i = 0; i < numImages; ++i) {
+    VLHIP_REQUIRE(offsets_host[i + 1] >= offsets_host[i], "offsets must be non-decreasing");
+    gs[i + 1] = gs[i] + (long long)((offsets_host[i + 1] - offsets_host[i] + GR - 1) / GR);
+  }
+  std::vector<int4> tab((size_t)gs[numImages] ? (size_t)gs[numImages] : 1);
+  size_t b0 = 0;
+  while (b0 < numImages) {
+    size_t b1 = b0 + 1;
+    while (b1 < numImages && (size_t)(gs[b1 + 1] - gs[b0]) * recBytes <= kRecordBudget) ++b1;
+    PlanBatch b;
+    b.b0 = b0;
+    b.n = b1 - b0;
+    b.rowBase = offsets_host[b0];
+    b.nG = (size_t)(gs[b1] - gs[b0]);
+    b.gOff = gs[b0];
+    VLHIP_REQUIRE(offsets_host[b1] - offsets_host[b0] < (uint64_t(1) << 31), "image batch exceeds 2^31 descriptors");
+    for (size_t i = b0; i < b1; ++i)
+      for (long long k = 0; k < gs[i + 1] - gs[i]; ++k)
+        tab[(size_t)(gs[i] + k)] = make_int4((int)(offsets_host[i] - b.rowBase + (uint64_t)(GR * k)),
+                                             (int)(offsets_host[i + 1] - b.rowBase), (int)i, 0);
+    P->maxGroups = std::max(P->maxGroups, b.nG);
+    P->batches.push_back(b);
+    b0 = b1;
+  }
+  VLHIP_CHECK(hipMalloc(&P->dOff, (numImages + 1) * sizeof(uint64_t)));
+  VLHIP_CHECK(hipMalloc(&P->dGstart, (numImages + 1) * sizeof(long long)));
+  VLHIP_CHECK(hipMalloc(&P->dGtab, tab.size() * sizeof(int4)));
+  VLHIP_CHECK(hipMalloc(&P->dRec, (P->maxGroups ? P->maxGroups : 1) * recBytes));
+  const int Kp = (int)((K + 127) / 128) * 128 > (int)((K + 255) / 256) * 256 ? (int)((K + 127) / 128) * 128
+                                                                              : (int)((K + 255) / 256) * 256;
+  VLHIP_CHECK(hipMalloc(&P->dTables, mfma_tables_bytes<_Float16>(P->NS * 16, Kp)));
+  VLHIP_CHECK(hipMalloc(&P->dNorm, numImages * (size_t)(Kp / 128) * sizeof(double)));
+  if (vladFlags >= 0) VLHIP_CHECK(hipMalloc(&P->dAssign, (P->totalRows ? P->totalRows : 1) * sizeof(uint32_t)));
+  VLHIP_CHECK(hipMemcpy(P->dOff, offsets_host, (numImages + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
+  VLHIP_CHECK(hipMemcpy(P->dGstart, gs.data(), (numImages + 1) * sizeof(long long), hipMemcpyHostToDevice));
+  VLHIP_CHECK(hipMemcpy(P->dGtab, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice));
+  VLHIP_CHECK(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+  plan_set_attrs(P->NS);
+  return P;
+}
+
+void vlhip_encode_plan_delete(void* plan) {
+  if (!plan) return;
+  std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+  EncodePlan* P = (EncodePlan*)plan;
+  if (P->stream) VLHIP_CHECK(hipStreamSynchronize(P->stream));
+  if (P->exec) VLHIP_CHECK(hipGraphExecDestroy(P->exec));
+  if (P->graph) VLHIP_CHECK(hipGraphDestroy(P->graph));
+  void* bufs[] = {P->dOff, P->dGstart, P->dGtab, P->dRec, P->dTables, P->dNorm, P->dAssign};
+  for (void* b : bufs)
+    if (b) VLHIP_CHECK(hipFree(b));
+  if (P->stream) VLHIP_CHECK(hipStreamDestroy(P->stream));
+  delete P;
+}
+
+// One encode of all the plan's images (FP16 compensated Fisher vectors and,
+// when the plan has VLAD flags, hard-assignment VLAD against the means).
+// Returns when the results are complete.
+void vlhip_encode_plan_run(void* plan, void const* data, void const* means, void const* covs, void const* priors,
+                           void* fisherEnc, void* vladEnc, unsigned long long* numTerms) {
+  std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+  EncodePlan* P = (EncodePlan*)plan;
+  sync();  // inputs produced on the library stream are complete
+  const float* x = (const float*)data;
+  const void* key[7] = {data, means, covs, priors, fisherEnc, vladEnc, numTerms};
+  hipStream_t saved = ctx().stream;
+  const bool savedTiming = ctx().timing;
+  ctx().stream = P->stream;
+  if (P->variant < 0) {
+    // autotuner: reuse a cached choice for this shape, else time both tilings
+    std::vector<long long> shape = {(long long)P->D, (long long)P->K, (long long)P->flags,
+                                    (long long)(64 - __builtin_clzll((unsigned long long)P->totalRows | 1))};
+    for (auto& e : tuning_cache())
+      if (e.first == shape) P->variant = e.second;
+    if (P->variant < 0) {
+      hipEvent_t e0, e1;
+      VLHIP_CHECK(hipEventCreate(&e0));
+      VLHIP_CHECK(hipEventCreate(&e1));
+      ctx().timing = false;
+      for (int v = 0; v < 2; ++v) {
+        if (v == 0 && (P->NS > 5 || P->K <= 128)) continue;
+        plan_enqueue_any(*P, v, x, (const float*)means, (const float*)covs, (const float*)priors, (float*)fisherEnc,
+                         (float*)vladEnc, numTerms);  // warm-up
+        VLHIP_CHECK(hipEventRecord(e0, P->stream));
+        plan_enqueue_any(*P, v, x, (const float*)means, (const float*)covs, (const float*)priors, (float*)fisherEnc,
+                         (float*)vladEnc, numTerms);
+        VLHIP_CHECK(hipEventRecord(e1, P->stream));
+        VLHIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        VLHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        P->variantMs[v] = ms;
+      }
+      VLHIP_CHECK(hipEventDestroy(e0));
+      VLHIP_CHECK(hipEventDestroy(e1));
+      P->variant = (P->variantMs[0] >= 0.0 && (P->variantMs[1] < 0.0 || P->variantMs[0] <= P->variantMs[1])) ? 0 : 1;
+      tuning_cache().push_back({shape, P->variant});
+    }
+  }
+  bool same = P->exec != nullptr;
+  for (int i = 0; i < 7; ++i) same = same && P->key[i] == key[i];
+  if (!same) {
+    if (P->exec) VLHIP_CHECK(hipGraphExecDestroy(P->exec));
+    if (P->graph) VLHIP_CHECK(hipGraphDestroy(P->graph));
+    P->exec = nullptr;
+    P->graph = nullptr;
+    ctx().timing = false;  // no event records inside the capture
+    VLHIP_CHECK(hipStreamBeginCapture(P->stream, hipStreamCaptureModeRelaxed));
+    plan_enqueue_any(*P, P->variant, x, (const float*)means, (const float*)covs, (const float*)priors,
+                     (float*)fisherEnc, (float*)vladEnc, numTerms);
+    VLHIP_CHECK(hipStreamEndCapture(P->stream, &P->graph));
+    VLHIP_CHECK(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+    for (int i = 0; i < 7; ++i) P->key[i] = key[i];
+  }
+  ctx().timing = savedTiming;
+  ctx().stream = saved;
+  VLHIP_CHECK(hipGraphLaunch(P->exec, P->stream));
+  VLHIP_CHECK(hipStreamSynchronize(P->stream));
+}
+
+void vlhip_encode_plan_info(void* plan, int* variant, double* variantMs, size_t* numBatches) {
+  EncodePlan* P = (EncodePlan*)plan;
+  if (variant) *variant = P->variant;
+  if (variantMs) { variantMs[0] = P->variantMs[0]; variantMs[1] = P->variantMs[1]; }
+  if (numBatches) *numBatches = P->batches.size();
+}
+
+}  // extern "C"

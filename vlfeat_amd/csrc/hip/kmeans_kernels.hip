@@ -203,6 +203,9 @@ void l2_to_point_t(T* out, const T* x, const T* point, size_t N, size_t D, int l
   }
 }
 
+template void quantize_t<float>(uint32_t*, float*, const float*, const float*, size_t, size_t, size_t, int);
+template void quantize_t<double>(uint32_t*, double*, const double*, const double*, size_t, size_t, size_t, int);
+
 }  // namespace vlhip
 
 using namespace vlhip;

@@ -136,24 +136,135 @@ __global__ __launch_bounds__(256) void vlad_accum_kernel(
 }
 
 // vlad.c:230-241: enc /= max(||enc||, 1e-12), the squared norm summed in
-// index order by one lane (bitwise the reference's sequential loop).
+// index order by one lane (bitwise the reference's sequential loop); the
+// workgroup stages the entries through LDS in chunks so that the lane's
+// dependent additions do not wait on global-memory latency.
+constexpr int VL_NCHUNK = 4096;
 template <typename T>
 __global__ __launch_bounds__(256) void vlad_normalize_kernel(T* __restrict__ enc, size_t len) {
 #pragma clang fp contract(off)
   __shared__ T nsh;
+  __shared__ T buf[VL_NCHUNK];
   T* e = enc + (size_t)blockIdx.x * len;
-  if (threadIdx.x == 0) {
-    T n = T(0);
-    for (size_t i = 0; i < len; ++i) {
-      const T z = e[i];
-      n = n + z * z;
+  T n = T(0);
+  for (size_t c0 = 0; c0 < len; c0 += VL_NCHUNK) {
+    const int m = (int)min((size_t)VL_NCHUNK, len - c0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += blockDim.x) buf[i] = e[c0 + i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int i = 0; i < m; ++i) {
+        const T z = buf[i];
+        n = n + z * z;
+      }
     }
+  }
+  if (threadIdx.x == 0) {
     n = dev_sqrt<T>(n);
     nsh = (T)(((double)n > 1e-12) ? (double)n : 1e-12);
   }
   __syncthreads();
-  const T n = nsh;
-  for (size_t i = threadIdx.x; i < len; i += blockDim.x) e[i] = e[i] / n;
+  const T nn = nsh;
+  for (size_t i = threadIdx.x; i < len; i += blockDim.x) e[i] = e[i] / nn;
+}
+
+// Hard assignments (one cluster per descriptor): workgroup (image, block of
+// VH_COMPS clusters), thread d owns dimension d of every cluster of the block
+// and walks the image's descriptors in index order, adding x_id to the LDS
+// accumulator of the descriptor's cluster when it is in the block -- the same
+// sequence of roundings per (cluster, dimension) as vlad_accum_kernel with a
+// one-hot q (vlad.c:181-188), without scanning every cluster for every row.
+// Finalisation as vlad_accum_kernel (vlad.c:191-227).  D <= 128.
+template <typename T>
+struct VladHardBlock { static constexpr int kComps = sizeof(T) == 4 ? 256 : 96; };
+
+template <typename T>
+__global__ __launch_bounds__(128) void vlad_hard_kernel(const T* __restrict__ x, const uint64_t* __restrict__ offsets,
+                                                        int numImages, int D, int K, const T* __restrict__ means,
+                                                        const uint32_t* __restrict__ hard, int flags,
+                                                        T* __restrict__ enc, int numKB) {
+#pragma clang fp contract(off)
+  constexpr int CB = VladHardBlock<T>::kComps;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* acc = reinterpret_cast<T*>(smem_raw);                             // [CB][D]
+  double* mass = reinterpret_cast<double*>(smem_raw + (((size_t)CB * D * sizeof(T) + 15) & ~size_t(15)));
+  T* nrm = reinterpret_cast<T*>(mass + CB);                           // [CB]
+  const int b = blockIdx.x;
+  int img, kb;
+  if (numImages >= 8) {
+    const int xcd = b & 7, slot = b >> 3;
+    img = (slot / numKB) * 8 + xcd;
+    kb = slot % numKB;
+  } else {
+    img = b / numKB;
+    kb = b % numKB;
+  }
+  if (img >= numImages) return;
+  const int tid = threadIdx.x;
+  const int k0 = kb * CB;
+  for (int i = tid; i < CB * D; i += blockDim.x) acc[i] = T(0);
+  for (int i = tid; i < CB; i += blockDim.x) mass[i] = 0.0;
+  __syncthreads();
+  const long long rBegin = (long long)offsets[img], rEnd = (long long)offsets[img + 1];
+  const int d = tid;
+  for (long long r0 = rBegin; r0 < rEnd; r0 += 8) {
+    const int nr = (int)min((long long)8, rEnd - r0);
+    uint32_t kk[8];
+    T xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      kk[u] = u < nr ? hard[r0 + u] : 0xffffffffu;
+      xv[u] = (u < nr && d < D) ? x[(r0 + u) * D + d] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int kl = (int)kk[u] - k0;
+      if (kk[u] != 0xffffffffu && kl >= 0 && kl < CB) {
+        if (d < D) acc[kl * D + d] = (T)((double)acc[kl * D + d] + (double)xv[u]);
+        if (tid == 0) mass[kl] += 1.0;
+      }
+    }
+  }
+  __syncthreads();
+  // vlad.c:191-213 per (cluster, dimension), then the per-cluster norms
+  for (int i = tid; i < CB * D; i += blockDim.x) {
+    const int c = i / D, dd = i - c * D;
+    const int k = k0 + c;
+    if (k >= K) continue;
+    T e = acc[i];
+    const T mu = means[(size_t)k * D + dd];
+    const double ms = mass[c];
+    if (ms > 0) {
+      if (flags & 8) {
+        e = (T)((double)e / ms);
+        e = e - mu;
+      } else {
+        e = (T)((double)e - ms * (double)mu);
+      }
+    }
+    if (flags & 2) e = signed_sqrt(e);
+    acc[i] = e;
+  }
+  __syncthreads();
+  if (flags & 1) {
+    for (int c = tid; c < CB; c += blockDim.x) {
+      T n = T(0);
+      for (int dd = 0; dd < D; ++dd) {
+        const T z = acc[c * D + dd];
+        n = n + z * z;
+      }
+      n = dev_sqrt<T>(n);
+      nrm[c] = (T)(((double)n > 1e-12) ? (double)n : 1e-12);
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < CB * D; i += blockDim.x) {
+    const int c = i / D;
+    const int k = k0 + c;
+    if (k >= K) continue;
+    const T z = acc[i];
+    enc[(size_t)img * K * D + (size_t)k * D + (i - c * D)] = (flags & 1) ? z / nrm[c] : z;
+  }
 }
 
 template <typename T, int DPT>
@@ -163,7 +274,15 @@ static void launch_vlad(T* enc, const T* means, int D, int K, const T* x, const 
   const size_t shmem = ((size_t)VL_ROWS * (D + 1) + VL_ROWS * VL_COMPS + (size_t)VL_COMPS * D) * sizeof(T);
   const int padImages = numImages >= 8 ? ((numImages + 7) / 8) * 8 : numImages;
   const unsigned blocks = (unsigned)((size_t)padImages * numKB);
-  if (hard) {
+  if (hard && D <= 128) {
+    constexpr int CB = VladHardBlock<T>::kComps;
+    const int nkb = (K + CB - 1) / CB;
+    const size_t sh = (((size_t)CB * D * sizeof(T) + 15) & ~size_t(15)) + CB * sizeof(double) + CB * sizeof(T);
+    auto kern = vlad_hard_kernel<T>;
+    VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    VLHIP_LAUNCH("vlad_hard", kern, dim3((unsigned)((size_t)padImages * nkb)), dim3(128), sh, x, offsets, numImages,
+                 D, K, means, hard, flags, enc, nkb);
+  } else if (hard) {
     auto kern = vlad_accum_kernel<T, DPT, true>;
     VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
     VLHIP_LAUNCH("vlad_accum", kern, dim3(blocks), dim3(256), shmem, x, offsets, numImages, D, K, means, assign,
@@ -196,6 +315,11 @@ void vlad_t(T* enc, const T* means, size_t D, size_t K, const T* x, const uint64
   else if (D <= 256) launch_vlad<T, 16>(enc, means, d, k, x, offsets, n, assign, hard, flags);
   else launch_vlad<T, 32>(enc, means, d, k, x, offsets, n, assign, hard, flags);
 }
+
+template void vlad_t<float>(float*, const float*, size_t, size_t, const float*, const uint64_t*, size_t,
+                            const float*, const uint32_t*, int);
+template void vlad_t<double>(double*, const double*, size_t, size_t, const double*, const uint64_t*, size_t,
+                             const double*, const uint32_t*, int);
 
 }  // namespace vlhip
 

@@ -36,17 +36,27 @@ double run_estep(const T* x, size_t N, size_t D, size_t K, const T* means, const
 // Divides every image's encoding by max(||enc||, 1e-12) (fisher.c:474-485).
 void launch_fisher_normalize(float* enc, const double* normpart, int numKB, size_t numImages, size_t len);
 
-// Split-16 matrix-core Fisher encoder (fisher_mfma.hip); fp16 selects fp16
-// operands, otherwise bf16.
 // Learned-gate setting of the matrix-core encoder (nullptr: dense).
 struct GateUse {
   void* gate;
   int M;       // candidates per descriptor (<= 16)
   float need;  // predicted mass a descriptor needs to skip the dense E-step
 };
+
+// Split-16 matrix-core Fisher encoder (fisher_mfma.hip); fp16 selects fp16
+// operands, otherwise bf16.
 void fisher_split16(float* enc, const float* means, const float* covs, const float* priors, size_t D, size_t K,
                     const float* x, const uint64_t* offsets, size_t numImages, int flags,
                     unsigned long long* numTerms, bool fp16,
                     const GateUse* gate);
+
+// VLAD of a batch of images (vlad_kernels.hip); exactly one of assign / hard.
+template <typename T>
+void vlad_t(T* enc, const T* means, size_t D, size_t K, const T* x, const uint64_t* offsets, size_t numImages,
+            const T* assign, const uint32_t* hard, int flags);
+
+// Exact nearest-centre quantizer (kmeans_kernels.hip); dist may be nullptr.
+template <typename T>
+void quantize_t(uint32_t* assign, T* dist, const T* x, const T* centers, size_t N, size_t D, size_t K, int lanes);
 
 }  // namespace vlhip
