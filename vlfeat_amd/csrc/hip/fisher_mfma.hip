@@ -467,33 +467,45 @@ __global__ __launch_bounds__(ML_GROUPS * 64, 1) void mfma_lse_kernel(
   u32x4 wpre[STG];
   w_load<H, STG>(wpre, Whi, Wlo, 0, Dp, srcLo, srcOff);
 
-  // A fragments of this wave's 32 rows, resident for the whole sweep (16-byte
-  // loads where the row is whole and aligned; g holds Dp entries, zero past D)
+  // Stage the wave's 32 centred descriptors x' = x - g (zero past D and past
+  // the image) in LDS with 16-byte loads; the weight buffers are free until
+  // the sweep starts and hold exactly 8 waves x 32 rows x (Dp + 4) floats.
+  // Both operand layouts are then read from LDS.
+  constexpr int XS = Dp + 4;
+  static_assert(ML_GROUPS * GR * XS * sizeof(float) <= sizeof(wsh), "staging must fit the weight buffers");
   const bool vecx = (D & 3) == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0;
+  float* xsh = reinterpret_cast<float*>(&wsh[0][0][0][0]) + w * GR * XS;
+  for (int i = lane; i < GR * (Dp / 4); i += 64) {
+    const int r = i / (Dp / 4), c4 = (i - r * (Dp / 4)) * 4;
+    const long long row = row0 + r;
+    f32x4 v = f32x4{0.f, 0.f, 0.f, 0.f};
+    if (row < rowEnd) {
+      if (vecx && c4 + 4 <= D) {
+        v = *reinterpret_cast<const f32x4*>(x + row * D + c4);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] = c4 + e < D ? x[row * D + c4 + e] : 0.f;
+      }
+      const f32x4 gg = *reinterpret_cast<const f32x4*>(g + c4);  // g holds Dp entries, zero past D
+      v = v - gg;
+    }
+    *reinterpret_cast<f32x4*>(xsh + r * XS + c4) = v;
+  }
+  __syncthreads();
+  // A fragments of this wave's 32 rows, resident for the whole sweep
   V8 ahi[2][NS], alo[2][NS];
 #pragma unroll
   for (int j = 0; j < NS; ++j) {
     const int f0 = 32 * j + 8 * lg;
     const bool sq = f0 < Dp;
     const int d0 = sq ? f0 : f0 - Dp;
-    const f32x4 g0 = *reinterpret_cast<const f32x4*>(g + d0);
-    const f32x4 g1 = *reinterpret_cast<const f32x4*>(g + d0 + 4);
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      const long long row = row0 + 16 * t + lr;
+      const f32x4 a = *reinterpret_cast<const f32x4*>(xsh + (16 * t + lr) * XS + d0);
+      const f32x4 b = *reinterpret_cast<const f32x4*>(xsh + (16 * t + lr) * XS + d0 + 4);
       float xc[8];
-      if (vecx && row < rowEnd && d0 + 8 <= D) {
-        const f32x4 a = *reinterpret_cast<const f32x4*>(x + row * D + d0);
-        const f32x4 b = *reinterpret_cast<const f32x4*>(x + row * D + d0 + 4);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) { xc[v] = a[v] - g0[v]; xc[4 + v] = b[v] - g1[v]; }
-      } else {
-#pragma unroll
-        for (int v = 0; v < 8; ++v) {
-          const int d = d0 + v;
-          xc[v] = (row < rowEnd && d < D) ? x[row * D + d] - g[d] : 0.f;
-        }
-      }
+      for (int v = 0; v < 4; ++v) { xc[v] = a[v]; xc[4 + v] = b[v]; }
       make_frag<H>(xc, sq, s1, s2, ahi[t][j], alo[t][j]);
       if (live) {
         Ap[((t * NS + j) * 2 + 0) * 64 + lane] = ahi[t][j];
@@ -506,12 +518,10 @@ __global__ __launch_bounds__(ML_GROUPS * 64, 1) void mfma_lse_kernel(
 #pragma unroll
     for (int ft = 0; ft < NS; ++ft) {
       const int d = 16 * ft + lr;
-      const float gd = d < D ? g[d] : 0.f;
       V8 b1h, b1l, b2h, b2l;
 #pragma unroll
       for (int v = 0; v < 8; ++v) {
-        const long long row = row0 + 16 * (v >> 2) + 4 * lg + (v & 3);
-        const float xc = (row < rowEnd && d < D) ? x[row * D + d] - gd : 0.f;
+        const float xc = xsh[(16 * (v >> 2) + 4 * lg + (v & 3)) * XS + d];
         H h, l;
         split16<H>(xc * s1, h, l);
         b1h[v] = h;
@@ -526,6 +536,7 @@ __global__ __launch_bounds__(ML_GROUPS * 64, 1) void mfma_lse_kernel(
       Bp[(3 * NS + ft) * 64 + lane] = b2l;
     }
   }
+  __syncthreads();  // the staging area becomes the weight buffers
 
   LseState st;
 #pragma unroll
