@@ -131,7 +131,7 @@ VLHIP_API void vlhip_fisher_gate_stats (unsigned long long * groups, unsigned lo
  * VLAD of every image with each descriptor assigned to its nearest mean
  * (kmeans.c:655-702, vlad.c:163-242); its kernel sequence is captured into a
  * hipGraph on the first run and replayed while the pointers stay the same.
- * Float data, dimension <= 96.  Runs are synchronous. */
+ * Float data, dimension <= 128.  Runs are synchronous. */
 VLHIP_API void * vlhip_encode_plan_new (size_t dimension, size_t numClusters, uint64_t const * offsets_host,
                                         size_t numImages, int fisherFlags, int vladFlags) ;
 VLHIP_API void   vlhip_encode_plan_run (void * plan, void const * data, void const * means,

@@ -34,6 +34,8 @@ CASES = {
     "mix64_k64": (lambda: synth.mixture(3000, 64, 64, seed=0), 64),
     "sift40_k100": (lambda: synth.sift_like(2500, 40, seed=2), 100),
     "mix24_k37": (lambda: synth.mixture(1500, 24, 10, seed=3), 37),
+    "sift96_k160": (lambda: synth.sift_like(2000, 96, seed=5), 160),
+    "sift128_k200": (lambda: synth.sift_like(2000, 128, seed=6, project=False), 200),
 }
 
 

@@ -217,7 +217,8 @@ __device__ __forceinline__ void make_frag(const float (&xc)[8], bool square, flo
 // over rows does not depend on the order, only on A and B agreeing on it.
 constexpr int GR = 32;
 constexpr int MB_COMPS = 64;
-constexpr int ML_GROUPS = 8;  // groups (waves) per first-pass workgroup
+// groups (waves) per first-pass workgroup: 8, or 4 when D > 80 (registers)
+template <int NS> struct LseGroups { static constexpr int value = NS <= 5 ? 8 : 4; };
 
 template <int NS>
 struct GroupRec {
@@ -410,7 +411,7 @@ __device__ __forceinline__ void lse_fused(f32x4 (&accNew)[4][2], const f32x4 (&a
 }
 
 // ---------------------------------------------------------------------------
-// First pass.  Workgroup: 8 waves, wave = one group of 32 rows.  The wave
+// First pass.  Workgroup: 8 waves (4 when D > 80), wave = one group of 32 rows.  The wave
 // builds its A fragments from the fp32 descriptors, stores them and the B
 // fragments in the group record, then sweeps all components: the split
 // weights of 64 components at a time are double-buffered in LDS (the next
@@ -420,7 +421,7 @@ __device__ __forceinline__ void lse_fused(f32x4 (&accNew)[4][2], const f32x4 (&a
 // index of the maximum) merged across the 16 lanes that share rows at the end.
 // The log-sum-exp update of a block is issued with the products of the next.
 template <typename H, int NS, bool ARG>
-__global__ __launch_bounds__(ML_GROUPS * 64, 1) void mfma_lse_kernel(
+__global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
     const float* __restrict__ x, int D, int Kp,
     const H* __restrict__ Whi, const H* __restrict__ Wlo, const float* __restrict__ bias,
     const MfmaScales* __restrict__ scp, const float* __restrict__ g,
@@ -430,7 +431,8 @@ __global__ __launch_bounds__(ML_GROUPS * 64, 1) void mfma_lse_kernel(
   typedef GroupRec<NS> R;
   constexpr int Dp = NS * 16;
   constexpr int WS = 2 * Dp + 8;                 // LDS row stride (halves) of the staged weights
-  constexpr int NT = ML_GROUPS * 64;
+  constexpr int LG = LseGroups<NS>::value;
+  constexpr int NT = LG * 64;
   constexpr int PART = MB_COMPS * (2 * Dp / 8);  // 16-byte words of one weight part per block
   constexpr int STG = 2 * PART / NT;             // words per thread per block (= NS)
   static_assert(STG * NT == 2 * PART, "staging must divide evenly");
@@ -438,7 +440,7 @@ __global__ __launch_bounds__(ML_GROUPS * 64, 1) void mfma_lse_kernel(
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int lr = lane & 15, lg = lane >> 4;
-  const long long grp = (long long)blockIdx.x * ML_GROUPS + w;  // relative to the batch
+  const long long grp = (long long)blockIdx.x * LG + w;  // relative to the batch
   const bool live = grp < numGroups;  // uniform per wave; dead waves still take part in barriers
   // batch-relative first row of the group and end of its image
   long long row0 = 0, rowEnd = 0;
@@ -469,10 +471,10 @@ __global__ __launch_bounds__(ML_GROUPS * 64, 1) void mfma_lse_kernel(
 
   // Stage the wave's 32 centred descriptors x' = x - g (zero past D and past
   // the image) in LDS with 16-byte loads; the weight buffers are free until
-  // the sweep starts and hold exactly 8 waves x 32 rows x (Dp + 4) floats.
+  // the sweep starts and hold at least LG waves x 32 rows x (Dp + 4) floats.
   // Both operand layouts are then read from LDS.
   constexpr int XS = Dp + 4;
-  static_assert(ML_GROUPS * GR * XS * sizeof(float) <= sizeof(wsh), "staging must fit the weight buffers");
+  static_assert(LG * GR * XS * sizeof(float) <= sizeof(wsh), "staging must fit the weight buffers");
   const bool vecx = (D & 3) == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0;
   float* xsh = reinterpret_cast<float*>(&wsh[0][0][0][0]) + w * GR * XS;
   for (int i = lane; i < GR * (Dp / 4); i += 64) {
@@ -1051,12 +1053,13 @@ static void mfma_batch_launch(const MfmaTables<H>& t, const float* xb, int D, in
                               double* normpart, unsigned long long* numTerms, int numKB,
                               const unsigned char* gflag, const int2* list, int M, size_t shmem) {
   if (nG > 0) {
-    const unsigned lb = (unsigned)((nG + ML_GROUPS - 1) / ML_GROUPS);
+    constexpr int LG = LseGroups<NS>::value;
+    const unsigned lb = (unsigned)((nG + LG - 1) / LG);
     if (flags & 4) {
-      VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, true>), dim3(lb), dim3(ML_GROUPS * 64), 0, xb, D, Kp, t.Whi,
+      VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, true>), dim3(lb), dim3(LG * 64), 0, xb, D, Kp, t.Whi,
                    t.Wlo, t.bias, t.scp, t.g, rec, (long long)nG, gtab, gflag);
     } else {
-      VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, false>), dim3(lb), dim3(ML_GROUPS * 64), 0, xb, D, Kp, t.Whi,
+      VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, false>), dim3(lb), dim3(LG * 64), 0, xb, D, Kp, t.Whi,
                    t.Wlo, t.bias, t.scp, t.g, rec, (long long)nG, gtab, gflag);
     }
   }
@@ -1149,10 +1152,13 @@ template <typename H, int NS>
 static void run_mfma_k(float* enc, const float* means, const float* covs, const float* priors, size_t D, size_t K,
                        const float* x, const uint64_t* offsets, size_t numImages, int flags,
                        unsigned long long* numTerms, const std::vector<uint64_t>& off, const GateUse* gu) {
-  if (K > 128 && NS <= 5)
-    run_mfma<H, NS, 2, 8>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off, gu);
-  else
-    run_mfma<H, NS, 2, 4>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off, gu);
+  if constexpr (NS <= 5) {
+    if (K > 128) {
+      run_mfma<H, NS, 2, 8>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off, gu);
+      return;
+    }
+  }
+  run_mfma<H, NS, 2, 4>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off, gu);
 }
 
 template <typename H>
@@ -1169,8 +1175,10 @@ static void mfma_dispatch(float* enc, const float* means, const float* covs, con
     case 64: run_mfma_k<H, 4>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off, gu); break;
     case 80: run_mfma_k<H, 5>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off, gu); break;
     case 96: run_mfma_k<H, 6>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off, gu); break;
+    case 112: run_mfma_k<H, 7>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off, gu); break;
+    case 128: run_mfma_k<H, 8>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off, gu); break;
     default: fatal("vlhip_fisher_encode_batch", __FILE__, __LINE__,
-                   "the matrix-core Fisher encoder supports dimension <= 96");
+                   "the matrix-core Fisher encoder supports dimension <= 128");
   }
 }
 
@@ -1243,10 +1251,13 @@ static void plan_enqueue(EncodePlan& P, const float* x, const float* means, cons
 template <typename H, int NS>
 static void plan_enqueue_variant(EncodePlan& P, int variant, const float* x, const float* means, const float* covs,
                                  const float* priors, float* enc, float* vladEnc, unsigned long long* numTerms) {
-  if (variant == 0 && NS <= 5)
-    plan_enqueue<H, NS, 2, 8>(P, x, means, covs, priors, enc, vladEnc, numTerms);
-  else
-    plan_enqueue<H, NS, 2, 4>(P, x, means, covs, priors, enc, vladEnc, numTerms);
+  if constexpr (NS <= 5) {
+    if (variant == 0) {
+      plan_enqueue<H, NS, 2, 8>(P, x, means, covs, priors, enc, vladEnc, numTerms);
+      return;
+    }
+  }
+  plan_enqueue<H, NS, 2, 4>(P, x, means, covs, priors, enc, vladEnc, numTerms);
 }
 
 static void plan_enqueue_any(EncodePlan& P, int variant, const float* x, const float* means, const float* covs,
@@ -1257,7 +1268,9 @@ static void plan_enqueue_any(EncodePlan& P, int variant, const float* x, const f
     case 3: plan_enqueue_variant<_Float16, 3>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
     case 4: plan_enqueue_variant<_Float16, 4>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
     case 5: plan_enqueue_variant<_Float16, 5>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
-    default: plan_enqueue_variant<_Float16, 6>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
+    case 6: plan_enqueue_variant<_Float16, 6>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
+    case 7: plan_enqueue_variant<_Float16, 7>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
+    default: plan_enqueue_variant<_Float16, 8>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
   }
 }
 
@@ -1274,7 +1287,9 @@ static void plan_set_attrs(int NS) {
     case 3: plan_set_attr<3, 2, 8>(); plan_set_attr<3, 2, 4>(); break;
     case 4: plan_set_attr<4, 2, 8>(); plan_set_attr<4, 2, 4>(); break;
     case 5: plan_set_attr<5, 2, 8>(); plan_set_attr<5, 2, 4>(); break;
-    default: plan_set_attr<6, 2, 4>(); break;
+    case 6: plan_set_attr<6, 2, 4>(); break;
+    case 7: plan_set_attr<7, 2, 4>(); break;
+    default: plan_set_attr<8, 2, 4>(); break;
   }
 }
 
@@ -1285,7 +1300,9 @@ static size_t plan_record_bytes(int NS) {
     case 3: return GroupRec<3>::kBytes;
     case 4: return GroupRec<4>::kBytes;
     case 5: return GroupRec<5>::kBytes;
-    default: return GroupRec<6>::kBytes;
+    case 6: return GroupRec<6>::kBytes;
+    case 7: return GroupRec<7>::kBytes;
+    default: return GroupRec<8>::kBytes;
   }
 }
 
@@ -1309,7 +1326,7 @@ void* vlhip_encode_plan_new(size_t D, size_t K, uint64_t const* offsets_host, si
                             int vladFlags) {
   ensure_init();
   std::lock_guard<std::recursive_mutex> lk(ctx().mu);
-  VLHIP_REQUIRE(D >= 1 && D <= 96 && K >= 1, "encode plans need 1 <= dimension <= 96 and numClusters >= 1");
+  VLHIP_REQUIRE(D >= 1 && D <= 128 && K >= 1, "encode plans need 1 <= dimension <= 128 and numClusters >= 1");
   VLHIP_REQUIRE(numImages >= 1, "encode plans need at least one image");
   EncodePlan* P = new EncodePlan();
   P->D = D;
