@@ -61,7 +61,7 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def fit_gmm(x_host: np.ndarray, K: int, iters: int, seed: int):
+def fit_gmm(x_host: np.ndarray, K: int, iters: int, seed: int, cov_bound: float = 1e-6):
     """GMM for the workload: EM on the GPU from random data points."""
     import vlfeat_amd as va
     rng = np.random.default_rng(seed)
@@ -69,7 +69,7 @@ def fit_gmm(x_host: np.ndarray, K: int, iters: int, seed: int):
     covs = np.tile(np.var(x_host, 0), (K, 1)).astype(np.float32)
     pri = np.full(K, 1.0 / K, np.float32)
     m, c, p, ll, _ = va.vl_gmm(x_host, K, init="custom", means=means, covs=covs, priors=pri,
-                               max_iter=iters, cov_bound=1e-6, want_post=False)
+                               max_iter=iters, cov_bound=cov_bound, want_post=False)
     return m, c, p, ll
 
 
@@ -213,6 +213,224 @@ def batched_config(args) -> int:
     return 0
 
 
+def _em_finalize(out: np.ndarray, m_old: np.ndarray, K: int, D: int, lb: float):
+    """M-step finalisation of gmm.c:1169-1202 in FP64, applied alike to every
+    path's moments [S0 | mass | S1 | S2] for the error report."""
+    S0 = out[:K]
+    S1 = out[2 * K:2 * K + K * D].reshape(K, D)
+    S2 = out[2 * K + K * D:].reshape(K, D)
+    upd = (S0 >= 1e-6 / K)[:, None]
+    s0 = np.where(S0 > 0, S0, 1.0)[:, None]
+    mu = np.where(upd, S1 / s0, S1)
+    cv = np.maximum(np.where(upd, S2 / s0 - (mu - m_old) ** 2, S2), lb)
+    pri = S0 / max(S0.sum(), 1e-12)
+    return {"S0": S0, "S1": S1, "S2": S2, "means": mu, "covariances": cv, "priors": pri}
+
+
+def _err(a, b) -> dict:
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    d = np.abs(a - b)
+    nz = np.abs(b) > 0
+    return {"max_abs": float(d.max()), "max_rel": float((d[nz] / np.abs(b[nz])).max()) if nz.any() else 0.0,
+            "rel_l2": float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))}
+
+
+def em_config(args) -> int:
+    """Precision form of the EM iteration: N = 4,000,000 SIFT-like 128-d
+    descriptors (rectified, L2-normalised, clipped at 0.2, renormalised; not
+    projected), K = 1024.  A step is one EM iteration on device-resident data:
+    the log-likelihood and the M-step moments of the current model
+    (vlhip_gmm_em_step) in the compensated FP16 (value) and BF16 matrix-core
+    modes, and the FP32 parity path (posterior matrix + moments) beside them.
+    The model is a 5-iteration FP32 EM fit on 500k of the descriptors
+    (variance floor 1e-4).  Errors per output (log-likelihood, moments,
+    means / covariances / priors after the M-step) against FP64:
+      * on all N descriptors against the library's FP64 path (double data);
+      * on the first --em-oracle-rows descriptors against the FP64 CPU oracle,
+        which also pins that FP64 GPU path.
+    cpu_baseline: the reference's OpenMP EM (oracle/_ref), one iteration timed
+    as the difference of 2- and 1-iteration runs on a bounded sample."""
+    import threading
+
+    import torch
+    import vlfeat_amd as va
+    from vlfeat_amd import synth
+
+    N, D, K = args.em_n, 128, 1024
+    lb = 1e-4
+    L = va.lib()
+    L.vlhip_init()
+    dev = torch.device("cuda")
+    log(f"em: generating {N} descriptors (D={D}) on the GPU")
+    x = synth.sift_like_torch(N, D, seed=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    m, c, p, ll_fit = fit_gmm(x[:500_000].cpu().numpy(), K, iters=5, seed=0, cov_bound=lb)
+    log(f"em: model fitted in {time.perf_counter() - t0:.1f} s (FP32 EM, 5 iterations, 500k descriptors)")
+    md, cd, pd = (torch.as_tensor(a, device=dev) for a in (m, c, p))
+    KD = K * D
+    outs = {}
+
+    def mfma_step(mode, xp, n):
+        o = np.zeros(2 * K + 2 * KD)
+        ll = L.vlhip_gmm_em_step(mode, o.ctypes.data, pd.data_ptr(), md.data_ptr(), cd.data_ptr(), D, K, xp, n)
+        return ll, o
+
+    q32 = torch.empty((N, K), dtype=torch.float32, device=dev)
+
+    def fp32_step(xp, n):
+        o = np.zeros(2 * K + 2 * KD)
+        ll = L.vlhip_gmm_posteriors(1, q32.data_ptr(), K, n, pd.data_ptr(), md.data_ptr(), D, cd.data_ptr(), xp)
+        L.vlhip_gmm_moments(1, o.ctypes.data, q32.data_ptr(), xp, md.data_ptr(), n, D, K, 0.01 / K)
+        return ll, o
+
+    steps = {"f16x3": lambda: mfma_step(va.MODE_F16X3, x.data_ptr(), N),
+             "bf16x3": lambda: mfma_step(va.MODE_BF16X3, x.data_ptr(), N),
+             "fp32": lambda: fp32_step(x.data_ptr(), N)}
+    timing = {}
+    for name, fn in steps.items():
+        for _ in range(args.warmup):
+            fn()
+        L.vlhip_timing_reset()
+        L.vlhip_timing_enable(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ll, o = fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        L.vlhip_timing_enable(0)
+        kt = va.kernel_timings()
+        timing[name] = {"ms_per_iteration": dt * 1000, "descriptors_per_s": N / dt,
+                        "kernels_ms_per_iteration": {k: v[0] / args.steps for k, v in kt.items() if v[1]},
+                        "kernel_launches": {k: v[1] for k, v in kt.items() if v[1]}}
+        outs[name] = (ll, o)
+        log(f"em: {name}: {dt * 1000:.1f} ms per EM iteration; " +
+            ", ".join(f"{k}={v[0] / args.steps:.2f}ms" for k, v in kt.items() if v[1]))
+    del q32
+
+    # FP64 over all N descriptors (the library's double path)
+    x64 = x.double()
+    m64, c64, p64 = (t.double() for t in (md, cd, pd))
+    q64 = torch.empty((N, K), dtype=torch.float64, device=dev)
+    o64 = np.zeros(2 * K + 2 * KD)
+    ll64 = L.vlhip_gmm_posteriors(2, q64.data_ptr(), K, N, p64.data_ptr(), m64.data_ptr(), D, c64.data_ptr(),
+                                  x64.data_ptr())
+    L.vlhip_gmm_moments(2, o64.ctypes.data, q64.data_ptr(), x64.data_ptr(), m64.data_ptr(), N, D, K, 0.01 / K)
+    torch.cuda.synchronize()
+    del q64
+    f64 = _em_finalize(o64, m.astype(np.float64), K, D, lb)
+    full = {}
+    for name, (ll, o) in outs.items():
+        fo = _em_finalize(o, m.astype(np.float64), K, D, lb)
+        e = {"loglikelihood_rel": abs(ll - ll64) / abs(ll64), "mass": _err(o[K:2 * K], o64[K:2 * K])}
+        e.update({k: _err(fo[k], f64[k]) for k in ("S0", "S1", "S2", "means", "covariances", "priors")})
+        full[name] = e
+        log(f"em: {name} vs FP64 (all N): LL {e['loglikelihood_rel']:.2e}, means {e['means']['rel_l2']:.2e}, "
+            f"covs {e['covariances']['rel_l2']:.2e}, priors {e['priors']['rel_l2']:.2e}")
+
+    # FP64 CPU oracle on a subsample (threads over row blocks)
+    from oracle import oracle, reference
+    O = oracle()
+    ns = min(args.em_oracle_rows, N)
+    xs = x[:ns].cpu().numpy()
+    nth = 16
+    parts = [None] * nth
+    bounds = np.linspace(0, ns, nth + 1).astype(int)
+
+    def work(i):
+        parts[i] = O.posteriors(xs[bounds[i]:bounds[i + 1]], m, c, p)
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=work, args=(i,)) for i in range(nth)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    qo = np.concatenate([q for q, _ in parts])
+    llo = float(sum(l for _, l in parts))
+    log(f"em: oracle posteriors of {ns} descriptors in {time.perf_counter() - t0:.1f} s ({nth} threads)")
+    xo = xs.astype(np.float64)
+    qt = np.where(qo < 0.01 / K, 0.0, qo)
+    S0 = qt.sum(0)
+    S1 = qt.T @ xo
+    mo = m.astype(np.float64)
+    S2 = qt.T @ (xo * xo) - 2 * mo * S1 + mo * mo * S0[:, None]
+    oo = np.concatenate([S0, qo.sum(0), S1.ravel(), S2.ravel()])
+    fo_ref = _em_finalize(oo, m.astype(np.float64), K, D, lb)
+    sub_outs = {"f16x3": mfma_step(va.MODE_F16X3, x.data_ptr(), ns), "bf16x3": mfma_step(va.MODE_BF16X3, x.data_ptr(), ns)}
+    q32 = torch.empty((ns, K), dtype=torch.float32, device=dev)
+    sub_outs["fp32"] = fp32_step(x.data_ptr(), ns)
+    del q32
+    q64 = torch.empty((ns, K), dtype=torch.float64, device=dev)
+    o64s = np.zeros(2 * K + 2 * KD)
+    ll64s = L.vlhip_gmm_posteriors(2, q64.data_ptr(), K, ns, p64.data_ptr(), m64.data_ptr(), D, c64.data_ptr(),
+                                   x64.data_ptr())
+    L.vlhip_gmm_moments(2, o64s.ctypes.data, q64.data_ptr(), x64.data_ptr(), m64.data_ptr(), ns, D, K, 0.01 / K)
+    sub_outs["fp64_gpu"] = (ll64s, o64s)
+    del q64, x64
+    sub = {}
+    for name, (ll, o) in sub_outs.items():
+        fo = _em_finalize(o, m.astype(np.float64), K, D, lb)
+        e = {"loglikelihood_rel": abs(ll - llo) / abs(llo), "mass": _err(o[K:2 * K], oo[K:2 * K])}
+        e.update({k: _err(fo[k], fo_ref[k]) for k in ("S0", "S1", "S2", "means", "covariances", "priors")})
+        sub[name] = e
+        log(f"em: {name} vs FP64 oracle ({ns} rows): LL {e['loglikelihood_rel']:.2e}, "
+            f"means {e['means']['rel_l2']:.2e}, covs {e['covariances']['rel_l2']:.2e}")
+
+    # roofline of the dominant matrix-core kernel of the F16X3 iteration
+    kt = timing["f16x3"]["kernels_ms_per_iteration"]
+    kl = timing["f16x3"]["kernel_launches"]
+    flop_desc = {"mfma_em_moments": 8 * D * K, "mfma_lse": 4 * D * K}
+    kname = max((k for k in kt if k in flop_desc), key=lambda k: kt[k])
+    per_launch_ms = kt[kname] * args.steps / kl[kname]
+    desc_per_launch = N / (kl[kname] / args.steps)
+    achieved = flop_desc[kname] * desc_per_launch / (per_launch_ms / 1000) / 1e12
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_F16_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_F16_TFLOPS, "traffic": None, "kernel": kname, "kernel_avg_ms": per_launch_ms,
+                "flop_per_descriptor": flop_desc[kname], "descriptors_per_launch": desc_per_launch}
+
+    cpu = None
+    R = reference()
+    if R is not None and not args.no_cpu:
+        R.set_num_threads(0)
+        threads = int(R.vl_get_max_threads())
+        nc = min(args.em_cpu_rows, N)
+        xc = x[:nc].cpu().numpy()
+        # a 1-iteration vl_gmm_em run is E-step, M-step, E-step: one iteration
+        # (E + M) is its time minus that of one E-step
+        t0 = time.perf_counter()
+        R.gmm_posteriors(xc, m, c, p)
+        t1 = time.perf_counter()
+        R.gmm(xc, K, init="custom", means=m, covs=c, priors=p, max_iter=1, want_post=False)
+        t2 = time.perf_counter()
+        it_s = max((t2 - t1) - (t1 - t0), 1e-9)
+        cpu = {"value": nc / it_s, "unit": "descriptors/s", "cores": threads, "kind": "reference",
+               "sample": f"{nc} descriptors, K={K}, D={D}: one EM iteration (E-step + M-step) of the reference's "
+                         f"vl_gmm_em (OpenMP), timed as a 1-iteration vl_gmm_cluster run ({t2 - t1:.1f} s) minus "
+                         f"one vl_get_gmm_data_posteriors_f call ({t1 - t0:.1f} s)"}
+        log(f"em: reference CPU {cpu['value']:.3g} descriptors/s per EM iteration ({threads} threads)")
+
+    result = {
+        "metric": "EM iteration descriptors/s at N=4M, D=128, K=1024 (precision form)",
+        "value": timing["f16x3"]["descriptors_per_s"], "unit": "descriptors/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": timing["f16x3"]["ms_per_iteration"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp16",
+        "data": "synthetic (SIFT-like 128-d, not projected, seed 1; model: 5-iteration FP32 EM fit)",
+        "config": {"workload": "one EM iteration (LL + M-step moments) over 4M descriptors, K=1024, D=128",
+                   "N": N, "K": K, "D": D, "variance_floor": lb},
+        "modes": timing,
+        "speedup_f16x3_vs_fp32_gpu": timing["fp32"]["ms_per_iteration"] / timing["f16x3"]["ms_per_iteration"],
+        "speedup_bf16x3_vs_fp32_gpu": timing["fp32"]["ms_per_iteration"] / timing["bf16x3"]["ms_per_iteration"],
+        "errors_vs_fp64_gpu_all_rows": full,
+        "errors_vs_fp64_oracle_subsample": {"rows": ns, "per_mode": sub},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,8 +445,12 @@ def main() -> int:
     ap.add_argument("--no-gated", action="store_true", help="skip the learned-gate leg")
     ap.add_argument("--gate-train", type=int, default=2_000_000, help="descriptors the gate is trained on")
     ap.add_argument("--gate-need", type=float, default=0.999)
-    ap.add_argument("--config", default="headline", choices=["headline", "batched"],
-                    help="batched: the uneven-image Fisher+VLAD plan benchmark (K=512) instead of the headline")
+    ap.add_argument("--config", default="headline", choices=["headline", "batched", "em"],
+                    help="batched: the uneven-image Fisher+VLAD plan benchmark (K=512); em: the EM precision "
+                         "form (N=4M, D=128, K=1024) instead of the headline")
+    ap.add_argument("--em-n", type=int, default=4_000_000)
+    ap.add_argument("--em-oracle-rows", type=int, default=65536)
+    ap.add_argument("--em-cpu-rows", type=int, default=65536)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -238,6 +460,8 @@ def main() -> int:
         return 2
     if args.config == "batched":
         return batched_config(args)
+    if args.config == "em":
+        return em_config(args)
 
     import torch
     import vlfeat_amd as va

@@ -70,4 +70,16 @@ VL_EXPORT VlGMMInitialization vl_gmm_get_initialization (VlGMM const * self) ;
 VL_EXPORT VlKMeans * vl_gmm_get_kmeans_init_object (VlGMM const * self) ;
 VL_EXPORT double const * vl_gmm_get_covariance_lower_bounds (VlGMM const * self) ;
 
+/* No counterpart in the reference (whose EM has one float path): arithmetic
+ * of the EM iterations of a float mixture.  VL_GMM_PRECISION_FP32, the
+ * default, follows the reference's float computation; the compensated
+ * 16-bit modes run the E-step and the M-step moments as matrix-core
+ * contractions (dimension <= 128) with FP64 log-determinants, log-likelihood
+ * and moment reductions.  Double mixtures ignore the setting. */
+#define VL_GMM_PRECISION_FP32     0
+#define VL_GMM_PRECISION_BF16X3   1
+#define VL_GMM_PRECISION_F16X3    3
+VL_EXPORT void vl_gmm_set_precision (VlGMM * self, int precision) ;
+VL_EXPORT int vl_gmm_get_precision (VlGMM const * self) ;
+
 #endif /* VL_GMM_H */

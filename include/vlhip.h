@@ -87,6 +87,20 @@ VLHIP_API void vlhip_gmm_moments (int dtype, double * out_host,
                                   void const * posteriors, void const * data, void const * oldMeans,
                                   size_t numData, size_t dimension, size_t numClusters, double threshold) ;
 
+/* One EM step on the matrix cores for float data, dimension <= 128 (mode
+ * VLHIP_MODE_BF16X3 or VLHIP_MODE_F16X3).  Replaces, for the EM loop of
+ * gmm.c:1220-1308, the E-step of vlhip_gmm_posteriors (gmm.c:711-798) and
+ * the moments of vlhip_gmm_moments (gmm.c:1064-1167) with compensated 16-bit
+ * contractions accumulated in FP32 and reduced in FP64; the posterior matrix
+ * is never materialised.  Returns the log-likelihood of the parameters
+ * (FP64 sum); when out_host is not NULL also fills it, as vlhip_gmm_moments
+ * does, with the M-step moments of these posteriors against oldMeans = means
+ * (threshold 0.01 / numClusters).  All array arguments are device pointers
+ * except out_host. */
+VLHIP_API double vlhip_gmm_em_step (int mode, double * out_host, void const * priors, void const * means,
+                                    void const * covariances, size_t dimension, size_t numClusters,
+                                    void const * data, size_t numData) ;
+
 /* Empty-mode split, gmm.c:968-990: datum i moves its mass p_ij + p_ii to
  * component iCl when data[i, dim] < mu, otherwise to jCl. */
 VLHIP_API void vlhip_gmm_split (int dtype, void * posteriors, void const * data, size_t numData,

@@ -109,6 +109,10 @@ _SIGS = {
 _EXT_SIGS = {
     "vl_fisher_encode_batch": (None, [_p, C.c_uint32, _p, _sz, _sz, _p, _p, _p, _p, _sz, C.c_int, _p]),
     "vl_fisher_set_precision": (None, [C.c_int]),
+    "vl_gmm_set_precision": (None, [_p, C.c_int]),
+    "vl_gmm_get_precision": (C.c_int, [_p]),
+    "vlhip_gmm_em_step": (C.c_double, [C.c_int, _p, _p, _p, _p, C.c_size_t, C.c_size_t, _p, C.c_size_t]),
+    "vlhip_gmm_moments": (None, [C.c_int, _p, _p, _p, _p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_double]),
     "vl_fisher_get_precision": (C.c_int, []),
     "vl_fisher_set_gate": (None, [_p, C.c_int, C.c_float]),
     "vlhip_fisher_set_gate": (None, [_p, C.c_int, C.c_float]),
@@ -197,8 +201,9 @@ class CLib:
         return q, ll
 
     def gmm(self, data, K, *, init="custom", means=None, covs=None, priors=None, max_iter=50,
-            reps=1, cov_bound=None, kmeans=None, verbosity=0, want_post=True):
-        """vl_gmm-style driver: returns (means, covs, priors, LL, posteriors)."""
+            reps=1, cov_bound=None, kmeans=None, verbosity=0, want_post=True, precision=0):
+        """vl_gmm-style driver: returns (means, covs, priors, LL, posteriors).
+        precision: VL_GMM_PRECISION_* of the EM iterations (0 = FP32 parity path)."""
         data = np.ascontiguousarray(data)
         N, D = data.shape
         t = dtype_code(data)
@@ -207,6 +212,8 @@ class CLib:
             self.vl_gmm_set_max_num_iterations(g, max_iter)
             self.vl_gmm_set_num_repetitions(g, reps)
             self.vl_gmm_set_verbosity(g, verbosity)
+            if precision:
+                self.vl_gmm_set_precision(g, int(precision))
             if cov_bound is not None:
                 if np.ndim(cov_bound) == 0:
                     self.vl_gmm_set_covariance_lower_bound(g, float(cov_bound))

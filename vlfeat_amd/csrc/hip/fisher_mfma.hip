@@ -750,14 +750,23 @@ __device__ __forceinline__ void rec_store(const u32x4 (&pre)[PER], unsigned char
 //            S1 += q x'_hi + q x'_lo,  S2 += q x'^2_hi + q x'^2_lo.
 // The finalisation works on the accumulators in registers: lane l holds
 // features 16ft + (l & 15) of components 16c + 4(l >> 4) + r.
-template <typename T, typename H, int NS, int CT, int WAVES>
+//
+// With EM the "images" are consecutive chunks of an EM data set and the
+// kernel produces the M-step moments of gmm.c:1093-1149 instead of a Fisher
+// vector: posteriors below emThr (the M-step threshold 0.01/K, scaled by sq)
+// are dropped, the unthresholded mass is summed on the side (FP32 per group,
+// FP64 across groups), and the chunk's raw moments about g,
+//   [S0 K | mass K | sum q x' (K x D) | sum q x'^2 (K x D)]   (FP64),
+// are written to emOut[chunk].
+template <typename T, typename H, int NS, int CT, int WAVES, bool EM = false>
 __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
     const unsigned char* __restrict__ rec, const uint64_t* __restrict__ offsets, const long long* __restrict__ gstart,
     int imgBegin, int numImages, int D, int K, const H* __restrict__ Whi, const H* __restrict__ Wlo,
     const float* __restrict__ bias, const MfmaScales* __restrict__ scp, const float* __restrict__ mup,
     const float* __restrict__ isg, const float* __restrict__ sinv, const T* __restrict__ priors, int flags,
     T* __restrict__ enc, double* __restrict__ normpart, unsigned long long* __restrict__ numTerms, int numKB,
-    const unsigned char* __restrict__ gflag, const int2* __restrict__ list, int M) {
+    const unsigned char* __restrict__ gflag, const int2* __restrict__ list, int M, float emThr,
+    double* __restrict__ emOut) {
   typedef typename VecOf<H>::type V8;
   typedef GroupRec<NS> R;
   constexpr int Dp = NS * 16;
@@ -779,7 +788,7 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
   const long long nImg = (long long)offsets[img + 1] - (long long)offsets[img];
   const long long gBase = gstart[imgBegin];
   const long long gB = gstart[img] - gBase, gE = gstart[img + 1] - gBase;  // this image's records
-  const bool fast = (flags & 4) != 0;
+  const bool fast = !EM && (flags & 4) != 0;
   const MfmaScales sc = *scp;
 
   V8 whi[CT][NS], wlo[CT][NS];
@@ -808,9 +817,12 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
 #pragma unroll
   for (int v = 0; v < 8; ++v) ones[v] = H(1.f);
   unsigned terms = 0;
+  double massd[CT];  // EM: unthresholded posterior mass of component kw + 16c + lr
+#pragma unroll
+  for (int c = 0; c < CT; ++c) massd[c] = 0.0;
   // (double)q < 1e-6 for a float q is q < the next float above float(1e-6);
   // the posteriors are compared after the exact power-of-two scaling by sq
-  const float qminS = __uint_as_float(0x358637beu) * sc.sq;  // 1.0000001e-6 * sq
+  const float qminS = EM ? emThr : __uint_as_float(0x358637beu) * sc.sq;  // 1.0000001e-6 * sq
 
   u32x4 pre[PER];
   if (gB < gE) {
@@ -908,6 +920,9 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
               aq[c][4 * t + r] = hit ? H(sc.sq) : H(0.f);
             }
       } else {
+        float msum[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) msum[c] = 0.f;
 #pragma unroll
         for (int t = 0; t < 2; ++t)
 #pragma unroll
@@ -917,8 +932,14 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
               const float v = __builtin_amdgcn_exp2f(acc[t][c][r]) * rs4[t][r];
               const bool keep = v >= qminS;
               terms += keep ? 1u : 0u;
+              // padding rows (rs4 = 0) may have 2^acc = inf: keep them out of the mass
+              if (EM) msum[c] += rs4[t][r] > 0.f ? v : 0.f;
               aq[c][4 * t + r] = H(keep ? v : 0.f);
             }
+        if (EM) {
+#pragma unroll
+          for (int c = 0; c < CT; ++c) massd[c] += (double)msum[c];
+        }
       }
     }
     // posterior mass: every column of q x ones is the row sum
@@ -940,6 +961,32 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
   // ---- finalisation in registers
   const double invq = 1.0 / (double)sc.sq;
   const double inv1 = invq / (double)sc.s1, inv2 = invq / (double)sc.s2;
+  if constexpr (EM) {
+    const size_t KD = (size_t)K * D;
+    double* P = emOut + (size_t)img * (2 * (size_t)K + 2 * KD);
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      double m = massd[c];
+      m += __shfl_xor(m, 16, 64);
+      m += __shfl_xor(m, 32, 64);
+      const int kl = kw + 16 * c + lr;
+      if (lg == 0 && kl < K) P[K + kl] = m * invq;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = kw + 16 * c + 4 * lg + r;
+        if (k >= K) continue;
+        if (lr == 0) P[k] = (double)macc0[c][r] * invq;
+#pragma unroll
+        for (int ft = 0; ft < NS; ++ft) {
+          const int d = 16 * ft + lr;
+          if (d >= D) continue;
+          P[2 * (size_t)K + (size_t)k * D + d] = (double)macc[c][0][ft][r] * inv1;
+          P[2 * (size_t)K + KD + (size_t)k * D + d] = (double)macc[c][1][ft][r] * inv2;
+        }
+      }
+    }
+    return;
+  }
   const bool sqrtFlag = (flags & 1) != 0;
   T* encU = enc + (size_t)img * 2 * K * D;
   T* encV = encU + (size_t)K * D;
@@ -1066,7 +1113,7 @@ static void mfma_batch_launch(const MfmaTables<H>& t, const float* xb, int D, in
   const int padImages = n >= 8 ? ((n + 7) / 8) * 8 : n;
   VLHIP_LAUNCH("mfma_fisher", (mfma_fisher_kernel<float, H, NS, CT, WAVES>), dim3((unsigned)((size_t)padImages * numKB)),
                dim3(WAVES * 64), shmem, rec, offsets, gstart, b0, n, D, K, t.Whi, t.Wlo, t.bias, t.scp, t.mup,
-               t.isg, t.sinv, priors, flags, enc, normpart, numTerms, numKB, gflag, list, M);
+               t.isg, t.sinv, priors, flags, enc, normpart, numTerms, numKB, gflag, list, M, 0.f, nullptr);
 }
 
 template <typename H, int NS, int CT, int WAVES>
@@ -1316,11 +1363,194 @@ void fisher_split16(float* enc, const float* means, const float* covs, const flo
   else mfma_dispatch<__bf16>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, gu);
 }
 
+
+// ---------------------------------------------------------------------------
+// EM on the matrix cores (vl_gmm_em with a compensated precision).  One EM
+// step over N descriptors: the data set is cut into chunks of C rows that play
+// the part of images (groups of 32 rows, per-chunk partials), the first pass
+// writes the group records and row statistics, the log-likelihood
+//   LL = sum_i log sum_k pi_k N(x_i; mu_k, Sigma_k)          (gmm.c:786)
+// is summed from the row statistics in FP64, and the second pass (EM mode of
+// mfma_fisher_kernel) writes per-chunk FP64 moments about g that one kernel
+// reduces over the chunks in a fixed order and converts to the moments the
+// M-step finalisation takes (gmm.c:1093-1149):
+//   S1 = sum q x = S1' + g S0,   S2 = sum q (x - mu_old)^2 = S2' - 2 a S1' + a^2 S0,
+// a = mu_old - g, S1' = sum q x', S2' = sum q x'^2.
+
+// ln of the row sums: log2 sum_k 2^l_k = -stat0 + log2(sq) - log2(stat1),
+// summed over the real rows (stat1 = 0 marks padding rows).
+template <int NS>
+__global__ __launch_bounds__(256) void mfma_em_ll_kernel(const unsigned char* __restrict__ rec, long long numGroups,
+                                                         const MfmaScales* __restrict__ scp, double* __restrict__ part) {
+  typedef GroupRec<NS> R;
+  __shared__ double red[256];
+  const double lsq = log2((double)scp->sq);
+  double s = 0.0;
+  const long long rows = numGroups * GR;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < rows; i += (long long)gridDim.x * 256) {
+    const long long grp = i / GR;
+    const int r = (int)(i - grp * GR);
+    const float* Sf = reinterpret_cast<const float*>(rec + (size_t)grp * R::kBytes + R::kA + R::kB);
+    const float rs = Sf[GR + r];
+    if (rs > 0.f) s += lsq - (double)Sf[r] - log2((double)rs);
+  }
+  const double b = block_sum_256(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = b;
+}
+
+// Fixed-order FP64 sum of the per-chunk moments and the change of origin;
+// one thread per (component, dimension).  out = [S0 K | mass K | S1 KD | S2 KD].
+__global__ __launch_bounds__(256) void mfma_em_reduce_kernel(const double* __restrict__ part, int chunks, int D, int K,
+                                                             const float* __restrict__ oldMeans,
+                                                             const float* __restrict__ g, double* __restrict__ out) {
+  const size_t KD = (size_t)K * D;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= KD) return;
+  const size_t k = i / D, d = i - k * D;
+  const size_t PS = 2 * (size_t)K + 2 * KD;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, ms = 0.0;
+  for (int c = 0; c < chunks; ++c) {
+    const double* P = part + (size_t)c * PS;
+    s0 += P[k];
+    s1 += P[2 * K + i];
+    s2 += P[2 * K + KD + i];
+    if (d == 0) ms += P[K + k];
+  }
+  const double gd = (double)g[d];
+  const double a = (double)oldMeans[i] - gd;
+  out[2 * K + i] = s1 + gd * s0;
+  out[2 * K + KD + i] = s2 - 2.0 * a * s1 + a * a * s0;
+  if (d == 0) {
+    out[k] = s0;
+    out[K + k] = ms;
+  }
+}
+
+// Rows per chunk: 4096 from N = 2^18 rows on (short enough that the FP32
+// matrix accumulators stay accurate), fewer below so that there are at least
+// 64 chunks, more only when N needs more than 2048 chunks.
+static size_t em_chunk_rows(size_t N) {
+  size_t c = (N + 63) / 64;
+  c = std::max<size_t>(256, std::min<size_t>(4096, c));
+  c = std::max(c, (N + 2047) / 2048);
+  return (c + GR - 1) / GR * GR;
+}
+
+template <typename H, int NS, int CT, int WAVES>
+static double run_em_step(double* out_host, const float* means, const float* covs, const float* priors, size_t D,
+                          size_t K, const float* x, size_t N) {
+  constexpr int Dp = NS * 16;
+  constexpr int BC = WAVES * 16 * CT;
+  typedef GroupRec<NS> R;
+  const int Kp = (int)((K + BC - 1) / BC) * BC;
+  unsigned char* base = (unsigned char*)scratch(SLOT_PREP, mfma_tables_bytes<H>(Dp, Kp));
+  const MfmaTables<H> t = mfma_tables_carve<H>(base, Dp, Kp);
+  mfma_tables_launch<H>(t, means, covs, priors, (int)D, Dp, (int)K, Kp);
+  const int numKB = Kp / BC;
+
+  const size_t C = em_chunk_rows(N);
+  const size_t numChunks = (N + C - 1) / C;
+  std::vector<uint64_t> off(numChunks + 1);
+  std::vector<long long> gs(numChunks + 1);
+  gs[0] = 0;
+  for (size_t i = 0; i <= numChunks; ++i) off[i] = std::min<uint64_t>(N, (uint64_t)(i * C));
+  for (size_t i = 0; i < numChunks; ++i) gs[i + 1] = gs[i] + (long long)((off[i + 1] - off[i] + GR - 1) / GR);
+  uint64_t* offsets = (uint64_t*)scratch(SLOT_MISC, (numChunks + 1) * sizeof(uint64_t));
+  long long* gstart = (long long*)scratch(SLOT_GROUPS, (numChunks + 1) * sizeof(long long));
+  upload(offsets, off.data(), (numChunks + 1) * sizeof(uint64_t));
+  upload(gstart, gs.data(), (numChunks + 1) * sizeof(long long));
+
+  const size_t KD = K * D;
+  const size_t PS = 2 * K + 2 * KD;
+  double* part = out_host ? (double*)scratch(SLOT_PARTIAL, numChunks * PS * sizeof(double)) : nullptr;
+  float qthr = (float)(0.01 / (double)K);  // smallest float q with !((double)q < 0.01 / K)
+  if ((double)qthr < 0.01 / (double)K) qthr = std::nextafter(qthr, INFINITY);
+  const size_t shmem = mfma_shmem<H, NS, CT, WAVES>(false);
+  auto kern = mfma_fisher_kernel<float, H, NS, CT, WAVES, true>;
+  VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+  // the scaled threshold needs the device scale sq: read it once
+  MfmaScales hs;
+  download(&hs, t.scp, sizeof(MfmaScales));
+  const float emThr = qthr * hs.sq;
+
+  constexpr unsigned kLLBlocks = 512;
+  std::vector<double> llh;
+  size_t b0 = 0;
+  while (b0 < numChunks) {
+    size_t b1 = b0 + 1;
+    while (b1 < numChunks && (size_t)(gs[b1 + 1] - gs[b0]) * R::kBytes <= kRecordBudget) ++b1;
+    const uint64_t rowBase = off[b0];
+    const size_t nG = (size_t)(gs[b1] - gs[b0]);
+    const int n = (int)(b1 - b0);
+    unsigned char* rec = (unsigned char*)scratch(SLOT_FISHER_WORK, nG * R::kBytes);
+    int4* gtab = (int4*)scratch(SLOT_GROUPTAB, nG * sizeof(int4));
+    double* llp = (double*)scratch(SLOT_MISC2, kLLBlocks * sizeof(double));
+    VLHIP_LAUNCH("mfma_group_table", mfma_group_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, offsets,
+                 gstart, (int)b0, n, gtab);
+    constexpr int LG = LseGroups<NS>::value;
+    VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, false>), dim3((unsigned)((nG + LG - 1) / LG)), dim3(LG * 64), 0,
+                 x + rowBase * D, (int)D, Kp, t.Whi, t.Wlo, t.bias, t.scp, t.g, rec, (long long)nG, gtab,
+                 (const unsigned char*)nullptr);
+    VLHIP_LAUNCH("mfma_em_ll", mfma_em_ll_kernel<NS>, dim3(kLLBlocks), dim3(256), 0, rec, (long long)nG, t.scp, llp);
+    if (out_host) {
+      const int padImages = n >= 8 ? ((n + 7) / 8) * 8 : n;
+      VLHIP_LAUNCH("mfma_em_moments", (mfma_fisher_kernel<float, H, NS, CT, WAVES, true>), dim3((unsigned)((size_t)padImages * numKB)), dim3(WAVES * 64), shmem, rec,
+                   offsets, gstart, (int)b0, n, (int)D, (int)K, t.Whi, t.Wlo, t.bias, t.scp, t.mup, t.isg, t.sinv,
+                   priors, 0, (float*)nullptr, (double*)nullptr, (unsigned long long*)nullptr, numKB,
+                   (const unsigned char*)nullptr, (const int2*)nullptr, 0, emThr, part);
+    }
+    const size_t o = llh.size();
+    llh.resize(o + kLLBlocks);
+    download(llh.data() + o, llp, kLLBlocks * sizeof(double));  // synchronous: llp is reused by the next batch
+    b0 = b1;
+  }
+  double ll2 = 0.0;
+  for (double v : llh) ll2 += v;
+  if (out_host) {
+    double* out = (double*)scratch(SLOT_REDUCED, PS * sizeof(double));
+    VLHIP_LAUNCH("mfma_em_reduce", mfma_em_reduce_kernel, dim3((unsigned)((KD + 255) / 256)), dim3(256), 0, part,
+                 (int)numChunks, (int)D, (int)K, means, t.g, out);
+    download(out_host, out, PS * sizeof(double));
+  }
+  return ll2 * 0.6931471805599453;  // ln 2
+}
+
+template <typename H>
+static double em_dispatch(double* out, const float* means, const float* covs, const float* priors, size_t D, size_t K,
+                          const float* x, size_t N) {
+  switch ((D + 15) / 16) {
+#define EM_CASE(NS)                                                                      \
+  case NS:                                                                               \
+    if constexpr (NS <= 4) {                                                             \
+      if (K > 128) return run_em_step<H, NS, 2, 8>(out, means, covs, priors, D, K, x, N); \
+    }                                                                                    \
+    return run_em_step<H, NS, 2, 4>(out, means, covs, priors, D, K, x, N);
+    EM_CASE(1) EM_CASE(2) EM_CASE(3) EM_CASE(4) EM_CASE(5) EM_CASE(6) EM_CASE(7) EM_CASE(8)
+#undef EM_CASE
+    default: fatal("vlhip_gmm_em_step", __FILE__, __LINE__, "the matrix-core EM step supports dimension <= 128");
+  }
+}
+
 }  // namespace vlhip
 
 using namespace vlhip;
 
 extern "C" {
+
+double vlhip_gmm_em_step(int mode, double* out_host, void const* priors, void const* means, void const* covs,
+                         size_t D, size_t K, void const* data, size_t N) {
+  ensure_init();
+  std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+  VLHIP_REQUIRE(mode == VLHIP_MODE_BF16X3 || mode == VLHIP_MODE_F16X3,
+                "the matrix-core EM step takes VLHIP_MODE_BF16X3 or VLHIP_MODE_F16X3");
+  VLHIP_REQUIRE(N >= 1 && K >= 1 && D >= 1 && D <= 128,
+                "the matrix-core EM step needs numData >= 1, numClusters >= 1 and 1 <= dimension <= 128");
+  if (mode == VLHIP_MODE_F16X3)
+    return em_dispatch<_Float16>(out_host, (const float*)means, (const float*)covs, (const float*)priors, D, K,
+                                 (const float*)data, N);
+  return em_dispatch<__bf16>(out_host, (const float*)means, (const float*)covs, (const float*)priors, D, K,
+                             (const float*)data, N);
+}
 
 void* vlhip_encode_plan_new(size_t D, size_t K, uint64_t const* offsets_host, size_t numImages, int fisherFlags,
                             int vladFlags) {

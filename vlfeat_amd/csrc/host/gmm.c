@@ -51,6 +51,8 @@ struct _VlGMM
   void * dParams ; vl_size dParamsBytes ;
   vl_size postRows ;           /* rows held by dPost (0 = none) */
   vl_bool hostPostValid ;      /* host posteriors mirror dPost */
+  vl_bool devPostStale ;       /* dPost predates the last E-step (matrix-core EM) */
+  int precision ;              /* VL_GMM_PRECISION_* of the EM iterations */
 } ;
 
 /* ---------------------------------------------------------------- */
@@ -177,6 +179,7 @@ vl_gmm_new_copy (VlGMM const * self)
   gmm->maxNumIterations = self->maxNumIterations ;
   gmm->numRepetitions = self->numRepetitions ;
   gmm->verbosity = self->verbosity ;
+  gmm->precision = self->precision ;
   gmm->LL = self->LL ;
   memcpy(gmm->means, self->means, size * self->numClusters * self->dimension) ;
   memcpy(gmm->covariances, self->covariances, size * self->numClusters * self->dimension) ;
@@ -204,6 +207,22 @@ VL_EXPORT VlGMMInitialization vl_gmm_get_initialization (VlGMM const * self) { r
 VL_EXPORT void vl_gmm_set_initialization (VlGMM * self, VlGMMInitialization init) { self->initialization = init ; }
 VL_EXPORT VlKMeans * vl_gmm_get_kmeans_init_object (VlGMM const * self) { return self->kmeansInit ; }
 VL_EXPORT double const * vl_gmm_get_covariance_lower_bounds (VlGMM const * self) { return self->sigmaLowBound ; }
+
+/* No counterpart in the reference: arithmetic of the EM iterations of float
+ * mixtures (dimension <= 128 for the matrix-core modes; other cases, and
+ * double data, always use the FP32 / FP64 parity path). */
+VL_EXPORT void
+vl_gmm_set_precision (VlGMM * self, int precision)
+{
+  if (precision != VL_GMM_PRECISION_FP32 && precision != VL_GMM_PRECISION_BF16X3 &&
+      precision != VL_GMM_PRECISION_F16X3) {
+    VL_PRINTF("vl_gmm_set_precision: unknown precision %d\n", precision) ;
+    abort() ;
+  }
+  self->precision = precision ;
+}
+
+VL_EXPORT int vl_gmm_get_precision (VlGMM const * self) { return self->precision ; }
 
 VL_EXPORT void
 vl_gmm_set_num_repetitions (VlGMM * self, vl_size numRepetitions)
@@ -259,6 +278,20 @@ vl_gmm_get_posteriors (VlGMM const * cself)
   VlGMM * self = (VlGMM*) cself ;
   vl_size es = vl_get_type_size(self->dataType) ;
   if (self->postRows == 0) return self->posteriors ;
+  if (self->devPostStale) {
+    /* the matrix-core EM keeps no posterior matrix: the posteriors of the
+       last E-step are those of the final parameters, recomputed once here
+       with the FP32 E-step */
+    unsigned char * p = (unsigned char*) self->dParams ;
+    vl_size KD = self->numClusters * self->dimension ;
+    vlhip_lock() ;
+    gmm_upload_params(self) ;
+    vlhip_gmm_posteriors((int)self->dataType, self->dPost, self->numClusters, self->postRows,
+                         p + 2 * es * KD, p, self->dimension, p + es * KD, self->dData) ;
+    vlhip_unlock() ;
+    self->devPostStale = VL_FALSE ;
+    self->hostPostValid = VL_FALSE ;
+  }
   if (!self->hostPostValid) {
     vl_size bytes = es * self->numClusters * self->postRows ;
     if (!self->posteriors) self->posteriors = vl_malloc(bytes) ;

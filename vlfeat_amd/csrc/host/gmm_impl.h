@@ -143,7 +143,26 @@ FN(gmm_restart) (VlGMM * self, vl_size numData, double const * massIn)
   return splits ;
 }
 
-/* EM iterations, gmm.c:1220-1308. */
+/* Whether some component would be restarted (the test of gmm.c:897-899). */
+static vl_bool
+FN(gmm_needs_restart) (VlGMM * self, vl_size numData, double const * mass)
+{
+  vl_uindex k ;
+  if (self->numClusters <= 1) return VL_FALSE ;
+  for (k = 0 ; k < self->numClusters ; ++k) {
+    /* the complement of the skip test of gmm_restart, so that no component
+       it would restart is missed */
+    if (!(mass[k] >= GMM_MIN_POSTERIOR * VL_MAX(1.0, (double)numData / self->numClusters))) return VL_TRUE ;
+  }
+  return VL_FALSE ;
+}
+
+/* EM iterations, gmm.c:1220-1308.  With a matrix-core precision (float data,
+ * dimension <= 128) each iteration is one vlhip_gmm_em_step: log-likelihood
+ * and M-step moments without a posterior matrix.  An iteration that may
+ * restart an empty component recomputes its posteriors and moments with the
+ * FP32 parity path first, so that the split of gmm.c:968-990 works on the
+ * posterior matrix exactly as in the FP32 mode. */
 static double
 FN(gmm_em) (VlGMM * self, TYPE const * data, vl_size numData)
 {
@@ -154,11 +173,15 @@ FN(gmm_em) (VlGMM * self, TYPE const * data, vl_size numData)
   double LL = -VL_INFINITY_D ;
   double * out = (double*) vl_malloc(sizeof(double) * (2 * K + 2 * KD)) ;
   unsigned char * p ;
+  int mfma = (TYPE_CODE == VL_TYPE_FLOAT && self->precision != VL_GMM_PRECISION_FP32 &&
+              D <= 128 && numData > 0) ;
+  int mode = (self->precision == VL_GMM_PRECISION_F16X3) ? VLHIP_MODE_F16X3 : VLHIP_MODE_BF16X3 ;
 
   vlhip_lock() ;
   _vl_gmm_prepare_for_data(self, numData) ;
   gmm_stage_data(self, data, numData) ;
   self->postRows = numData ;
+  self->devPostStale = VL_FALSE ;
   p = (unsigned char*) self->dParams ;
   FN(gmm_apply_bounds)(self) ;
 
@@ -166,8 +189,15 @@ FN(gmm_em) (VlGMM * self, TYPE const * data, vl_size numData)
     double eps ;
     double time = vl_get_cpu_time() ;
     gmm_upload_params(self) ;
-    LL = vlhip_gmm_posteriors(TYPE_CODE, self->dPost, K, numData,
-                              p + 2 * sizeof(TYPE) * KD, p, D, p + sizeof(TYPE) * KD, self->dData) ;
+    if (mfma) {
+      /* the last iteration (iteration == maxNumIterations) needs no moments */
+      LL = vlhip_gmm_em_step(mode, (iteration < self->maxNumIterations) ? out : NULL, p + 2 * sizeof(TYPE) * KD, p,
+                             p + sizeof(TYPE) * KD, D, K, self->dData, numData) ;
+      self->devPostStale = VL_TRUE ;
+    } else {
+      LL = vlhip_gmm_posteriors(TYPE_CODE, self->dPost, K, numData,
+                                p + 2 * sizeof(TYPE) * KD, p, D, p + sizeof(TYPE) * KD, self->dData) ;
+    }
     if (self->verbosity > 1) {
       VL_PRINTF("gmm: em: expectation step completed in %.2f s\n", vl_get_cpu_time() - time) ;
     }
@@ -191,8 +221,15 @@ FN(gmm_em) (VlGMM * self, TYPE const * data, vl_size numData)
     }
     previousLL = LL ;
 
-    FN(gmm_moments)(self, out, numData) ;
+    if (!mfma) FN(gmm_moments)(self, out, numData) ;
     if (iteration > 1) {
+      if (mfma && FN(gmm_needs_restart)(self, numData, out + K)) {
+        /* this iteration's posteriors and moments on the parity path */
+        vlhip_gmm_posteriors(TYPE_CODE, self->dPost, K, numData,
+                             p + 2 * sizeof(TYPE) * KD, p, D, p + sizeof(TYPE) * KD, self->dData) ;
+        self->devPostStale = VL_FALSE ;
+        FN(gmm_moments)(self, out, numData) ;
+      }
       if (FN(gmm_restart)(self, numData, out + K) > 0) {
         FN(gmm_moments)(self, out, numData) ;
       }
