@@ -57,7 +57,7 @@ constexpr double kLog2eD = 1.4426950408889634;  // log2(e)
 struct MfmaScales {
   float s1, s2;     // x' and x'^2 multipliers
   float sq;         // posterior multiplier
-  float pad;
+  unsigned bound;   // bits of the (non-negative) range bound on |x'|, max-reduced atomically
 };
 
 // ---------------------------------------------------------------------------
@@ -66,70 +66,82 @@ struct MfmaScales {
 //   bias    [Kp]        b_k (FP32), -inf for skipped / padding components
 //   mup, isg, sinv [Kp][Dp]  mu', sigma^-1, sigma^-2 (FP32) for finalisation
 
-// g = sum_k pi_k mu_k / sum_k pi_k, and the range bound used to pick scales.
-template <typename T, typename H>
-__global__ void mfma_center_kernel(const T* __restrict__ means, const T* __restrict__ covs,
-                                   const T* __restrict__ priors, int D, int K, float* __restrict__ g,
-                                   MfmaScales* __restrict__ sc, int isHalf) {
+// g_d = sum_k pi_k mu_kd / sum_k pi_k (FP64 sums), one workgroup per dimension
+// (zero for the padding dimensions D <= d < Dp).
+template <typename T>
+__global__ __launch_bounds__(256) void mfma_center_kernel(const T* __restrict__ means, const T* __restrict__ priors,
+                                                          int D, int K, float* __restrict__ g) {
   __shared__ double red[256];
-  __shared__ float gsh[512];
-  const int t = threadIdx.x;
-  for (int d = 0; d < D; ++d) {
-    double a = 0.0, w = 0.0;
+  const int d = blockIdx.x, t = threadIdx.x;
+  double a = 0.0, w = 0.0;
+  if (d < D) {
     for (int k = t; k < K; k += 256) {
       a += (double)priors[k] * (double)means[(size_t)k * D + d];
       w += (double)priors[k];
     }
-    const double sa = block_sum_256(a, red);
-    const double sw = block_sum_256(w, red);
-    if (t == 0) {
-      const float gd = (float)(sw > 0 ? sa / sw : 0.0);
-      g[d] = gd;
-      if (d < 512) gsh[d] = gd;
-    }
   }
-  __syncthreads();
-  // bound on |x'|: max over components of |mu'| + 6 sigma
-  double mx = 0.0;
-  for (int i = t; i < K * D; i += 256) {
-    const int d = i % D;
-    const double gd = (d < 512) ? (double)gsh[d] : (double)g[d];
-    const double v = fabs((double)means[i] - gd) + 6.0 * sqrt((double)covs[i]);
-    mx = v > mx ? v : mx;
+  const double sa = block_sum_256(a, red);
+  const double sw = block_sum_256(w, red);
+  if (t == 0) g[d] = (d < D && sw > 0) ? (float)(sa / sw) : 0.f;
+}
+
+// Range bound on |x'|: max over components and dimensions of |mu' | + 6 sigma,
+// max-reduced into sc->bound (zeroed beforehand; a non-negative float's bits
+// order as unsigned integers).
+template <typename T>
+__global__ __launch_bounds__(256) void mfma_bound_kernel(const T* __restrict__ means, const T* __restrict__ covs,
+                                                         int D, int K, const float* __restrict__ g,
+                                                         MfmaScales* __restrict__ sc) {
+  __shared__ float red[256];
+  const int t = threadIdx.x;
+  const size_t i = (size_t)blockIdx.x * 256 + t;
+  float v = 0.f;
+  if (i < (size_t)K * D) {
+    const int d = (int)(i % D);
+    const double x = fabs((double)means[i] - (double)g[d]) + 6.0 * sqrt((double)covs[i]);
+    v = x > 0.0 ? (float)x : 0.f;  // NaN (a negative variance) does not raise the bound
   }
-  red[t] = mx;
+  red[t] = v;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (t < s) red[t] = red[t] > red[t + s] ? red[t] : red[t + s];
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) red[t] = fmaxf(red[t], red[t + o]);
     __syncthreads();
   }
-  if (t == 0) {
-    const double xm = red[0] > 1e-30 ? red[0] : 1.0;
-    MfmaScales s;
-    if (isHalf) {
-      // put |x'| and |x'|^2 near 2^8 and the posterior (<= 1) at 2^14
-      s.s1 = (float)exp2(floor(log2(256.0 / xm)));
-      s.s2 = (float)exp2(floor(log2(256.0 / (xm * xm))));
-      s.sq = 16384.0f;
-    } else {
-      s.s1 = 1.0f;
-      s.s2 = 1.0f;
-      s.sq = 1.0f;
-    }
-    s.pad = 0.f;
-    *sc = s;
+  if (t == 0 && red[0] > 0.f) atomicMax(&sc->bound, __float_as_uint(red[0]));
+}
+
+// Scales from the bound: FP16 puts |x'| and |x'|^2 near 2^8 and the posterior
+// (<= 1) at 2^14; BF16 needs none.
+__device__ __forceinline__ MfmaScales mfma_scales(unsigned boundBits, bool isHalf) {
+  MfmaScales s;
+  const double b = (double)__uint_as_float(boundBits);
+  const double xm = b > 1e-30 ? b : 1.0;
+  if (isHalf) {
+    s.s1 = (float)exp2(floor(log2(256.0 / xm)));
+    s.s2 = (float)exp2(floor(log2(256.0 / (xm * xm))));
+    s.sq = 16384.0f;
+  } else {
+    s.s1 = 1.0f;
+    s.s2 = 1.0f;
+    s.sq = 1.0f;
   }
+  s.bound = boundBits;
+  return s;
 }
 
 template <typename T, typename H>
 __global__ void mfma_tables_kernel(const T* __restrict__ means, const T* __restrict__ covs,
                                    const T* __restrict__ priors, int D, int Dp, int K, int Kp,
-                                   const float* __restrict__ g, const MfmaScales* __restrict__ sc,
+                                   const float* __restrict__ g, MfmaScales* __restrict__ scp,
                                    H* __restrict__ Whi, H* __restrict__ Wlo, float* __restrict__ bias,
                                    float* __restrict__ mup, float* __restrict__ isg, float* __restrict__ sinv) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const MfmaScales sc = mfma_scales(scp->bound, std::is_same<H, _Float16>::value);
+  // every thread has read the bound before the first workgroup's thread 0 stores the scales
+  __syncthreads();
+  if (k == 0) *scp = sc;
   if (k >= Kp) return;
-  const float e1 = 1.0f / sc->s1, e2 = 1.0f / sc->s2;
+  const float e1 = 1.0f / sc.s1, e2 = 1.0f / sc.s2;
   H* wh = Whi + (size_t)k * 2 * Dp;
   H* wl = Wlo + (size_t)k * 2 * Dp;
   if (k >= K) {
@@ -1083,10 +1095,10 @@ static void zero_words(void* p, size_t words) {
 template <typename H>
 static void mfma_tables_launch(const MfmaTables<H>& t, const float* means, const float* covs, const float* priors,
                                int D, int Dp, int K, int Kp) {
-  const bool isHalf = std::is_same<H, _Float16>::value;
-  zero_words(t.g, (size_t)Dp);
-  VLHIP_LAUNCH("mfma_center", (mfma_center_kernel<float, H>), dim3(1), dim3(256), 0, means, covs, priors, D, K, t.g,
-               t.scp, isHalf ? 1 : 0);
+  zero_words(&t.scp->bound, 1);
+  VLHIP_LAUNCH("mfma_center", (mfma_center_kernel<float>), dim3(Dp), dim3(256), 0, means, priors, D, K, t.g);
+  VLHIP_LAUNCH("mfma_bound", (mfma_bound_kernel<float>), dim3((unsigned)(((size_t)K * D + 255) / 256)), dim3(256), 0,
+               means, covs, D, K, t.g, t.scp);
   VLHIP_LAUNCH("mfma_tables", (mfma_tables_kernel<float, H>), dim3((Kp + 127) / 128), dim3(128), 0, means, covs,
                priors, D, Dp, K, Kp, t.g, t.scp, t.Whi, t.Wlo, t.bias, t.mup, t.isg, t.sinv);
 }
