@@ -99,6 +99,19 @@ def cpu_legs(x_host, offsets, m, c, p, budget_s: float, gpu_enc, gpu_fp32_enc, n
             "value": done_desc / dt, "unit": "descriptors/s", "cores": threads, "kind": "reference",
             "sample": f"{done_img} images x {PER_IMAGE} descriptors (K={K_COMP}, D={DIM}, improved FV), "
                       f"vl_fisher_encode per image, OpenMP inside the image, {dt:.1f} s"}
+        # image-parallel upper bound of the CPU path (as an application's parfor
+        # over images): one host thread per image, no OpenMP inside a call
+        import concurrent.futures as cf
+        R.set_num_threads(1)
+        t0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(threads) as ex:
+            list(ex.map(lambda i: R.fisher(x_host[offsets[i]:offsets[i + 1]], m, c, p, FLAGS), range(done_img)))
+        dt2 = time.perf_counter() - t0
+        R.set_num_threads(0)
+        out["cpu_image_parallel"] = {
+            "value": done_desc / dt2, "unit": "descriptors/s", "cores": threads, "kind": "reference",
+            "sample": f"the same {done_img} images, {threads} host threads each encoding whole images "
+                      f"(vl_set_num_threads(1)), {dt2:.1f} s"}
         nc = min(len(encs), gpu_fp32_enc.shape[0])
         r = np.stack(encs[:nc]).astype(np.float64)
         g = gpu_fp32_enc[:nc].astype(np.float64)
@@ -440,6 +453,106 @@ def em_config(args) -> int:
     return 0
 
 
+def dropin_config(args) -> int:
+    """Plain FP32 drop-in at SIFT scale, through VLFeat's host C API exactly as a
+    caller of the reference uses it: vl_gmm_cluster on N = 1,000,000 SIFT-like
+    descriptors projected to 80-d (seed 1), K = 256, 30 EM iterations from a
+    seeded custom start (random data points, data variance, uniform priors);
+    then vl_fisher_encode (improved) and vl_vlad_encode (one-hot assignments from
+    vl_kmeans_quantize against the GMM means) of 1,000 images x 3,000 descriptors
+    (seed 2).  The same calls run on the reference (oracle/_ref, OpenMP over the
+    host cores) on the same inputs: wall-clock of both, and the parity of every
+    output at full size (means / covariances / priors / LL after 30 iterations;
+    every Fisher vector, VLAD vector and hard assignment computed by both from
+    the reference's GMM)."""
+    import torch
+    import vlfeat_amd as va
+    from oracle import reference
+    from vlfeat_amd import synth
+
+    N, D, K, n_img, per = args.dropin_n, DIM, 256, args.dropin_images, PER_IMAGE
+    L = va.lib()
+    L.vlhip_init()
+    log(f"dropin: generating {N} + {n_img * per} descriptors")
+    x = synth.sift_like_torch(N, D, seed=1).cpu().numpy()
+    xe = synth.sift_like_torch(n_img * per, D, seed=2).cpu().numpy()
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(7)
+    m0 = x[rng.choice(N, K, replace=False)].copy()
+    c0 = np.tile(np.var(x, 0), (K, 1)).astype(np.float32)
+    p0 = np.full(K, 1.0 / K, np.float32)
+
+    def run(lib, model=None):
+        """EM, then the encoders with `model` (the reference's GMM, so that the
+        encoders' parity does not include the EM's) or this run's own GMM."""
+        out = {}
+        t0 = time.perf_counter()
+        m, c, p, ll, _ = lib.gmm(x, K, init="custom", means=m0, covs=c0, priors=p0, max_iter=30, want_post=False)
+        out["em_s"] = time.perf_counter() - t0
+        em = (m, c, p, ll)
+        if model is not None:
+            m, c, p = model
+        t0 = time.perf_counter()
+        fv = np.stack([lib.fisher(xe[i * per:(i + 1) * per], m, c, p, FLAGS)[0] for i in range(n_img)])
+        out["fisher_s"] = time.perf_counter() - t0
+        km = lib.kmeans_new(np.float32)
+        lib.vl_kmeans_set_centers(km, m.ctypes.data, D, K)
+        t0 = time.perf_counter()
+        a = lib.kmeans_quantize(km, xe)[0]
+        out["quantize_s"] = time.perf_counter() - t0
+        lib.vl_kmeans_delete(km)
+        t0 = time.perf_counter()
+        vl = []
+        for i in range(n_img):
+            oh = np.zeros((per, K), np.float32)
+            oh[np.arange(per), a[i * per:(i + 1) * per]] = 1
+            vl.append(lib.vlad(xe[i * per:(i + 1) * per], m, oh, 0))
+        out["vlad_s"] = time.perf_counter() - t0
+        return out, em + (fv, a, np.stack(vl))
+
+    R = reference() if not args.no_cpu else None
+    r_t = r = None
+    if R is not None:
+        R.set_num_threads(0)
+        r_t, r = run(R)
+        log(f"dropin: reference ({int(R.vl_get_max_threads())} threads): EM {r_t['em_s']:.2f} s, "
+            f"Fisher {r_t['fisher_s']:.2f} s, quantize {r_t['quantize_s']:.2f} s, VLAD {r_t['vlad_s']:.2f} s")
+    g_t, g = run(L, model=r[:3] if r is not None else None)
+    log(f"dropin: GPU library: EM {g_t['em_s']:.2f} s, Fisher {g_t['fisher_s']:.2f} s, "
+        f"quantize {g_t['quantize_s']:.2f} s, VLAD {g_t['vlad_s']:.2f} s")
+    result = {
+        "metric": "FP32 drop-in at SIFT scale (config 2): host C API, GPU library vs reference",
+        "value": n_img * per / (g_t["fisher_s"] + g_t["vlad_s"] + g_t["quantize_s"]),
+        "unit": "descriptors/s (Fisher + quantize + VLAD through the host API, PCIe included)",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": 1000 * (g_t["fisher_s"] + g_t["vlad_s"] + g_t["quantize_s"]),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (SIFT-like projected to 80-d; EM data seed 1, images seed 2)",
+        "config": {"workload": "vl_gmm_cluster N=1M K=256 D=80 30 iterations; improved FV + VLAD of "
+                               f"{n_img} images x {per}", "N": N, "K": K, "D": D, "images": n_img},
+        "gpu_library_s": g_t,
+    }
+    if R is not None:
+        rel = lambda a, b: float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(np.asarray(b, np.float64)))
+        fv_err = np.linalg.norm(g[4].astype(np.float64) - r[4], axis=1) / np.linalg.norm(r[4].astype(np.float64), axis=1)
+        vl_err = np.linalg.norm(g[6].astype(np.float64) - r[6], axis=1) / np.maximum(
+            np.linalg.norm(r[6].astype(np.float64), axis=1), 1e-30)
+        parity = {"means_rel_l2": rel(g[0], r[0]), "covariances_rel_l2": rel(g[1], r[1]),
+                  "priors_rel_l2": rel(g[2], r[2]), "loglikelihood_rel": abs(g[3] - r[3]) / abs(r[3]),
+                  "fisher_max_rel_l2": float(fv_err.max()), "fisher_median_rel_l2": float(np.median(fv_err)),
+                  "assignments_equal": int((g[5] == r[5]).sum()), "assignments_total": int(g[5].size),
+                  "vlad_max_rel_l2": float(vl_err.max())}
+        log("dropin: parity " + ", ".join(f"{k}={v:.3g}" for k, v in parity.items()))
+        result["reference_s"] = r_t
+        result["parity_vs_reference"] = parity
+        result["speedup_vs_reference"] = {k: r_t[k] / g_t[k] for k in g_t}
+        result["cpu_baseline"] = {
+            "value": n_img * per / (r_t["fisher_s"] + r_t["vlad_s"] + r_t["quantize_s"]), "unit": result["unit"],
+            "cores": int(R.vl_get_max_threads()), "kind": "reference",
+            "sample": f"the same {n_img} images through the reference's host API (OpenMP inside each call)"}
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -454,10 +567,12 @@ def main() -> int:
     ap.add_argument("--no-gated", action="store_true", help="skip the learned-gate leg")
     ap.add_argument("--gate-train", type=int, default=2_000_000, help="descriptors the gate is trained on")
     ap.add_argument("--gate-need", type=float, default=0.999)
-    ap.add_argument("--config", default="headline", choices=["headline", "batched", "em"],
+    ap.add_argument("--config", default="headline", choices=["headline", "batched", "em", "dropin"],
                     help="batched: the uneven-image Fisher+VLAD plan benchmark (K=512); em: the EM precision "
                          "form (N=4M, D=128, K=1024) instead of the headline")
     ap.add_argument("--em-n", type=int, default=4_000_000)
+    ap.add_argument("--dropin-n", type=int, default=1_000_000)
+    ap.add_argument("--dropin-images", type=int, default=1000)
     ap.add_argument("--em-oracle-rows", type=int, default=65536)
     ap.add_argument("--em-cpu-rows", type=int, default=131072)
     ap.add_argument("--copy-probe", action="store_true",
@@ -471,6 +586,8 @@ def main() -> int:
         return batched_config(args)
     if args.config == "em":
         return em_config(args)
+    if args.config == "dropin":
+        return dropin_config(args)
 
     import torch
     import vlfeat_amd as va

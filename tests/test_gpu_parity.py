@@ -171,6 +171,27 @@ def test_fisher_batch_uneven_equals_per_image(cfg2):
     assert np.all(enc[0] == 0) and np.all(enc[8] == 0)
 
 
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_fisher_segments_in_parallel_equal_sequential(cfg2, dt):
+    # a batch large enough to fill the device sums each image's segments in
+    # one workgroup, one image alone sums them in parallel workgroups: the
+    # encodings must agree bit for bit (and stay within the oracle's bar)
+    x, m, c, p = (a.astype(dt) for a in cfg2)
+    K = m.shape[0]
+    rng = np.random.default_rng(3)
+    counts = rng.integers(1, 1300, size=300)
+    counts[[0, 7]] = [1299, 257]
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint64)
+    xs = np.concatenate([x] * int(np.ceil(off[-1] / x.shape[0])))[: int(off[-1])]
+    enc, terms = va.vl_fisher_batch(xs, off, m, c, p, 3)
+    for i in (0, 7, 42):
+        e1, n1 = va.vl_fisher(xs[off[i]:off[i + 1]], m, c, p, 3)
+        assert np.array_equal(enc[i], e1), i
+        assert terms[i] == n1
+    eo, _ = oracle().fisher(xs[off[0]:off[1]], m, c, p, 3)
+    assert rel(enc[0], eo) < (1e-4 if dt == np.float32 else 1e-10)
+
+
 def test_fisher_skips_tiny_prior_component(cfg1):
     x, m, c, p = cfg1
     p = p.copy()

@@ -1467,7 +1467,7 @@ static double run_em_step(double* out_host, const float* means, const float* cov
   gs[0] = 0;
   for (size_t i = 0; i <= numChunks; ++i) off[i] = std::min<uint64_t>(N, (uint64_t)(i * C));
   for (size_t i = 0; i < numChunks; ++i) gs[i + 1] = gs[i] + (long long)((off[i + 1] - off[i] + GR - 1) / GR);
-  uint64_t* offsets = (uint64_t*)scratch(SLOT_MISC, (numChunks + 1) * sizeof(uint64_t));
+  uint64_t* offsets = (uint64_t*)scratch(SLOT_EMCHUNKS, (numChunks + 1) * sizeof(uint64_t));
   long long* gstart = (long long*)scratch(SLOT_GROUPS, (numChunks + 1) * sizeof(long long));
   upload(offsets, off.data(), (numChunks + 1) * sizeof(uint64_t));
   upload(gstart, gs.data(), (numChunks + 1) * sizeof(long long));
@@ -1496,7 +1496,7 @@ static double run_em_step(double* out_host, const float* means, const float* cov
     const int n = (int)(b1 - b0);
     unsigned char* rec = (unsigned char*)scratch(SLOT_FISHER_WORK, nG * R::kBytes);
     int4* gtab = (int4*)scratch(SLOT_GROUPTAB, nG * sizeof(int4));
-    double* llp = (double*)scratch(SLOT_MISC2, kLLBlocks * sizeof(double));
+    double* llp = (double*)scratch(SLOT_EMLL, kLLBlocks * sizeof(double));
     VLHIP_LAUNCH("mfma_group_table", mfma_group_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, offsets,
                  gstart, (int)b0, n, gtab);
     constexpr int LG = LseGroups<NS>::value;

@@ -31,17 +31,24 @@ namespace vlhip {
   do { if (!(cond)) ::vlhip::fatal(#cond, __FILE__, __LINE__, (msg)); } while (0)
 
 // Scratch slots: each is a grow-only device buffer owned by the context.
+// SLOT_DATA, SLOT_PARAMS, SLOT_POST, SLOT_ENC, SLOT_MISC and SLOT_MISC2 are the
+// buffers vlhip_buffer hands to callers, who pass them back as kernel
+// arguments: kernels' own work buffers must never use them (growing one frees
+// it under the caller's pointer).
 enum Slot {
-  SLOT_DATA = 0,      // uploaded descriptors (host entry points)
-  SLOT_PARAMS,        // uploaded means / covariances / priors
+  SLOT_DATA = 0,      // caller: uploaded descriptors (host entry points)
+  SLOT_PARAMS,        // caller: uploaded means / covariances / priors
   SLOT_PREP,          // per-component derived terms (inverse covariances, constants)
-  SLOT_POST,          // N x K posteriors
+  SLOT_POST,          // caller: N x K posteriors
   SLOT_ROWSTAT,       // per-descriptor (max logit, sum) and argmax
   SLOT_PARTIAL,       // per-block partial sums (LL, M-step moments)
   SLOT_REDUCED,       // reduced M-step moments (FP64)
-  SLOT_ENC,           // encodings
-  SLOT_MISC,          // offsets, counters, assignments
-  SLOT_MISC2,
+  SLOT_ENC,           // caller: encodings
+  SLOT_MISC,          // caller: offsets, counters, assignments
+  SLOT_MISC2,         // caller
+  SLOT_SEGMENTS,      // per-segment sums of the FP32 Fisher encoder's parallel-segment mode
+  SLOT_EMCHUNKS,      // chunk offsets of the matrix-core EM step
+  SLOT_EMLL,          // log-likelihood partials of the matrix-core EM step
   SLOT_GATE,          // learned-gate activations / candidate lists
   SLOT_GATE2,
   SLOT_FISHER_WORK,   // per-batch work buffer of the Fisher encoders (posteriors or operand records)
