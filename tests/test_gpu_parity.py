@@ -206,6 +206,34 @@ def test_fisher_skips_tiny_prior_component(cfg1):
 
 # ---------------------------------------------------------------- VLAD
 
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("flags", [0, 1, 2, 8, 15])
+def test_vlad_dense_onehot_equals_hard(dt, flags):
+    # a dense matrix of one-hot rows (and empty rows) is encoded from its
+    # cluster indices: the same numbers as the dense computation
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((700, 96)).astype(dt)
+    mu = rng.standard_normal((40, 96)).astype(dt)
+    a = rng.integers(0, 40, 700).astype(np.uint32)
+    a[::97] = 0xFFFFFFFF
+    dense = np.zeros((700, 40), dt)
+    rows = np.nonzero(a != 0xFFFFFFFF)[0]
+    dense[rows, a[rows]] = 1
+    enc = va.vl_vlad(x, mu, dense, flags)
+    assert np.array_equal(enc, va.vl_vlad_hard(x, mu, a, flags))
+    assert rel(enc, oracle().vlad(x, mu, dense, flags)) < (1e-5 if dt == np.float32 else 1e-12)
+    R = reference()
+    if R is not None:
+        assert np.array_equal(enc, R.vlad(x, mu, dense, flags))
+    # one weight other than 1 keeps the dense computation
+    dense2 = dense.copy()
+    dense2[rows[0], a[rows[0]]] = 0.5
+    enc2 = va.vl_vlad(x, mu, dense2, flags)
+    assert rel(enc2, oracle().vlad(x, mu, dense2, flags)) < (1e-5 if dt == np.float32 else 1e-12)
+    if R is not None:
+        assert np.array_equal(enc2, R.vlad(x, mu, dense2, flags))
+
+
 @pytest.mark.parametrize("flags", [0, 1, 2, 4, 8, 15, 4 | 8])
 def test_vlad_soft_float(flags):
     rng = np.random.default_rng(0)

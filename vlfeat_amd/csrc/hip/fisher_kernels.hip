@@ -321,7 +321,10 @@ __global__ __launch_bounds__(256) void fisher_split_combine_kernel(
   if (tid == 0) normpart[(size_t)img * numKB + kb] = bn;
 }
 
-// fisher.c:474-485: enc /= max(||enc||_2, 1e-12).
+// fisher.c:474-485: enc /= max(||enc||_2, 1e-12).  Workgroup (image = x,
+// slice of FN_SLICE entries = y): each workgroup of an image sums the same norm partials in
+// the same order, so they all divide by the same value.
+constexpr int FN_SLICE = 8192;
 template <typename T>
 __global__ __launch_bounds__(256) void fisher_normalize_kernel(T* __restrict__ enc, const double* __restrict__ normpart,
                                                               int numKB, size_t len) {
@@ -336,12 +339,19 @@ __global__ __launch_bounds__(256) void fisher_normalize_kernel(T* __restrict__ e
   __syncthreads();
   const T n = nsh;
   T* e = enc + (size_t)img * len;
-  for (size_t i = threadIdx.x; i < len; i += blockDim.x) e[i] = e[i] / n;
+  const size_t end = min(len, (size_t)(blockIdx.y + 1) * FN_SLICE);
+  for (size_t i = (size_t)blockIdx.y * FN_SLICE + threadIdx.x; i < end; i += blockDim.x) e[i] = e[i] / n;
+}
+
+template <typename T>
+static void launch_normalize_t(T* enc, const double* normpart, int numKB, size_t numImages, size_t len) {
+  VLHIP_LAUNCH("fisher_normalize", (fisher_normalize_kernel<T>),
+               dim3((unsigned)numImages, (unsigned)((len + FN_SLICE - 1) / FN_SLICE)), dim3(256), 0, enc, normpart,
+               numKB, len);
 }
 
 void launch_fisher_normalize(float* enc, const double* normpart, int numKB, size_t numImages, size_t len) {
-  VLHIP_LAUNCH("fisher_normalize", (fisher_normalize_kernel<float>), dim3((unsigned)numImages), dim3(256), 0, enc,
-               normpart, numKB, len);
+  launch_normalize_t<float>(enc, normpart, numKB, numImages, len);
 }
 
 constexpr size_t kAccumLds = 96 * 1024;
@@ -416,8 +426,7 @@ void fisher_fp_t(T* enc, const T* means, const T* covs, const T* priors, size_t 
     g0 = g1;
   }
   if (flags & 2) {
-    VLHIP_LAUNCH("fisher_normalize", (fisher_normalize_kernel<T>), dim3((unsigned)numImages), dim3(256), 0, enc,
-                 normpart, numKB, (size_t)2 * K * D);
+    launch_normalize_t<T>(enc, normpart, numKB, numImages, (size_t)2 * K * D);
   }
 }
 

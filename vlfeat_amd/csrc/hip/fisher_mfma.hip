@@ -129,33 +129,29 @@ __device__ __forceinline__ MfmaScales mfma_scales(unsigned boundBits, bool isHal
   return s;
 }
 
+// One wave per component: lane l handles dimensions l and l + 64 (Dp <= 128);
+// the log-determinant and the quadratic term of the bias are wave sums in FP64
+// (fixed butterfly order).
 template <typename T, typename H>
-__global__ void mfma_tables_kernel(const T* __restrict__ means, const T* __restrict__ covs,
-                                   const T* __restrict__ priors, int D, int Dp, int K, int Kp,
-                                   const float* __restrict__ g, MfmaScales* __restrict__ scp,
-                                   H* __restrict__ Whi, H* __restrict__ Wlo, float* __restrict__ bias,
-                                   float* __restrict__ mup, float* __restrict__ isg, float* __restrict__ sinv) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void mfma_tables_kernel(const T* __restrict__ means, const T* __restrict__ covs,
+                                                         const T* __restrict__ priors, int D, int Dp, int K, int Kp,
+                                                         const float* __restrict__ g, MfmaScales* __restrict__ scp,
+                                                         H* __restrict__ Whi, H* __restrict__ Wlo,
+                                                         float* __restrict__ bias, float* __restrict__ mup,
+                                                         float* __restrict__ isg, float* __restrict__ sinv) {
+  const int k = blockIdx.x, lane = threadIdx.x;
+  // the bound does not change here (the scales are stored with the same bits)
   const MfmaScales sc = mfma_scales(scp->bound, std::is_same<H, _Float16>::value);
-  // every thread has read the bound before the first workgroup's thread 0 stores the scales
-  __syncthreads();
-  if (k == 0) *scp = sc;
+  if (k == 0 && lane == 0) *scp = sc;
   if (k >= Kp) return;
   const float e1 = 1.0f / sc.s1, e2 = 1.0f / sc.s2;
   H* wh = Whi + (size_t)k * 2 * Dp;
   H* wl = Wlo + (size_t)k * 2 * Dp;
-  if (k >= K) {
-    for (int f = 0; f < 2 * Dp; ++f) { wh[f] = H(0.f); wl[f] = H(0.f); }
-    for (int d = 0; d < Dp; ++d) {
-      mup[(size_t)k * Dp + d] = 0.f; isg[(size_t)k * Dp + d] = 0.f; sinv[(size_t)k * Dp + d] = 0.f;
-    }
-    bias[k] = -INFINITY;
-    return;
-  }
+  const bool real = k < K;
   double logdet = 0.0, quad = 0.0;
-  for (int d = 0; d < Dp; ++d) {
+  for (int d = lane; d < Dp; d += 64) {
     float w2 = 0.f, w1 = 0.f, mu = 0.f, is = 0.f, si = 0.f;
-    if (d < D) {
+    if (real && d < D) {
       const double c = (double)covs[(size_t)k * D + d];
       const double s = 1.0 / c;
       const double m = (double)means[(size_t)k * D + d] - (double)g[d];
@@ -176,9 +172,20 @@ __global__ void mfma_tables_kernel(const T* __restrict__ means, const T* __restr
     isg[(size_t)k * Dp + d] = is;
     sinv[(size_t)k * Dp + d] = si;
   }
-  const double p = (double)priors[k];
-  const double logw = (p < 1e-6) ? -(double)INFINITY : log(p);
-  bias[k] = (float)((logw - 0.5 * (double)D * kLog2Pi - 0.5 * logdet - 0.5 * quad) * kLog2eD);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    logdet += __shfl_xor(logdet, o, 64);
+    quad += __shfl_xor(quad, o, 64);
+  }
+  if (lane == 0) {
+    if (!real) {
+      bias[k] = -INFINITY;
+    } else {
+      const double p = (double)priors[k];
+      const double logw = (p < 1e-6) ? -(double)INFINITY : log(p);
+      bias[k] = (float)((logw - 0.5 * (double)D * kLog2Pi - 0.5 * logdet - 0.5 * quad) * kLog2eD);
+    }
+  }
 }
 
 // Split v into two 16-bit parts.
@@ -1099,8 +1106,8 @@ static void mfma_tables_launch(const MfmaTables<H>& t, const float* means, const
   VLHIP_LAUNCH("mfma_center", (mfma_center_kernel<float>), dim3(Dp), dim3(256), 0, means, priors, D, K, t.g);
   VLHIP_LAUNCH("mfma_bound", (mfma_bound_kernel<float>), dim3((unsigned)(((size_t)K * D + 255) / 256)), dim3(256), 0,
                means, covs, D, K, t.g, t.scp);
-  VLHIP_LAUNCH("mfma_tables", (mfma_tables_kernel<float, H>), dim3((Kp + 127) / 128), dim3(128), 0, means, covs,
-               priors, D, Dp, K, Kp, t.g, t.scp, t.Whi, t.Wlo, t.bias, t.mup, t.isg, t.sinv);
+  VLHIP_LAUNCH("mfma_tables", (mfma_tables_kernel<float, H>), dim3(Kp), dim3(64), 0, means, covs, priors, D, Dp, K,
+               Kp, t.g, t.scp, t.Whi, t.Wlo, t.bias, t.mup, t.isg, t.sinv);
 }
 
 // Both passes over one batch of images [b0, b0 + n) whose descriptors start at

@@ -136,27 +136,36 @@ __global__ __launch_bounds__(256) void vlad_accum_kernel(
 }
 
 // vlad.c:230-241: enc /= max(||enc||, 1e-12), the squared norm summed in
-// index order by one lane (bitwise the reference's sequential loop); the
-// workgroup stages the entries through LDS in chunks so that the lane's
-// dependent additions do not wait on global-memory latency.
+// index order by one lane (bitwise the reference's sequential loop).  The
+// workgroup stages the squares z*z (each rounded as the reference rounds it)
+// through LDS in chunks, so that the lane's only work is the chain of
+// dependent additions, fed by 16-byte LDS reads.
 constexpr int VL_NCHUNK = 4096;
 template <typename T>
 __global__ __launch_bounds__(256) void vlad_normalize_kernel(T* __restrict__ enc, size_t len) {
 #pragma clang fp contract(off)
   __shared__ T nsh;
-  __shared__ T buf[VL_NCHUNK];
+  __shared__ __align__(16) T buf[VL_NCHUNK];
   T* e = enc + (size_t)blockIdx.x * len;
   T n = T(0);
   for (size_t c0 = 0; c0 < len; c0 += VL_NCHUNK) {
     const int m = (int)min((size_t)VL_NCHUNK, len - c0);
     __syncthreads();
-    for (int i = threadIdx.x; i < m; i += blockDim.x) buf[i] = e[c0 + i];
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      const T z = e[c0 + i];
+      buf[i] = z * z;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-      for (int i = 0; i < m; ++i) {
-        const T z = buf[i];
-        n = n + z * z;
+      constexpr int V = 16 / sizeof(T);
+      typedef T vec __attribute__((ext_vector_type(V)));
+      const int mv = m / V;
+      for (int i = 0; i < mv; ++i) {
+        const vec q = reinterpret_cast<const vec*>(buf)[i];
+#pragma unroll
+        for (int t = 0; t < V; ++t) n = n + q[t];
       }
+      for (int i = mv * V; i < m; ++i) n = n + buf[i];
     }
   }
   if (threadIdx.x == 0) {
@@ -169,22 +178,28 @@ __global__ __launch_bounds__(256) void vlad_normalize_kernel(T* __restrict__ enc
 }
 
 // Hard assignments (one cluster per descriptor): workgroup (image, block of
-// VH_COMPS clusters), thread d owns dimension d of every cluster of the block
-// and walks the image's descriptors in index order, adding x_id to the LDS
+// CB clusters), thread d owns dimension d of every cluster of the block and
+// walks the image's descriptors in index order, adding x_id to the LDS
 // accumulator of the descriptor's cluster when it is in the block -- the same
 // sequence of roundings per (cluster, dimension) as vlad_accum_kernel with a
 // one-hot q (vlad.c:181-188), without scanning every cluster for every row.
-// Finalisation as vlad_accum_kernel (vlad.c:191-227).  D <= 128.
+// The cluster indices and the thread's coordinates of VH_RS rows are held in
+// registers, and the next VH_RS rows are loaded while these are added.
+// Finalisation as vlad_accum_kernel (vlad.c:191-227).  D <= 128.  Large
+// blocks (256 clusters float / 96 double) for batches, 32-cluster blocks when
+// few images would leave the device idle.
+#ifndef VH_RS
+#define VH_RS 16
+#endif
 template <typename T>
 struct VladHardBlock { static constexpr int kComps = sizeof(T) == 4 ? 256 : 96; };
 
-template <typename T>
+template <typename T, int CB>
 __global__ __launch_bounds__(128) void vlad_hard_kernel(const T* __restrict__ x, const uint64_t* __restrict__ offsets,
                                                         int numImages, int D, int K, const T* __restrict__ means,
                                                         const uint32_t* __restrict__ hard, int flags,
                                                         T* __restrict__ enc, int numKB) {
 #pragma clang fp contract(off)
-  constexpr int CB = VladHardBlock<T>::kComps;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* acc = reinterpret_cast<T*>(smem_raw);                             // [CB][D]
   double* mass = reinterpret_cast<double*>(smem_raw + (((size_t)CB * D * sizeof(T) + 15) & ~size_t(15)));
@@ -207,23 +222,32 @@ __global__ __launch_bounds__(128) void vlad_hard_kernel(const T* __restrict__ x,
   __syncthreads();
   const long long rBegin = (long long)offsets[img], rEnd = (long long)offsets[img + 1];
   const int d = tid;
-  for (long long r0 = rBegin; r0 < rEnd; r0 += 8) {
-    const int nr = (int)min((long long)8, rEnd - r0);
-    uint32_t kk[8];
-    T xv[8];
+  uint32_t kk[VH_RS], kn[VH_RS];
+  T xv[VH_RS], xn[VH_RS];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      kk[u] = u < nr ? hard[r0 + u] : 0xffffffffu;
-      xv[u] = (u < nr && d < D) ? x[(r0 + u) * D + d] : T(0);
+  for (int u = 0; u < VH_RS; ++u) {
+    const bool ok = rBegin + u < rEnd;
+    kk[u] = ok ? hard[rBegin + u] : 0xffffffffu;
+    xv[u] = (ok && d < D) ? x[(rBegin + u) * D + d] : T(0);
+  }
+  for (long long r0 = rBegin; r0 < rEnd; r0 += VH_RS) {
+    const long long r1 = r0 + VH_RS;
+#pragma unroll
+    for (int u = 0; u < VH_RS; ++u) {
+      const bool ok = r1 + u < rEnd;
+      kn[u] = ok ? hard[r1 + u] : 0xffffffffu;
+      xn[u] = (ok && d < D) ? x[(r1 + u) * D + d] : T(0);
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < VH_RS; ++u) {
       const int kl = (int)kk[u] - k0;
       if (kk[u] != 0xffffffffu && kl >= 0 && kl < CB) {
         if (d < D) acc[kl * D + d] = (T)((double)acc[kl * D + d] + (double)xv[u]);
         if (tid == 0) mass[kl] += 1.0;
       }
     }
+#pragma unroll
+    for (int u = 0; u < VH_RS; ++u) { kk[u] = kn[u]; xv[u] = xn[u]; }
   }
   __syncthreads();
   // vlad.c:191-213 per (cluster, dimension), then the per-cluster norms
@@ -267,6 +291,18 @@ __global__ __launch_bounds__(128) void vlad_hard_kernel(const T* __restrict__ x,
   }
 }
 
+template <typename T, int CB>
+static void launch_vlad_hard(T* enc, const T* means, int D, int K, const T* x, const uint64_t* offsets, int numImages,
+                             const uint32_t* hard, int flags) {
+  const int nkb = (K + CB - 1) / CB;
+  const int padImages = numImages >= 8 ? ((numImages + 7) / 8) * 8 : numImages;
+  const size_t sh = (((size_t)CB * D * sizeof(T) + 15) & ~size_t(15)) + CB * sizeof(double) + CB * sizeof(T);
+  auto kern = vlad_hard_kernel<T, CB>;
+  VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+  VLHIP_LAUNCH("vlad_hard", kern, dim3((unsigned)((size_t)padImages * nkb)), dim3(128), sh, x, offsets, numImages, D,
+               K, means, hard, flags, enc, nkb);
+}
+
 template <typename T, int DPT>
 static void launch_vlad(T* enc, const T* means, int D, int K, const T* x, const uint64_t* offsets, int numImages,
                         const T* assign, const uint32_t* hard, int flags) {
@@ -276,12 +312,11 @@ static void launch_vlad(T* enc, const T* means, int D, int K, const T* x, const 
   const unsigned blocks = (unsigned)((size_t)padImages * numKB);
   if (hard && D <= 128) {
     constexpr int CB = VladHardBlock<T>::kComps;
-    const int nkb = (K + CB - 1) / CB;
-    const size_t sh = (((size_t)CB * D * sizeof(T) + 15) & ~size_t(15)) + CB * sizeof(double) + CB * sizeof(T);
-    auto kern = vlad_hard_kernel<T>;
-    VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    VLHIP_LAUNCH("vlad_hard", kern, dim3((unsigned)((size_t)padImages * nkb)), dim3(128), sh, x, offsets, numImages,
-                 D, K, means, hard, flags, enc, nkb);
+    const int cus = ctx().num_cus > 0 ? ctx().num_cus : 256;
+    if ((long long)numImages * ((K + CB - 1) / CB) < cus)
+      launch_vlad_hard<T, 32>(enc, means, D, K, x, offsets, numImages, hard, flags);
+    else
+      launch_vlad_hard<T, CB>(enc, means, D, K, x, offsets, numImages, hard, flags);
   } else if (hard) {
     auto kern = vlad_accum_kernel<T, DPT, true>;
     VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));

@@ -14,6 +14,7 @@
 
 #include <assert.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 static __thread int g_precision = VL_FISHER_PRECISION_FP32 ;
@@ -119,6 +120,47 @@ vl_fisher_encode_batch (void * enc, vl_type dataType,
              offsets, numImages, flags, numTerms) ;
 }
 
+/* A dense assignment matrix whose rows each hold at most one positive entry,
+ * equal to 1, is a hard assignment: only positive entries contribute
+ * (vlad.c:181-188) and a weight of 1 adds the descriptor unscaled, so the
+ * encoding from the cluster indices (0xffffffff for an empty row) is the same
+ * number for number and the caller's N x K matrix need not be copied to the
+ * GPU.  Returns 0 when some row is not of that form. */
+#define VLAD_ONEHOT_ROW(NAME, TYPE)                                                  \
+  __attribute__((optimize("O3", "tree-vectorize")))                                  \
+  static int NAME (TYPE const * row, vl_size numClusters, vl_uint32 * out)           \
+  {                                                                                  \
+    unsigned positives = 0 ;                                                         \
+    vl_uindex k ;                                                                    \
+    for (k = 0 ; k < numClusters ; ++k) positives += row[k] > 0 ;                    \
+    if (positives == 0) { *out = 0xffffffffu ; return 1 ; }                          \
+    if (positives > 1) return 0 ;                                                    \
+    for (k = 0 ; k < numClusters ; ++k) {                                            \
+      if (row[k] > 0) {                                                              \
+        if (row[k] != 1) return 0 ;                                                  \
+        *out = (vl_uint32) k ;                                                       \
+        return 1 ;                                                                   \
+      }                                                                              \
+    }                                                                                \
+    return 0 ;                                                                       \
+  }
+VLAD_ONEHOT_ROW(vlad_onehot_row_f, float)
+VLAD_ONEHOT_ROW(vlad_onehot_row_d, double)
+
+static int
+vlad_onehot_indices (vl_uint32 * out, vl_type dataType, void const * soft, vl_size numData, vl_size numClusters)
+{
+  vl_uindex i ;
+  if (numClusters >= 0xffffffffu) return 0 ;
+  for (i = 0 ; i < numData ; ++i) {
+    int ok = (dataType == VL_TYPE_FLOAT)
+      ? vlad_onehot_row_f((float const *) soft + i * numClusters, numClusters, out + i)
+      : vlad_onehot_row_d((double const *) soft + i * numClusters, numClusters, out + i) ;
+    if (!ok) return 0 ;
+  }
+  return 1 ;
+}
+
 static void
 vlad_run (void * enc, vl_type dataType, void const * means, vl_size dimension, vl_size numClusters,
           void const * data, vl_size numData, void const * soft, vl_uint32 const * hard, int flags)
@@ -128,7 +170,15 @@ vlad_run (void * enc, vl_type dataType, void const * means, vl_size dimension, v
   vl_uint64 offsets [2] ;
   void * dMeans, * dData, * dEnc, * dAssign ;
   vl_uint64 * dOff ;
+  vl_uint32 * indices = NULL ;
 
+  if (soft && numData) {
+    indices = (vl_uint32*) malloc(sizeof(vl_uint32) * numData) ;
+    if (indices && vlad_onehot_indices(indices, dataType, soft, numData, numClusters)) {
+      hard = indices ;
+      soft = NULL ;
+    }
+  }
   offsets[0] = 0 ;
   offsets[1] = numData ;
   vlhip_lock() ;
@@ -148,6 +198,7 @@ vlad_run (void * enc, vl_type dataType, void const * means, vl_size dimension, v
                           soft ? dAssign : NULL, soft ? NULL : (uint32_t const*)dAssign, flags) ;
   vlhip_memcpy_d2h(enc, dEnc, es * KD) ;
   vlhip_unlock() ;
+  free(indices) ;
 }
 
 VL_EXPORT void
