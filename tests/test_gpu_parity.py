@@ -288,6 +288,33 @@ def test_quantize_bit_exact(dtype, D):
 
 
 @needs_ref
+@pytest.mark.parametrize("D,K,scale", [(128, 1024, 1.0), (37, 100, 1.0), (80, 512, 1e3), (64, 256, 1e-3),
+                                       (16, 64, 1.0)])
+def test_quantize_screened_ties_and_scales(D, K, scale):
+    # float data with D <= 128 and K >= 64 takes the matrix-core screen; the
+    # assignments and distances must still be the reference's bit for bit,
+    # including exact ties (duplicate centres, rows equal to a centre) where
+    # the first index wins
+    rng = np.random.default_rng(D + K)
+    x = (synth.sift_like(6000, D, seed=D, project=D != 128) if D in (64, 80, 128)
+         else rng.standard_normal((6000, D))).astype(np.float32) * np.float32(scale)
+    cen = x[rng.choice(x.shape[0], K, replace=False)].copy()
+    cen[7] = cen[3]
+    cen[K - 1] = cen[3]
+    x[:50] = cen[3]                       # distance 0 to centres 3, 7 and K-1
+    x[50:60] = cen[5] + np.float32(scale * 1e-4)
+    a, d = va.vl_kmeans_quantize(x, cen)
+    assert np.all(a[:50] == 3)
+    R = reference()
+    km = R.kmeans_new(np.float32)
+    R.vl_kmeans_set_centers(km, cen.ctypes.data, D, K)
+    ar, dr = R.kmeans_quantize(km, x)
+    R.vl_kmeans_delete(km)
+    assert np.array_equal(a, ar)
+    assert np.array_equal(d, dr)
+
+
+@needs_ref
 @pytest.mark.parametrize("algo,comparisons", [(va.VlKMeansLloyd, 100), (va.VlKMeansANN, 20),
                                               (va.VlKMeansANN, 5000)])
 def test_kmeans_cluster_matches_reference(algo, comparisons):

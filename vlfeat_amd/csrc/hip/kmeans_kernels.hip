@@ -15,6 +15,7 @@
 #include "vlhip.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace vlhip {
 
@@ -172,6 +173,11 @@ template <typename T>
 void quantize_t(uint32_t* assign, T* dist, const T* x, const T* centers, size_t N, size_t D, size_t K, int lanes) {
   VLHIP_REQUIRE(K >= 1 && D >= 1, "numCenters and dimension must be positive");
   if (N == 0) return;
+  if constexpr (std::is_same<T, float>::value) {
+    if ((lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8) &&
+        quantize_screened(assign, dist, x, centers, N, D, K, lanes))
+      return;
+  }
   const unsigned blocks = (unsigned)((N + QZ_ROWS - 1) / QZ_ROWS);
   switch (lanes) {
     case 8: VLHIP_LAUNCH("kmeans_quantize", (quantize_kernel<T, 8>), dim3(blocks), dim3(256), 0, x, centers,

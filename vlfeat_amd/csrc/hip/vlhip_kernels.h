@@ -33,6 +33,11 @@ template <typename T>
 double run_estep(const T* x, size_t N, size_t D, size_t K, const T* means, const GmmPrep<T>& prep,
                  T* q, T* row_m, T* row_s, int* row_arg, const char* name, bool want_ll);
 
+// Exact float quantizer screened on the matrix cores (kmeans_mfma.hip);
+// returns false (nothing launched) when the shape is not supported.
+bool quantize_screened(uint32_t* assign, float* dist, const float* x, const float* centers, size_t N, size_t D,
+                       size_t K, int lanes);
+
 // Divides every image's encoding by max(||enc||, 1e-12) (fisher.c:474-485).
 void launch_fisher_normalize(float* enc, const double* normpart, int numKB, size_t numImages, size_t len);
 

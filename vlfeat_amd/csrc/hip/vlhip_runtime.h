@@ -49,6 +49,7 @@ enum Slot {
   SLOT_SEGMENTS,      // per-segment sums of the FP32 Fisher encoder's parallel-segment mode
   SLOT_EMCHUNKS,      // chunk offsets of the matrix-core EM step
   SLOT_EMLL,          // log-likelihood partials of the matrix-core EM step
+  SLOT_QTAB,          // split centre table of the matrix-core-screened quantizer
   SLOT_GATE,          // learned-gate activations / candidate lists
   SLOT_GATE2,
   SLOT_FISHER_WORK,   // per-batch work buffer of the Fisher encoders (posteriors or operand records)
