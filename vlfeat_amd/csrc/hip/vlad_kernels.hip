@@ -183,8 +183,9 @@ __global__ __launch_bounds__(256) void vlad_normalize_kernel(T* __restrict__ enc
 // accumulator of the descriptor's cluster when it is in the block -- the same
 // sequence of roundings per (cluster, dimension) as vlad_accum_kernel with a
 // one-hot q (vlad.c:181-188), without scanning every cluster for every row.
-// The cluster indices and the thread's coordinates of VH_RS rows are held in
-// registers, and the next VH_RS rows are loaded while these are added.
+// The rows of the block are found from a ballot over 64 cluster indices and
+// their coordinates loaded VH_RS at a time, so that a few memory round trips
+// serve 64 rows.
 // Finalisation as vlad_accum_kernel (vlad.c:191-227).  D <= 128.  Large
 // blocks (256 clusters float / 96 double) for batches, 32-cluster blocks when
 // few images would leave the device idle.
@@ -221,33 +222,41 @@ __global__ __launch_bounds__(128) void vlad_hard_kernel(const T* __restrict__ x,
   for (int i = tid; i < CB; i += blockDim.x) mass[i] = 0.0;
   __syncthreads();
   const long long rBegin = (long long)offsets[img], rEnd = (long long)offsets[img + 1];
-  const int d = tid;
-  uint32_t kk[VH_RS], kn[VH_RS];
-  T xv[VH_RS], xn[VH_RS];
+  const int d = tid, lane = tid & 63;
+  // 64 rows at a time: each wave reads their cluster indices (one per lane),
+  // the rows of this block are picked from the ballot in index order, their
+  // coordinates fetched VH_RS at a time, then added in order.  The two waves
+  // own disjoint dimensions and walk the rows independently.
+  for (long long r0 = rBegin; r0 < rEnd; r0 += 64) {
+    const long long rr = r0 + lane;
+    const uint32_t kk = rr < rEnd ? hard[rr] : 0xffffffffu;
+    const int kl = (int)kk - k0;
+    unsigned long long mask = __ballot(kk != 0xffffffffu && kl >= 0 && kl < CB);
+    while (mask) {
+      int us[VH_RS], ks[VH_RS];
 #pragma unroll
-  for (int u = 0; u < VH_RS; ++u) {
-    const bool ok = rBegin + u < rEnd;
-    kk[u] = ok ? hard[rBegin + u] : 0xffffffffu;
-    xv[u] = (ok && d < D) ? x[(rBegin + u) * D + d] : T(0);
-  }
-  for (long long r0 = rBegin; r0 < rEnd; r0 += VH_RS) {
-    const long long r1 = r0 + VH_RS;
+      for (int q = 0; q < VH_RS; ++q) {
+        if (mask) {
+          const int u = __builtin_ctzll(mask);
+          mask &= mask - 1;
+          us[q] = u;
+          ks[q] = __shfl(kl, u, 64);
+        } else {
+          us[q] = -1;
+          ks[q] = -1;
+        }
+      }
+      T xv[VH_RS];
 #pragma unroll
-    for (int u = 0; u < VH_RS; ++u) {
-      const bool ok = r1 + u < rEnd;
-      kn[u] = ok ? hard[r1 + u] : 0xffffffffu;
-      xn[u] = (ok && d < D) ? x[(r1 + u) * D + d] : T(0);
-    }
+      for (int q = 0; q < VH_RS; ++q) xv[q] = (us[q] >= 0 && d < D) ? x[(r0 + us[q]) * D + d] : T(0);
 #pragma unroll
-    for (int u = 0; u < VH_RS; ++u) {
-      const int kl = (int)kk[u] - k0;
-      if (kk[u] != 0xffffffffu && kl >= 0 && kl < CB) {
-        if (d < D) acc[kl * D + d] = (T)((double)acc[kl * D + d] + (double)xv[u]);
-        if (tid == 0) mass[kl] += 1.0;
+      for (int q = 0; q < VH_RS; ++q) {
+        if (ks[q] >= 0) {
+          if (d < D) acc[ks[q] * D + d] = (T)((double)acc[ks[q] * D + d] + (double)xv[q]);
+          if (tid == 0) mass[ks[q]] += 1.0;
+        }
       }
     }
-#pragma unroll
-    for (int u = 0; u < VH_RS; ++u) { kk[u] = kn[u]; xv[u] = xn[u]; }
   }
   __syncthreads();
   // vlad.c:191-213 per (cluster, dimension), then the per-cluster norms
