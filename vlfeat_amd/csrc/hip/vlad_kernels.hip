@@ -186,14 +186,14 @@ __global__ __launch_bounds__(256) void vlad_normalize_kernel(T* __restrict__ enc
 // The rows of the block are found from a ballot over 64 cluster indices and
 // their coordinates loaded VH_RS at a time, so that a few memory round trips
 // serve 64 rows.
-// Finalisation as vlad_accum_kernel (vlad.c:191-227).  D <= 128.  Large
-// blocks (256 clusters float / 96 double) for batches, 32-cluster blocks when
-// few images would leave the device idle.
+// Finalisation as vlad_accum_kernel (vlad.c:191-227).  D <= 128.  Blocks of
+// 32 clusters: every workgroup walks all of its image's cluster indices, so
+// small blocks mean many short-lived workgroups (16 per image at K = 512)
+// that keep the device busy whatever the number of images.
 #ifndef VH_RS
 #define VH_RS 16
 #endif
-template <typename T>
-struct VladHardBlock { static constexpr int kComps = sizeof(T) == 4 ? 256 : 96; };
+constexpr int VH_COMPS = 32;
 
 template <typename T, int CB>
 __global__ __launch_bounds__(128) void vlad_hard_kernel(const T* __restrict__ x, const uint64_t* __restrict__ offsets,
@@ -320,12 +320,7 @@ static void launch_vlad(T* enc, const T* means, int D, int K, const T* x, const 
   const int padImages = numImages >= 8 ? ((numImages + 7) / 8) * 8 : numImages;
   const unsigned blocks = (unsigned)((size_t)padImages * numKB);
   if (hard && D <= 128) {
-    constexpr int CB = VladHardBlock<T>::kComps;
-    const int cus = ctx().num_cus > 0 ? ctx().num_cus : 256;
-    if ((long long)numImages * ((K + CB - 1) / CB) < cus)
-      launch_vlad_hard<T, 32>(enc, means, D, K, x, offsets, numImages, hard, flags);
-    else
-      launch_vlad_hard<T, CB>(enc, means, D, K, x, offsets, numImages, hard, flags);
+    launch_vlad_hard<T, VH_COMPS>(enc, means, D, K, x, offsets, numImages, hard, flags);
   } else if (hard) {
     auto kern = vlad_accum_kernel<T, DPT, true>;
     VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
