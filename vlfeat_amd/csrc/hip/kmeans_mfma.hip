@@ -51,12 +51,30 @@ __device__ __forceinline__ float exact_l2(const float* __restrict__ a, const flo
   float acc[LANES];
 #pragma unroll
   for (int l = 0; l < LANES; ++l) acc[l] = 0.f;
-  for (int d = 0; d < Dv; d += LANES) {
+  if (LANES >= 4 && ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15) == 0) {
+    // the same operations, fed by 16-byte loads
+    const f32x4* a4 = reinterpret_cast<const f32x4*>(a);
+    const f32x4* b4 = reinterpret_cast<const f32x4*>(b);
+    for (int d = 0; d < Dv; d += LANES) {
 #pragma unroll
-    for (int l = 0; l < LANES; ++l) {
-      const float delta = a[d + l] - b[d + l];
-      const float d2 = delta * delta;
-      acc[l] = acc[l] + d2;
+      for (int h = 0; h < LANES / 4; ++h) {
+        const f32x4 av = a4[d / 4 + h], bv = b4[d / 4 + h];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float delta = av[e] - bv[e];
+          const float d2 = delta * delta;
+          acc[4 * h + e] = acc[4 * h + e] + d2;
+        }
+      }
+    }
+  } else {
+    for (int d = 0; d < Dv; d += LANES) {
+#pragma unroll
+      for (int l = 0; l < LANES; ++l) {
+        const float delta = a[d + l] - b[d + l];
+        const float d2 = delta * delta;
+        acc[l] = acc[l] + d2;
+      }
     }
   }
   float s;
@@ -112,6 +130,12 @@ __global__ __launch_bounds__(QM_WAVES * 64) void qm_quantize_kernel(
     const __bf16* __restrict__ chi, const __bf16* __restrict__ clo, const float* __restrict__ cc,
     const unsigned* __restrict__ maxcBits, uint32_t* __restrict__ assign, float* __restrict__ dist) {
   constexpr int Dq = NQ * 32;
+  constexpr int WS = Dq + 8;                       // LDS row stride (halves) of a staged centre
+  constexpr int NT = QM_WAVES * 64;
+  constexpr int PARTW = QM_BLOCK * Dq / 8;         // 16-byte words of one part (hi or lo) of a block
+  constexpr int STG = 2 * PARTW / NT;              // words per thread per block
+  static_assert(STG * NT == 2 * PARTW, "staging must divide evenly");
+  __shared__ __align__(16) __bf16 wsh[2][2][QM_BLOCK][WS];  // [buffer][hi, lo][centre][feature]
   __shared__ float xxs[QM_WAVES][32];
   __shared__ float epss[QM_WAVES][32];
   __shared__ int cnt[QM_WAVES][32];
@@ -170,9 +194,50 @@ __global__ __launch_bounds__(QM_WAVES * 64) void qm_quantize_kernel(
 #pragma unroll
     for (int r = 0; r < 4; ++r) thr[t][r] = INFINITY;
 
-  // sweep 0: the smallest screened distance per row; sweep 1: the candidates
-  for (int sweep = 0; sweep < 2; ++sweep) {
-    for (int k0 = 0; k0 < Kp; k0 += QM_BLOCK) {
+  // centre blocks double-buffered in LDS, the next one fetched into
+  // registers while the current one is multiplied (one barrier per block);
+  // steps 0 .. nb-1 are sweep 0 (smallest screened distance per row), steps
+  // nb .. 2nb-1 sweep 1 (the candidates)
+  const int nb = Kp / QM_BLOCK, steps = 2 * nb;
+  int srcOff[STG], dstOff[STG];
+#pragma unroll
+  for (int q = 0; q < STG; ++q) {
+    const int i = tid + NT * q;
+    const int part = i / PARTW, rem = i - part * PARTW;
+    const int c = rem / (Dq / 8), q8 = rem - c * (Dq / 8);
+    srcOff[q] = part * Kp * Dq + c * Dq + 8 * q8;  // + block * 64 * Dq, halves into [chi | clo]
+    dstOff[q] = (part * QM_BLOCK + c) * WS + 8 * q8;
+  }
+  typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+  u32x4 pre[STG];
+#define QM_STAGE_LOAD(BLK)                                                              \
+  {                                                                                     \
+    const int blk_ = (BLK);                                                             \
+    _Pragma("unroll") for (int q = 0; q < STG; ++q) {                                   \
+      const int o = srcOff[q] + blk_ * QM_BLOCK * Dq;                                   \
+      const __bf16* src = o < Kp * Dq ? chi + o : clo + (o - Kp * Dq);                  \
+      pre[q] = *reinterpret_cast<const u32x4*>(src);                                    \
+    }                                                                                   \
+  }
+#define QM_STAGE_STORE(BUF)                                                             \
+  {                                                                                     \
+    __bf16* dst_ = &wsh[(BUF)][0][0][0];                                                \
+    _Pragma("unroll") for (int q = 0; q < STG; ++q)                                     \
+      *reinterpret_cast<u32x4*>(dst_ + dstOff[q]) = pre[q];                             \
+  }
+  QM_STAGE_LOAD(0)
+  QM_STAGE_STORE(0)
+  if (steps > 1) QM_STAGE_LOAD(1 % nb)
+  __syncthreads();
+  for (int st = 0; st < steps; ++st) {
+    const int sweep = st >= nb ? 1 : 0;
+    const int k0 = (st - sweep * nb) * QM_BLOCK;
+    if (st + 1 < steps) {
+      QM_STAGE_STORE((st + 1) & 1)  // that buffer was last read in step st - 1, before the barrier
+      if (st + 2 < steps) QM_STAGE_LOAD((st + 2) % nb)
+    }
+    const __bf16* wb = &wsh[st & 1][0][0][0];
+    {
       f32x4 acc[4][2];
 #pragma unroll
       for (int c = 0; c < 4; ++c) acc[c][0] = acc[c][1] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -181,9 +246,8 @@ __global__ __launch_bounds__(QM_WAVES * 64) void qm_quantize_kernel(
         bf16x8 bh[4], bl[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-          const size_t o = (size_t)(k0 + 16 * c + lr) * Dq + 32 * j + 8 * lg;
-          bh[c] = *reinterpret_cast<const bf16x8*>(chi + o);
-          bl[c] = *reinterpret_cast<const bf16x8*>(clo + o);
+          bh[c] = *reinterpret_cast<const bf16x8*>(wb + (16 * c + lr) * WS + 32 * j + 8 * lg);
+          bl[c] = *reinterpret_cast<const bf16x8*>(wb + (QM_BLOCK + 16 * c + lr) * WS + 32 * j + 8 * lg);
         }
 #pragma unroll
         for (int c = 0; c < 4; ++c)
@@ -213,7 +277,7 @@ __global__ __launch_bounds__(QM_WAVES * 64) void qm_quantize_kernel(
           }
       }
     }
-    if (sweep == 0) {
+    if (st == nb - 1) {
       // row minimum over the 16 lanes holding the row's centres, plus 2 eps
 #pragma unroll
       for (int t = 0; t < 2; ++t)
@@ -225,8 +289,10 @@ __global__ __launch_bounds__(QM_WAVES * 64) void qm_quantize_kernel(
           thr[t][r] = m + 2.f * ep[t][r];
         }
     }
+    __syncthreads();
   }
-  __syncthreads();
+#undef QM_STAGE_LOAD
+#undef QM_STAGE_STORE
 
   // exact distances: lanes l and l + 32 serve row l & 31
   const int rw = lane & 31, half = lane >> 5;
