@@ -95,3 +95,39 @@ def test_gated_peaked_mixture_vs_fp64(peaked, flags):
     assert err <= 1e-3
     assert dt <= 0.01
     assert frac >= 0.2
+
+
+def test_gated_probe_large_batch(peaked):
+    # a batch larger than the gate's probe: a gate that serves nothing on the
+    # probe is not run on the rest (dense path, bit for bit F16X3); a useful
+    # gate keeps serving the whole batch.  Over 100 images the truncation
+    # error is printed per operating point (median and worst image).
+    _, m, c, p, g = peaked
+    x = synth.mixture(300000, 32, 96, seed=11)
+    off = (np.arange(101) * 3000).astype(np.uint64)
+    with va.fisher_precision(va.MODE_F16X3):
+        e16, t16 = va.vl_fisher_batch(x, off, m, c, p, 3)
+    va.set_fisher_gate(g, 16, 1.5)
+    try:
+        with va.fisher_precision(va.MODE_GATED):
+            eg, tg = va.vl_fisher_batch(x, off, m, c, p, 3)
+    finally:
+        va.set_fisher_gate(None)
+    assert np.array_equal(eg, e16) and np.array_equal(tg, t16)
+    stats = {}
+    for need in (0.999, 0.9999, 0.99999):
+        va.set_fisher_gate(g, 16, need)
+        va.fisher_gate_stats(reset=True)
+        try:
+            with va.fisher_precision(va.MODE_GATED):
+                eg, _ = va.vl_fisher_batch(x, off, m, c, p, 3)
+        finally:
+            va.set_fisher_gate(None)
+        groups, served = va.fisher_gate_stats(reset=True)
+        errs = np.array([rel(eg[i], e16[i]) for i in range(100)])
+        stats[need] = (served / groups, float(np.median(errs)), float(errs.max()))
+        print(f"large batch need={need}: served {served}/{groups} groups, rel diff to F16X3 median "
+              f"{np.median(errs):.2e}, 90th pct {np.percentile(errs, 90):.2e}, max {errs.max():.2e}")
+    assert stats[0.999][0] >= 0.2          # the gate stays on for the whole batch
+    assert stats[0.999][1] <= 1e-3         # typical image within the bound
+    assert stats[0.99999][2] <= stats[0.999][2]

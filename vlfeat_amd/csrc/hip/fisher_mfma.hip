@@ -1055,6 +1055,8 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
 
 // Bytes of group records materialised at once (batches of whole images).
 constexpr size_t kRecordBudget = size_t(8) << 30;
+// Rows of a batch on which the learned gate is tried before it runs on the rest.
+constexpr size_t kGateProbeRows = 262144;
 
 // Device tables of one encode, carved from one buffer:
 // [scales][g Dp][bias Kp][mup, isg, sinv Kp*Dp][Whi, Wlo Kp*2Dp].
@@ -1189,7 +1191,33 @@ static void run_mfma(float* enc, const float* means, const float* covs, const fl
         float* mass = (float*)gb;
         unsigned char* conf = gb + al256(rows * sizeof(float));
         gflag = conf + al256(rows);
-        vlhip_gate_select(gu->gate, x + rowBase * D, rows, gu->M, gu->need, cand, mass, conf);
+        // The gate first runs on a probe of the batch's first rows; when it
+        // serves fewer than 1% of the probe's groups, the rest of the batch
+        // takes the dense path without running the network (every group it
+        // does not serve is dense anyway, so only the gate's cost changes).
+        const size_t probe = std::min<size_t>(rows, kGateProbeRows);
+        vlhip_gate_select(gu->gate, x + rowBase * D, probe, gu->M, gu->need, cand, mass, conf);
+        if (probe < rows) {
+          std::vector<unsigned char> hc(probe);
+          download(hc.data(), conf, probe);
+          size_t inProbe = 0, confident = 0;
+          for (size_t i = b0; i < b1; ++i) {
+            const uint64_t a = off[i] - rowBase, e = off[i + 1] - rowBase;
+            for (uint64_t r = a; r < e && r + GR <= probe; r += GR) {
+              const uint64_t re = std::min<uint64_t>(e, r + GR);
+              bool all = true;
+              for (uint64_t q = r; q < re && all; ++q) all = hc[q] != 0;
+              ++inProbe;
+              confident += all ? 1 : 0;
+            }
+          }
+          if (inProbe >= 64 && confident * 100 < inProbe) {
+            VLHIP_CHECK(hipMemsetAsync(conf + probe, 0, rows - probe, ctx().stream));
+          } else {
+            vlhip_gate_select(gu->gate, x + (rowBase + probe) * D, rows - probe, gu->M, gu->need,
+                              cand + probe * gu->M, mass + probe, conf + probe);
+          }
+        }
         list = (int2*)scratch(SLOT_GATELIST, nG * GR * gu->M * sizeof(int2));
         VLHIP_LAUNCH("mfma_sparse", mfma_sparse_kernel<NS>, dim3((unsigned)((nG + 3) / 4)), dim3(256), 0,
                      x + rowBase * D, (int)D, (long long)nG, gtab, conf, cand, gu->M, t.bias, t.mup, t.sinv, t.g,
