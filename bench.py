@@ -61,6 +61,21 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_description(R) -> str:
+    """CPU model, logical CPUs visible to this process and the SIMD path the
+    reference's distance kernels take."""
+    model = "unknown CPU"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    simd = "AVX" if (R.vl_get_simd_enabled() and R.vl_cpu_has_avx()) else "SSE2/scalar"
+    return f"{model}, {len(os.sched_getaffinity(0))} logical CPUs visible, {simd} distance kernels"
+
+
 def fit_gmm(x_host: np.ndarray, K: int, iters: int, seed: int, cov_bound: float = 1e-6):
     """GMM for the workload: EM on the GPU from random data points."""
     import vlfeat_amd as va
@@ -98,7 +113,7 @@ def cpu_legs(x_host, offsets, m, c, p, budget_s: float, gpu_enc, gpu_fp32_enc, n
         out["cpu_baseline"] = {
             "value": done_desc / dt, "unit": "descriptors/s", "cores": threads, "kind": "reference",
             "sample": f"{done_img} images x {PER_IMAGE} descriptors (K={K_COMP}, D={DIM}, improved FV), "
-                      f"vl_fisher_encode per image, OpenMP inside the image, {dt:.1f} s"}
+                      f"vl_fisher_encode per image, OpenMP inside the image, {dt:.1f} s; {host_description(R)}"}
         # image-parallel upper bound of the CPU path (as an application's parfor
         # over images): one host thread per image, no OpenMP inside a call
         import concurrent.futures as cf
