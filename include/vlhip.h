@@ -41,6 +41,7 @@ VLHIP_API void * vlhip_malloc (size_t bytes) ;
 VLHIP_API void   vlhip_free (void * ptr) ;
 VLHIP_API void   vlhip_memcpy_h2d (void * dst, void const * src_host, size_t bytes) ;
 VLHIP_API void   vlhip_memcpy_d2h (void * dst_host, void const * src, size_t bytes) ;
+VLHIP_API void   vlhip_memcpy_d2d (void * dst, void const * src, size_t bytes) ;  /* stream-ordered */
 VLHIP_API void   vlhip_memset (void * dst, int value, size_t bytes) ;
 VLHIP_API void   vlhip_synchronize (void) ;
 VLHIP_API int    vlhip_num_compute_units (void) ;
@@ -182,6 +183,22 @@ VLHIP_API void vlhip_kmeans_quantize (int dtype, uint32_t * assignments, void * 
 VLHIP_API void vlhip_l2_to_point (int dtype, void * distances, void const * data, void const * point,
                                   size_t numData, size_t dimension, int lanes) ;
 
+/* Center update of Lloyd and ANN refinement, kmeans.c:899-931 (Lloyd) and
+ * 1075-1107 (ANN), L2 distance: center c becomes the mean of the data
+ * members[starts[c] .. starts[c+1]) (data indices in increasing order, so the
+ * float sums are the reference's, bit for bit), or datum restartRow[c] when
+ * the cluster is empty.  starts has numCenters + 1 entries. */
+VLHIP_API void vlhip_kmeans_update_centers (int dtype, void * centers, void const * data,
+                                            uint32_t const * members, uint64_t const * starts,
+                                            uint64_t const * restartRow, size_t numCenters, size_t dimension) ;
+
+/* Keep-previous rule of an ANN assignment update, kmeans.c:750-763: datum i
+ * keeps center previous[i], at its distance to that center (same lane order as
+ * the quantizer), unless distances[i] (from the new search) is strictly
+ * smaller.  assignments and distances are updated in place. */
+VLHIP_API void vlhip_kmeans_ann_merge (int dtype, uint32_t * assignments, void * distances,
+                                       uint32_t const * previous, void const * data, void const * centers,
+                                       size_t numData, size_t dimension, int lanes) ;
 
 /* ---- learned gate of the Fisher encoder (no counterpart in the reference) --
  * A perceptron dimension -> hidden1 -> hidden2 -> numClusters with GELU

@@ -330,3 +330,51 @@ def test_kmeans_cluster_matches_reference(algo, comparisons):
         lib.vl_kmeans_delete(km)
     assert np.array_equal(res[0][0], res[1][0])
     assert res[0][1] == res[1][1]
+
+
+@needs_ref
+@pytest.mark.parametrize("dtype,D,algo", [(np.float32, 13, va.VlKMeansLloyd), (np.float32, 24, va.VlKMeansANN),
+                                          (np.float64, 13, va.VlKMeansLloyd), (np.float64, 16, va.VlKMeansANN)])
+def test_kmeans_refine_with_empty_clusters_matches_reference(dtype, D, algo):
+    """Refinement from given centers, a quarter of them far from every datum:
+    those clusters are empty after the first assignment and restart from data
+    drawn by the seeded generator (kmeans.c:919-925 / 1095-1101); centers and
+    energy must equal the reference's bit for bit (Lloyd and an exhaustive ANN
+    search, which also applies the keep-previous rule of kmeans.c:750-763)."""
+    N, K = 30000, 48
+    x = synth.mixture(N, D, 12, seed=21, dtype=dtype)
+    rng = np.random.default_rng(3)
+    cen0 = x[rng.choice(N, K, replace=False)].copy()
+    cen0[::4] += dtype(50.0)
+    R = reference()
+    R.set_num_threads(1)
+    res = []
+    for lib in (va.lib(), R):
+        lib.seed(11)
+        km = lib.kmeans_new(dtype, algorithm=algo, max_iter=15, num_trees=2, max_comparisons=0)
+        lib.vl_kmeans_set_centers(km, cen0.ctypes.data, D, K)
+        e = lib.vl_kmeans_refine_centers(km, x.ctypes.data, N)
+        res.append((lib.kmeans_centers(km, dtype), e))
+        lib.vl_kmeans_delete(km)
+    assert np.array_equal(res[0][0], res[1][0])
+    assert res[0][1] == res[1][1]
+
+
+@needs_ref
+def test_kmeans_cluster_large_clusters_matches_reference():
+    """Lloyd on 200,000 descriptors and 16 clusters (12,500 members per cluster
+    on average, the unrolled part of the serial per-cluster sums): bit-exact
+    centers and energy."""
+    N, D, K = 200_000, 80, 16
+    x = synth.sift_like(N, D, seed=5)
+    R = reference()
+    R.set_num_threads(0)
+    res = []
+    for lib in (va.lib(), R):
+        lib.seed(2)
+        km = lib.kmeans_new(np.float32, algorithm=va.VlKMeansLloyd, max_iter=8)
+        e = lib.vl_kmeans_cluster(km, x.ctypes.data, D, N, K)
+        res.append((lib.kmeans_centers(km, np.float32), e))
+        lib.vl_kmeans_delete(km)
+    assert np.array_equal(res[0][0], res[1][0])
+    assert res[0][1] == res[1][1]

@@ -1,5 +1,6 @@
-// kmeans_kernels.hip -- exact L2 quantizer (vl/kmeans.c:655-702) and the
-// point-to-data distances used by k-means++ seeding (kmeans.c:610-616).
+// kmeans_kernels.hip -- exact L2 quantizer (vl/kmeans.c:655-702), the
+// point-to-data distances used by k-means++ seeding (kmeans.c:610-616), and
+// the center update and ANN keep-previous rule of the refinement loops.
 //
 // Bit-exact argmin parity: the reference evaluates each squared L2 distance
 // with the host's SIMD kernel -- LANES partial sums over d = l (mod LANES) for
@@ -169,6 +170,82 @@ __global__ __launch_bounds__(256) void l2_to_point_kernel(const T* __restrict__ 
   out[i] = s;
 }
 
+// Center update of Lloyd / ANN refinement (kmeans.c:899-931 and 1075-1107):
+// the reference zeroes the centers and adds every datum to its center in data
+// order, then divides by the cluster mass, or copies a random datum into an
+// empty center.  Here members[starts[c] .. starts[c+1]) lists the data of
+// cluster c in increasing data order, so each (c, d) sum below is the same
+// sequence of float (or double) additions as the reference's and the result
+// is bitwise equal; restartRow[c] is the datum drawn for an empty cluster.
+// Block (c, dimension slice of 64); the member indices are uniform per block,
+// eight data loads are issued before their adds.
+template <typename T>
+__global__ __launch_bounds__(64) void center_update_kernel(const T* __restrict__ x,
+                                                           const uint32_t* __restrict__ members,
+                                                           const uint64_t* __restrict__ starts,
+                                                           const uint64_t* __restrict__ restartRow, int D,
+                                                           T* __restrict__ centers) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.x;
+  const int d = blockIdx.y * 64 + threadIdx.x;
+  if (d >= D) return;
+  const uint64_t b = starts[c], e = starts[c + 1];
+  T out;
+  if (e > b) {
+    T s = T(0);
+    uint64_t j = b;
+    for (; j + 8 <= e; j += 8) {
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = x[(size_t)members[j + u] * D + d];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s = s + v[u];
+    }
+    for (; j < e; ++j) s = s + x[(size_t)members[j] * D + d];
+    out = s / (T)(e - b);
+  } else {
+    out = x[(size_t)restartRow[c] * D + d];
+  }
+  centers[(size_t)c * D + d] = out;
+}
+
+// ANN assignment update (kmeans.c:750-763 with update): a datum keeps its
+// previous center, at the distance to that center, unless the new nearest
+// center is strictly closer.  Distances in the host lane order.
+template <typename T, int LANES>
+__global__ __launch_bounds__(256) void ann_merge_kernel(const T* __restrict__ x, const T* __restrict__ centers,
+                                                        long long N, int D, const uint32_t* __restrict__ prev,
+                                                        uint32_t* __restrict__ assign, T* __restrict__ dist) {
+#pragma clang fp contract(off)
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const uint32_t a = prev[i];
+  const T* xi = x + i * D;
+  const T* ci = centers + (size_t)a * D;
+  const int Dv = (LANES > 1) ? (D / LANES) * LANES : 0;
+  T acc[LANES];
+#pragma unroll
+  for (int l = 0; l < LANES; ++l) acc[l] = T(0);
+  for (int d = 0; d < Dv; d += LANES) {
+#pragma unroll
+    for (int l = 0; l < LANES; ++l) {
+      const T delta = xi[d + l] - ci[d + l];
+      const T d2 = delta * delta;
+      acc[l] = acc[l] + d2;
+    }
+  }
+  T s = (LANES > 1) ? lane_hsum<T, LANES>(acc) : T(0);
+  for (int d = Dv; d < D; ++d) {
+    const T delta = xi[d] - ci[d];
+    const T d2 = delta * delta;
+    s = s + d2;
+  }
+  if (!(s > dist[i])) {
+    assign[i] = a;
+    dist[i] = s;
+  }
+}
+
 template <typename T>
 void quantize_t(uint32_t* assign, T* dist, const T* x, const T* centers, size_t N, size_t D, size_t K, int lanes) {
   VLHIP_REQUIRE(K >= 1 && D >= 1, "numCenters and dimension must be positive");
@@ -212,6 +289,24 @@ void l2_to_point_t(T* out, const T* x, const T* point, size_t N, size_t D, int l
 template void quantize_t<float>(uint32_t*, float*, const float*, const float*, size_t, size_t, size_t, int);
 template void quantize_t<double>(uint32_t*, double*, const double*, const double*, size_t, size_t, size_t, int);
 
+template <typename T>
+void ann_merge_t(uint32_t* assign, T* dist, const uint32_t* prev, const T* x, const T* centers, size_t N,
+                        size_t D, int lanes) {
+  if (N == 0) return;
+  const unsigned blocks = (unsigned)((N + 255) / 256);
+  switch (lanes) {
+    case 8: VLHIP_LAUNCH("kmeans_ann_merge", (ann_merge_kernel<T, 8>), dim3(blocks), dim3(256), 0, x, centers,
+                         (long long)N, (int)D, prev, assign, dist); break;
+    case 4: VLHIP_LAUNCH("kmeans_ann_merge", (ann_merge_kernel<T, 4>), dim3(blocks), dim3(256), 0, x, centers,
+                         (long long)N, (int)D, prev, assign, dist); break;
+    case 2: VLHIP_LAUNCH("kmeans_ann_merge", (ann_merge_kernel<T, 2>), dim3(blocks), dim3(256), 0, x, centers,
+                         (long long)N, (int)D, prev, assign, dist); break;
+    case 1: VLHIP_LAUNCH("kmeans_ann_merge", (ann_merge_kernel<T, 1>), dim3(blocks), dim3(256), 0, x, centers,
+                         (long long)N, (int)D, prev, assign, dist); break;
+    default: fatal("vlhip_kmeans_ann_merge", __FILE__, __LINE__, "lanes must be 1, 2, 4 or 8");
+  }
+}
+
 }  // namespace vlhip
 
 using namespace vlhip;
@@ -239,6 +334,39 @@ void vlhip_l2_to_point(int dtype, void* out, void const* data, void const* point
     l2_to_point_t<double>((double*)out, (const double*)data, (const double*)point, N, D, lanes);
   else
     fatal("vlhip_l2_to_point", __FILE__, __LINE__, "unknown data type");
+}
+
+void vlhip_kmeans_update_centers(int dtype, void* centers, void const* data, uint32_t const* members,
+                                 uint64_t const* starts, uint64_t const* restartRow, size_t numCenters,
+                                 size_t dimension) {
+  ensure_init();
+  std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+  VLHIP_REQUIRE(dimension >= 1 && dimension < (1u << 30), "dimension out of range");
+  VLHIP_REQUIRE(numCenters < (1u << 31), "numCenters out of range");
+  if (numCenters == 0) return;
+  const dim3 grid((unsigned)numCenters, (unsigned)((dimension + 63) / 64));
+  if (dtype == 1)
+    VLHIP_LAUNCH("kmeans_update", (center_update_kernel<float>), grid, dim3(64), 0, (const float*)data, members,
+                 starts, restartRow, (int)dimension, (float*)centers);
+  else if (dtype == 2)
+    VLHIP_LAUNCH("kmeans_update", (center_update_kernel<double>), grid, dim3(64), 0, (const double*)data, members,
+                 starts, restartRow, (int)dimension, (double*)centers);
+  else
+    fatal("vlhip_kmeans_update_centers", __FILE__, __LINE__, "unknown data type");
+}
+
+void vlhip_kmeans_ann_merge(int dtype, uint32_t* assignments, void* distances, uint32_t const* previous,
+                            void const* data, void const* centers, size_t numData, size_t dimension, int lanes) {
+  ensure_init();
+  std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+  if (dtype == 1)
+    ann_merge_t<float>(assignments, (float*)distances, previous, (const float*)data, (const float*)centers, numData,
+                       dimension, lanes);
+  else if (dtype == 2)
+    ann_merge_t<double>(assignments, (double*)distances, previous, (const double*)data, (const double*)centers,
+                        numData, dimension, lanes);
+  else
+    fatal("vlhip_kmeans_ann_merge", __FILE__, __LINE__, "unknown data type");
 }
 
 }  // extern "C"

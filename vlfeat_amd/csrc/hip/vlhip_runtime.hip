@@ -171,6 +171,13 @@ void vlhip_memcpy_d2h(void* dst, void const* src, size_t bytes) {
   download(dst, src, bytes);
 }
 
+void vlhip_memcpy_d2d(void* dst, void const* src, size_t bytes) {
+  ensure_init();
+  std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+  if (bytes == 0) return;
+  VLHIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx().stream));
+}
+
 void vlhip_memset(void* dst, int value, size_t bytes) {
   ensure_init();
   std::lock_guard<std::recursive_mutex> lk(ctx().mu);

@@ -6,9 +6,12 @@
  *   k-means++ seeding           kmeans.c:588-649 (host RNG; distances on the GPU in
  *                               the host's lane order, sums on the host in data order)
  *   exact quantization          kmeans.c:655-702 (GPU, bitwise-identical distances)
- *   Lloyd refinement            kmeans.c:828-978 (GPU assignment, host center update
- *                               summed in data order)
- *   ANN refinement              kmeans.c:1003-1156 (randomised kd-forest, kdtree.c)
+ *   Lloyd refinement            kmeans.c:828-978 (GPU assignment and center update,
+ *                               sums in data order; energy, masses and restart
+ *                               draws on the host)
+ *   ANN refinement              kmeans.c:1003-1156 (exhaustive search: as Lloyd plus
+ *                               the keep-previous rule on the GPU; otherwise the
+ *                               randomised kd-forest of kdtree.c on the host)
  *   repetitions                 kmeans.c:2023-2093
  * Only the L2 distance is provided on this path.  Elkan refinement returns the
  * Lloyd solution (both are exact Lloyd iterations; Elkan only skips distance

@@ -170,34 +170,90 @@ FN(km_quantize_ann) (VlKMeans * self, vl_uint32 * assignments, TYPE * distances,
   vl_kdforest_delete(forest) ;
 }
 
-/* Lloyd (kmeans.c:828-978) and ANN (kmeans.c:1003-1156) refinement: the
- * two differ only in how assignments are obtained. */
+/* The kd-forest build of an ANN quantization (kmeans.c:723-726) draws from
+ * the calling thread's generator; an exhaustive search does not need the
+ * forest, but it is built so that the generator advances as in the reference. */
+static void
+FN(km_forest_draws) (VlKMeans * self)
+{
+  VlKDForest * forest = vl_kdforest_new(self->dataType, self->dimension, self->numTrees, self->distance) ;
+  vl_kdforest_set_max_num_comparisons(forest, self->maxNumComparisons) ;
+  vl_kdforest_set_thresholding_method(forest, VL_KDTREE_MEDIAN) ;
+  vl_kdforest_build(forest, self->numCenters, self->centers) ;
+  vl_kdforest_delete(forest) ;
+}
+
+/* Lloyd (kmeans.c:828-978) and ANN (kmeans.c:1003-1156) refinement; the two
+ * differ only in how assignments are obtained.  The data are staged on the
+ * GPU once.  Per iteration the GPU computes the assignments (exact quantizer;
+ * for an exhaustive ANN search also the keep-previous rule of kmeans.c:750-763)
+ * and the new centers; the host sums the energy in data order in double
+ * (kmeans.c:862-864), counts the cluster masses, lists each cluster's data in
+ * data order and draws the restart datum of every empty cluster in cluster
+ * order (kmeans.c:919-925), so the generator and every center agree with the
+ * reference bit for bit.  A non-exhaustive ANN search runs the kd-forest on
+ * the host. */
 static double
 FN(km_refine) (VlKMeans * self, TYPE const * data, vl_size numData)
 {
-  vl_size c, d, x, iteration ;
+  vl_size const K = self->numCenters, D = self->dimension ;
+  vl_size const nd = numData ? numData : 1 ;
+  vl_size const es = sizeof(TYPE) ;
+  vl_size c, x, iteration ;
   double previousEnergy = VL_INFINITY_D ;
   double initialEnergy = VL_INFINITY_D ;
   double energy ;
-  vl_bool ann = (self->algorithm == VlKMeansANN) ;
-  TYPE * distances = (TYPE*) vl_malloc(sizeof(TYPE) * (numData ? numData : 1)) ;
-  vl_uint32 * assignments = (vl_uint32*) vl_malloc(sizeof(vl_uint32) * (numData ? numData : 1)) ;
-  vl_size * clusterMasses = (vl_size*) vl_malloc(sizeof(vl_size) * (numData > self->numCenters ? numData : self->numCenters)) ;
+  vl_bool const ann = (self->algorithm == VlKMeansANN) ;
+  vl_bool const annExhaustive = ann && (self->maxNumComparisons == 0 || self->maxNumComparisons >= K) ;
+  int const lanes = host_lanes(self->dataType) ;
+  TYPE * distances = (TYPE*) vl_malloc(es * nd) ;
+  vl_uint32 * assignments = (vl_uint32*) vl_malloc(sizeof(vl_uint32) * nd) ;
+  vl_size * clusterMasses = (vl_size*) vl_malloc(sizeof(vl_size) * (K ? K : 1)) ;
+  vl_size * cursor = (vl_size*) vl_malloc(sizeof(vl_size) * (K ? K : 1)) ;
+  /* one upload per iteration: members (numData indices), then the cluster
+   * starts (K + 1) and the restart data (K) as 64-bit values */
+  vl_size const memBytes = (sizeof(vl_uint32) * nd + 7) & ~(vl_size)7 ;
+  vl_size const updBytes = memBytes + sizeof(vl_uint64) * (2 * K + 1) ;
+  unsigned char * upd = (unsigned char*) vl_malloc(updBytes) ;
+  vl_uint32 * members = (vl_uint32*) upd ;
+  vl_uint64 * starts = (vl_uint64*) (upd + memBytes) ;
+  vl_uint64 * restartRow = starts + K + 1 ;
   VlRand * rand = vl_get_rand() ;
-  void * dData = NULL ;
+  void * dData, * dCenters, * dDist ;
+  unsigned char * dUpd ;
+  vl_uint32 * dAssign, * dPrev ;
 
-  if (!ann) {
-    vlhip_lock() ;
-    dData = vlhip_buffer(VLHIP_BUF_DATA, sizeof(TYPE) * self->dimension * (numData ? numData : 1)) ;
-    if (numData) vlhip_memcpy_h2d(dData, data, sizeof(TYPE) * self->dimension * numData) ;
+  if (numData > 0xffffffffu) {
+    VL_PRINTF("vl_kmeans_refine_centers: more than 2^32 data are not supported\n") ;
+    abort() ;
   }
+  vlhip_lock() ;
+  dData = vlhip_buffer(VLHIP_BUF_DATA, es * D * nd) ;
+  dCenters = vlhip_buffer(VLHIP_BUF_PARAMS, es * D * (K ? K : 1)) ;
+  dAssign = (vl_uint32*) vlhip_buffer(VLHIP_BUF_AUX, sizeof(vl_uint32) * nd) ;
+  dDist = vlhip_buffer(VLHIP_BUF_AUX2, es * nd) ;
+  dPrev = (vl_uint32*) vlhip_buffer(VLHIP_BUF_POST, sizeof(vl_uint32) * nd) ;
+  dUpd = (unsigned char*) vlhip_buffer(VLHIP_BUF_ENC, updBytes) ;
+  if (numData) vlhip_memcpy_h2d(dData, data, es * D * numData) ;
 
   for (energy = VL_INFINITY_D, iteration = 0 ; 1 ; ++iteration) {
     vl_size numRestarted = 0 ;
-    if (ann) {
+    if (ann && !annExhaustive) {
       FN(km_quantize_ann)(self, assignments, distances, data, numData, iteration > 0) ;
     } else {
-      quantize_staged(self, dData, numData, assignments, distances) ;
+      if (annExhaustive) FN(km_forest_draws)(self) ;
+      vlhip_memcpy_h2d(dCenters, self->centers, es * D * K) ;
+      if (annExhaustive && iteration > 0 && numData) {
+        vlhip_memcpy_d2d(dPrev, dAssign, sizeof(vl_uint32) * numData) ;
+      }
+      vlhip_kmeans_quantize(TYPE_CODE, dAssign, dDist, dData, dCenters, numData, D, K, lanes) ;
+      if (annExhaustive && iteration > 0) {
+        vlhip_kmeans_ann_merge(TYPE_CODE, dAssign, dDist, dPrev, dData, dCenters, numData, D, lanes) ;
+      }
+      if (numData) {
+        vlhip_memcpy_d2h(assignments, dAssign, sizeof(vl_uint32) * numData) ;
+        vlhip_memcpy_d2h(distances, dDist, es * numData) ;
+      }
     }
     energy = 0 ;
     for (x = 0 ; x < numData ; ++x) energy += distances[x] ;
@@ -213,34 +269,39 @@ FN(km_refine) (VlKMeans * self, TYPE const * data, vl_size numData)
       if (eps < self->minEnergyVariation) break ;
     }
     previousEnergy = energy ;
+    /* the reference would draw restart data from an empty set here */
+    if (numData == 0) break ;
 
-    memset(clusterMasses, 0, sizeof(vl_size) * (numData > self->numCenters ? numData : self->numCenters)) ;
+    memset(clusterMasses, 0, sizeof(vl_size) * K) ;
     for (x = 0 ; x < numData ; ++x) clusterMasses[assignments[x]] ++ ;
-    memset(self->centers, 0, sizeof(TYPE) * self->dimension * self->numCenters) ;
-    for (x = 0 ; x < numData ; ++x) {
-      TYPE * cpt = (TYPE*)self->centers + assignments[x] * self->dimension ;
-      TYPE const * xpt = data + x * self->dimension ;
-      for (d = 0 ; d < self->dimension ; ++d) cpt[d] += xpt[d] ;
+    starts[0] = 0 ;
+    for (c = 0 ; c < K ; ++c) {
+      cursor[c] = starts[c] ;
+      starts[c + 1] = starts[c] + clusterMasses[c] ;
     }
-    for (c = 0 ; c < self->numCenters ; ++c) {
-      TYPE * cpt = (TYPE*)self->centers + c * self->dimension ;
-      if (clusterMasses[c] > 0) {
-        TYPE mass = (TYPE) clusterMasses[c] ;
-        for (d = 0 ; d < self->dimension ; ++d) cpt[d] /= mass ;
-      } else {
-        vl_uindex r = vl_rand_uindex(rand, numData) ;
+    for (x = 0 ; x < numData ; ++x) members[cursor[assignments[x]] ++] = (vl_uint32) x ;
+    for (c = 0 ; c < K ; ++c) {
+      restartRow[c] = 0 ;
+      if (clusterMasses[c] == 0) {
+        restartRow[c] = vl_rand_uindex(rand, numData) ;
         numRestarted ++ ;
-        for (d = 0 ; d < self->dimension ; ++d) cpt[d] = data[r * self->dimension + d] ;
       }
     }
+    vlhip_memcpy_h2d(dUpd, upd, updBytes) ;
+    vlhip_kmeans_update_centers(TYPE_CODE, dCenters, dData, (vl_uint32 const*) dUpd,
+                                (vl_uint64 const*) (dUpd + memBytes), (vl_uint64 const*) (dUpd + memBytes) + K + 1,
+                                K, D) ;
+    vlhip_memcpy_d2h(self->centers, dCenters, es * D * K) ;
     if (self->verbosity && numRestarted) {
       VL_PRINTF("kmeans: iter %d: restarted %d centers\n", (int)iteration, (int)numRestarted) ;
     }
   }
-  if (!ann) vlhip_unlock() ;
+  vlhip_unlock() ;
   vl_free(distances) ;
   vl_free(assignments) ;
   vl_free(clusterMasses) ;
+  vl_free(cursor) ;
+  vl_free(upd) ;
   return energy ;
 }
 
