@@ -332,13 +332,14 @@ __device__ __forceinline__ void lse_mfma(f32x4 (&acc)[4][2], const typename VecO
 }
 
 // Online log-sum-exp update with the logits of components cb .. cb + 63
-// (lane holds components cb + 16c + lr); with ARG also the first maximum.
+// (lane holds components cb + 16c + lr; bblk holds their 64 biases, staged in
+// LDS); with ARG also the first maximum.
 template <bool ARG>
-__device__ __forceinline__ void lse_update(const f32x4 (&acc)[4][2], const float* __restrict__ bias, int cb, int lr,
+__device__ __forceinline__ void lse_update(const f32x4 (&acc)[4][2], const float* bblk, int cb, int lr,
                                            LseState& st) {
   float bc[4];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) bc[c] = bias[cb + 16 * c + lr];
+  for (int c = 0; c < 4; ++c) bc[c] = bblk[16 * c + lr];
 #pragma unroll
   for (int t = 0; t < 2; ++t)
 #pragma unroll
@@ -387,12 +388,12 @@ template <typename H, int NS, bool ARG>
 __device__ __forceinline__ void lse_fused(f32x4 (&accNew)[4][2], const f32x4 (&accOld)[4][2],
                                           const typename VecOf<H>::type (&ahi)[2][NS],
                                           const typename VecOf<H>::type (&alo)[2][NS], const H* w, int lr, int lg,
-                                          const float* __restrict__ bias, int cbOld, LseState& st) {
+                                          const float* bblkOld, int cbOld, LseState& st) {
   typedef typename VecOf<H>::type V8;
   constexpr int WS = 2 * NS * 16 + 8;
   float bc[4];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) bc[c] = bias[cbOld + 16 * c + lr];
+  for (int c = 0; c < 4; ++c) bc[c] = bblkOld[16 * c + lr];
 #pragma unroll
   for (int c = 0; c < 4; ++c) accNew[c][0] = accNew[c][1] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -456,6 +457,12 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
   constexpr int STG = 2 * PART / NT;             // words per thread per block (= NS)
   static_assert(STG * NT == 2 * PART, "staging must divide evenly");
   __shared__ __align__(16) H wsh[2][2][MB_COMPS][WS];  // [buffer][hi, lo][comp][feature]
+  // Biases of the staged blocks (block B in slot B & 3), fetched and stored
+  // with the block's weights: the log-sum-exp update of block b - 1 reads them
+  // from LDS while block b + 1 is being stored, so no global load is waited
+  // for inside the sweep (a global load issued after the prefetch of block
+  // b + 2 would make the wave wait for that prefetch as well).
+  __shared__ float bsh[4][MB_COMPS];
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int lr = lane & 15, lg = lane >> 4;
@@ -487,6 +494,7 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
   }
   u32x4 wpre[STG];
   w_load<H, STG>(wpre, Whi, Wlo, 0, Dp, srcLo, srcOff);
+  float bpre = tid < MB_COMPS ? bias[tid] : 0.f;
 
   // Stage the wave's 32 centred descriptors x' = x - g (zero past D and past
   // the image) in LDS with 16-byte loads; the weight buffers are free until
@@ -574,13 +582,23 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
   if (!__syncthreads_or(need)) return;
   const int nb = Kp / MB_COMPS;
   f32x4 accA[4][2], accB[4][2];
-  w_store<H, STG>(wpre, &wsh[0][0][0][0], dstOff);
-  if (nb > 1) w_load<H, STG>(wpre, Whi, Wlo, MB_COMPS, Dp, srcLo, srcOff);
+#define LSE_STAGE(B)                                                                                  \
+  {                                                                                                   \
+    w_store<H, STG>(wpre, &wsh[(B) & 1][0][0][0], dstOff);                                            \
+    if (tid < MB_COMPS) bsh[(B) & 3][tid] = bpre;                                                     \
+  }
+#define LSE_FETCH(B)                                                                                  \
+  {                                                                                                   \
+    w_load<H, STG>(wpre, Whi, Wlo, (B) * MB_COMPS, Dp, srcLo, srcOff);                                \
+    if (tid < MB_COMPS) bpre = bias[(B) * MB_COMPS + tid];                                            \
+  }
+  LSE_STAGE(0)
+  if (nb > 1) LSE_FETCH(1)
   __syncthreads();
   if (need) lse_mfma<H, NS>(accA, ahi, alo, &wsh[0][0][0][0], lr, lg);
   if (nb > 1) {
-    w_store<H, STG>(wpre, &wsh[1][0][0][0], dstOff);
-    if (nb > 2) w_load<H, STG>(wpre, Whi, Wlo, 2 * MB_COMPS, Dp, srcLo, srcOff);
+    LSE_STAGE(1)
+    if (nb > 2) LSE_FETCH(2)
   }
   __syncthreads();
   // step b: stage block b + 1 (its buffer held block b - 1, read before the
@@ -589,12 +607,12 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
   {                                                                                                   \
     const int b_ = (B);                                                                               \
     if (b_ + 1 < nb) {                                                                                \
-      w_store<H, STG>(wpre, &wsh[(b_ + 1) & 1][0][0][0], dstOff);                                     \
-      if (b_ + 2 < nb) w_load<H, STG>(wpre, Whi, Wlo, (b_ + 2) * MB_COMPS, Dp, srcLo, srcOff);         \
+      LSE_STAGE(b_ + 1)                                                                               \
+      if (b_ + 2 < nb) LSE_FETCH(b_ + 2)                                                              \
     }                                                                                                 \
     if (need)                                                                                         \
-      lse_fused<H, NS, ARG>(ACC_NEW, ACC_OLD, ahi, alo, &wsh[b_ & 1][0][0][0], lr, lg, bias,          \
-                            (b_ - 1) * MB_COMPS, st);                                                 \
+      lse_fused<H, NS, ARG>(ACC_NEW, ACC_OLD, ahi, alo, &wsh[b_ & 1][0][0][0], lr, lg,                \
+                            &bsh[(b_ - 1) & 3][0], (b_ - 1) * MB_COMPS, st);                          \
     __syncthreads();                                                                                  \
   }
   int b = 1;
@@ -604,11 +622,13 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
   }
   if (b < nb) {
     LSE_STEP(accB, accA, b)
-    if (need) lse_update<ARG>(accB, bias, (nb - 1) * MB_COMPS, lr, st);
+    if (need) lse_update<ARG>(accB, &bsh[(nb - 1) & 3][0], (nb - 1) * MB_COMPS, lr, st);
   } else {
-    if (need) lse_update<ARG>(accA, bias, (nb - 1) * MB_COMPS, lr, st);
+    if (need) lse_update<ARG>(accA, &bsh[(nb - 1) & 3][0], (nb - 1) * MB_COMPS, lr, st);
   }
 #undef LSE_STEP
+#undef LSE_STAGE
+#undef LSE_FETCH
 
   // merge the 16 lanes that hold the same rows (lanes lg*16 .. lg*16+15)
   const float sq = scp->sq;
@@ -853,6 +873,9 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
 
   for (long long gi = gB; gi < gE; ++gi) {
     const int buf = (int)((gi - gB) & 1);
+    // read before the prefetch of group gi + 2 is issued, so that waiting for
+    // the flag does not wait for that prefetch
+    const bool gateServed = gflag != nullptr && gflag[gi] != 0;
     if (gi + 1 < gE) {
       // every wave finished reading this buffer (group gi - 1) before the last barrier
       rec_store<PER, NT, NV>(pre, smem_raw + (size_t)(buf ^ 1) * R::kBytes);
@@ -862,7 +885,7 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
     const V8* Ap = reinterpret_cast<const V8*>(cur);
     const V8* Bp = reinterpret_cast<const V8*>(cur + R::kA);
     V8 aq[CT];
-    if (gflag != nullptr && gflag[gi] != 0) {
+    if (gateServed) {
       // gate-served group: scatter its listed posteriors into a rows x BC tile
       float* tile = reinterpret_cast<float*>(smem_raw + 2 * R::kBytes);
 #pragma unroll
