@@ -152,10 +152,31 @@ VLHIP_API void * vlhip_encode_plan_new (size_t dimension, size_t numClusters, ui
 VLHIP_API void   vlhip_encode_plan_run (void * plan, void const * data, void const * means,
                                         void const * covariances, void const * priors, void * fisherEnc,
                                         void * vladEnc, unsigned long long * numTerms) ;
+/* Load stage of a plan (apps/recognition/encodeImage.m:91-97): from the next
+ * run on, the data given to vlhip_encode_plan_run have inDimension columns and
+ * every descriptor is projected, y = projection (x - center), and, when
+ * renormalize is set, scaled to unit L2 norm (floor 1e-12) before it is
+ * encoded; see vlhip_project.  projection (dimension x inDimension, row-major)
+ * and center (inDimension) are device arrays copied into the plan. */
+VLHIP_API void   vlhip_encode_plan_set_projection (void * plan, size_t inDimension, void const * projection,
+                                                   void const * center, int renormalize) ;
 /* chosen tiling (0: 8 waves x 32 components per workgroup, 1: 4 waves x 32),
  * the measured milliseconds of each candidate (-1 if not tried), batches */
 VLHIP_API void   vlhip_encode_plan_info (void * plan, int * variant, double * variantMs, size_t * numBatches) ;
 VLHIP_API void   vlhip_encode_plan_delete (void * plan) ;
+
+/* ---- pre-encoding projection (apps/recognition) ---------------------------- */
+
+/* PCA / whitening projection of float descriptors before encoding, as
+ * apps/recognition/encodeImage.m:91-97 applies it (learned by
+ * trainEncoder.m:150-178, whitening folded into the rows of projection):
+ * out[i] = projection (data[i] - center), then, when renormalize is set,
+ * out[i] * (1 / max(||out[i]||, 1e-12)).  projection is outDimension x
+ * inDimension (row-major), center has inDimension entries; FP32 products and
+ * sums (matrix cores), outDimension <= 128.  No counterpart in the C library
+ * of the reference, whose application does this in MATLAB. */
+VLHIP_API void vlhip_project (void * out, void const * data, size_t numData, size_t inDimension,
+                              size_t outDimension, void const * projection, void const * center, int renormalize) ;
 
 /* ---- VLAD (vl/vlad.c) ---------------------------------------------------- */
 

@@ -143,6 +143,11 @@ _EXT_SIGS = {
     "vlhip_encode_plan_run": (None, [_p, _p, _p, _p, _p, _p, _p, _p]),
     "vlhip_encode_plan_info": (None, [_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "vlhip_encode_plan_delete": (None, [_p]),
+    "vlhip_encode_plan_set_projection": (None, [_p, _sz, _p, _p, C.c_int]),
+    "vlhip_project": (None, [_p, _p, _sz, _sz, _sz, _p, _p, C.c_int]),
+    "vlhip_memcpy_d2d": (None, [_p, _p, C.c_size_t]),
+    "vlhip_kmeans_update_centers": (None, [C.c_int, _p, _p, _p, _p, _p, _sz, _sz]),
+    "vlhip_kmeans_ann_merge": (None, [C.c_int, _p, _p, _p, _p, _p, _sz, _sz, C.c_int]),
     "vlhip_gate_new": (_p, [_sz, _sz, _sz, _sz, C.c_ulonglong]),
     "vlhip_gate_delete": (None, [_p]),
     "vlhip_gate_num_params": (_sz, [_p]),

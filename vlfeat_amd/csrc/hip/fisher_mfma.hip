@@ -1335,6 +1335,12 @@ struct EncodePlan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   const void* key[7] = {};
+  // optional load stage: descriptors of Din dimensions projected to D
+  size_t Din = 0;
+  int renorm = 0;
+  float* dProjP = nullptr;  // D x Din, owned copy
+  float* dProjC = nullptr;  // Din, owned copy
+  float* dProjX = nullptr;  // totalRows x D, the projected descriptors
 };
 
 // autotuner cache: (D, K, flags, rows bucket) -> variant
@@ -1379,6 +1385,11 @@ static void plan_enqueue_variant(EncodePlan& P, int variant, const float* x, con
 
 static void plan_enqueue_any(EncodePlan& P, int variant, const float* x, const float* means, const float* covs,
                              const float* priors, float* enc, float* vladEnc, unsigned long long* numTerms) {
+  if (P.Din) {
+    // load stage: the encoders below read the projected descriptors
+    project_t(P.dProjX, x, P.totalRows, P.Din, P.D, P.dProjP, P.dProjC, P.renorm);
+    x = P.dProjX;
+  }
   switch (P.NS) {
     case 1: plan_enqueue_variant<_Float16, 1>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
     case 2: plan_enqueue_variant<_Float16, 2>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
@@ -1689,7 +1700,8 @@ void vlhip_encode_plan_delete(void* plan) {
   if (P->stream) VLHIP_CHECK(hipStreamSynchronize(P->stream));
   if (P->exec) VLHIP_CHECK(hipGraphExecDestroy(P->exec));
   if (P->graph) VLHIP_CHECK(hipGraphDestroy(P->graph));
-  void* bufs[] = {P->dOff, P->dGstart, P->dGtab, P->dRec, P->dTables, P->dNorm, P->dAssign};
+  void* bufs[] = {P->dOff, P->dGstart, P->dGtab, P->dRec, P->dTables, P->dNorm, P->dAssign,
+                  P->dProjP, P->dProjC, P->dProjX};
   for (void* b : bufs)
     if (b) VLHIP_CHECK(hipFree(b));
   if (P->stream) VLHIP_CHECK(hipStreamDestroy(P->stream));
@@ -1758,6 +1770,36 @@ void vlhip_encode_plan_run(void* plan, void const* data, void const* means, void
   ctx().stream = saved;
   VLHIP_CHECK(hipGraphLaunch(P->exec, P->stream));
   VLHIP_CHECK(hipStreamSynchronize(P->stream));
+}
+
+// Load stage of a plan: from the next run on, the data passed to
+// vlhip_encode_plan_run have inDimension columns and are projected (and
+// renormalised when asked) to the plan's dimension before encoding, as
+// apps/recognition/encodeImage.m:91-97 does before vl_fisher / vl_vlad.
+// projection (D x inDimension) and center (inDimension) are device arrays,
+// copied into the plan.
+void vlhip_encode_plan_set_projection(void* plan, size_t inDimension, void const* projection, void const* center,
+                                      int renormalize) {
+  std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+  EncodePlan* P = (EncodePlan*)plan;
+  VLHIP_REQUIRE(inDimension >= 1 && inDimension < (1u << 20), "projection inDimension out of range");
+  if (P->stream) VLHIP_CHECK(hipStreamSynchronize(P->stream));
+  if (P->exec) VLHIP_CHECK(hipGraphExecDestroy(P->exec));
+  if (P->graph) VLHIP_CHECK(hipGraphDestroy(P->graph));
+  P->exec = nullptr;
+  P->graph = nullptr;
+  for (float** b : {&P->dProjP, &P->dProjC}) {
+    if (*b) VLHIP_CHECK(hipFree(*b));
+    *b = nullptr;
+  }
+  VLHIP_CHECK(hipMalloc(&P->dProjP, P->D * inDimension * sizeof(float)));
+  VLHIP_CHECK(hipMalloc(&P->dProjC, inDimension * sizeof(float)));
+  if (!P->dProjX) VLHIP_CHECK(hipMalloc(&P->dProjX, (P->totalRows ? P->totalRows : 1) * P->D * sizeof(float)));
+  sync();  // inputs produced on the library stream are complete
+  VLHIP_CHECK(hipMemcpy(P->dProjP, projection, P->D * inDimension * sizeof(float), hipMemcpyDeviceToDevice));
+  VLHIP_CHECK(hipMemcpy(P->dProjC, center, inDimension * sizeof(float), hipMemcpyDeviceToDevice));
+  P->Din = inDimension;
+  P->renorm = renormalize ? 1 : 0;
 }
 
 void vlhip_encode_plan_info(void* plan, int* variant, double* variantMs, size_t* numBatches) {

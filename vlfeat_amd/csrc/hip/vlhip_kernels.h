@@ -64,4 +64,9 @@ void vlad_t(T* enc, const T* means, size_t D, size_t K, const T* x, const uint64
 template <typename T>
 void quantize_t(uint32_t* assign, T* dist, const T* x, const T* centers, size_t N, size_t D, size_t K, int lanes);
 
+// Pre-encoding projection (project_kernels.hip): out = P (x - c), each row
+// optionally renormalised to unit L2 norm; float, outDimension <= 128.
+void project_t(float* out, const float* x, size_t N, size_t Din, size_t D, const float* P, const float* c,
+               int renorm);
+
 }  // namespace vlhip
