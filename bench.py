@@ -151,6 +151,45 @@ def cpu_legs(x_host, offsets, m, c, p, budget_s: float, gpu_enc, gpu_fp32_enc, n
     return out
 
 
+def projection_leg(L, x, N: int, steps: int) -> dict:
+    """The pre-encoding projection (apps/recognition/encodeImage.m:91-97) at the
+    workload's size: the same N SIFT-like descriptors before their projection
+    (128-d, generated from the same seed), projected to 80-d by the workload's
+    fixed orthonormal matrix with vlhip_project (center 0, no renormalisation),
+    so the output must reproduce the encoder's input.  Reported per launch: time
+    (hipEvents on the library stream), algorithmic bytes (128 x 4 read + 80 x 4
+    written per descriptor) and flops (2 x 128 x 80 per descriptor) against the
+    HBM and FP32 matrix peaks."""
+    import torch
+    from vlfeat_amd import synth
+    raw = synth.sift_like_torch(N, 128, seed=1)
+    P = torch.as_tensor(synth.projection(DIM, 128, 1), dtype=torch.float32, device=raw.device)
+    cen = torch.zeros(128, dtype=torch.float32, device=raw.device)
+    out = torch.empty((N, DIM), dtype=torch.float32, device=raw.device)
+    torch.cuda.synchronize()
+    L.vlhip_project(out.data_ptr(), raw.data_ptr(), N, 128, DIM, P.data_ptr(), cen.data_ptr(), 0)
+    L.vlhip_synchronize()
+    L.vlhip_timing_reset()
+    L.vlhip_timing_enable(1)
+    for _ in range(steps):
+        L.vlhip_project(out.data_ptr(), raw.data_ptr(), N, 128, DIM, P.data_ptr(), cen.data_ptr(), 0)
+    L.vlhip_synchronize()
+    L.vlhip_timing_enable(0)
+    import vlfeat_amd as va
+    ms, cnt = va.kernel_timings()["project"]
+    avg = ms / max(cnt, 1)
+    rel = float(torch.linalg.norm(out - x) / torch.linalg.norm(x))
+    byts, flops = N * (128 + DIM) * 4, N * 2 * 128 * DIM
+    del raw, out
+    res = {"kernel": "project", "ms_per_launch": avg, "descriptors_per_launch": N,
+           "GB_per_s": byts / avg / 1e6, "hbm_frac": byts / avg / 1e6 / PEAK_HBM_GBS,
+           "TFLOP_per_s": flops / avg / 1e9, "fp32_frac": flops / avg / 1e9 / PEAK_FP32_TFLOPS,
+           "rel_l2_vs_encoder_input": rel}
+    log(f"projection load stage: {avg:.3f} ms per launch, {res['GB_per_s']:.0f} GB/s, "
+        f"{res['TFLOP_per_s']:.1f} TF/s, rel. diff. to the encoder's input {rel:.2e}")
+    return res
+
+
 def traffic_per_descriptor(kernel: str):
     """HBM bytes per descriptor of `kernel` from the committed PMC summary
     (FETCH_SIZE and WRITE_SIZE passes scaled by a copy-kernel calibration), if any."""
@@ -590,6 +629,7 @@ def main() -> int:
     ap.add_argument("--dropin-images", type=int, default=1000)
     ap.add_argument("--em-oracle-rows", type=int, default=65536)
     ap.add_argument("--em-cpu-rows", type=int, default=131072)
+    ap.add_argument("--no-project", action="store_true", help="skip the projection load-stage leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -741,6 +781,9 @@ def main() -> int:
                           "speedup_vs_fp32_gpu": (result["fp32_gpu"]["ms_per_step"] / msg) if "fp32_gpu" in result
                           else None})
         result["gated_gpu"] = gate_info
+
+    if not args.no_project:
+        result["projection_load_stage"] = projection_leg(L, x, N, args.steps)
 
     if not args.no_cpu and enc32 is not None:
         x_host = x[: min(N, 1024 * PER_IMAGE)].cpu().numpy()
