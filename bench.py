@@ -512,7 +512,8 @@ def dropin_config(args) -> int:
     caller of the reference uses it: vl_gmm_cluster on N = 1,000,000 SIFT-like
     descriptors projected to 80-d (seed 1), K = 256, 30 EM iterations from a
     seeded custom start (random data points, data variance, uniform priors);
-    then vl_fisher_encode (improved) and vl_vlad_encode (one-hot assignments from
+    vl_kmeans_cluster (Lloyd, 10 iterations) on the same data; then
+    vl_fisher_encode (improved) and vl_vlad_encode (one-hot assignments from
     vl_kmeans_quantize against the GMM means) of 1,000 images x 3,000 descriptors
     (seed 2).  The same calls run on the reference (oracle/_ref, OpenMP over the
     host cores) on the same inputs: wall-clock of both, and the parity of every
@@ -555,6 +556,15 @@ def dropin_config(args) -> int:
         a = lib.kmeans_quantize(km, xe)[0]
         out["quantize_s"] = time.perf_counter() - t0
         lib.vl_kmeans_delete(km)
+        # vl_kmeans_cluster (Lloyd, random-selection seeding, 10 iterations) on
+        # the EM data, as the GMM's k-means initialisation would run it
+        lib.seed(0)
+        km = lib.kmeans_new(np.float32, algorithm=va.VlKMeansLloyd, max_iter=10)
+        t0 = time.perf_counter()
+        ke = lib.vl_kmeans_cluster(km, x.ctypes.data, D, N, K)
+        out["kmeans_s"] = time.perf_counter() - t0
+        kc = lib.kmeans_centers(km, np.float32)
+        lib.vl_kmeans_delete(km)
         t0 = time.perf_counter()
         vl = []
         for i in range(n_img):
@@ -562,7 +572,7 @@ def dropin_config(args) -> int:
             oh[np.arange(per), a[i * per:(i + 1) * per]] = 1
             vl.append(lib.vlad(xe[i * per:(i + 1) * per], m, oh, 0))
         out["vlad_s"] = time.perf_counter() - t0
-        return out, em + (fv, a, np.stack(vl))
+        return out, em + (fv, a, np.stack(vl), kc, ke)
 
     R = reference() if not args.no_cpu else None
     r_t = r = None
@@ -570,9 +580,10 @@ def dropin_config(args) -> int:
         R.set_num_threads(0)
         r_t, r = run(R)
         log(f"dropin: reference ({int(R.vl_get_max_threads())} threads): EM {r_t['em_s']:.2f} s, "
+            f"k-means {r_t['kmeans_s']:.2f} s, "
             f"Fisher {r_t['fisher_s']:.2f} s, quantize {r_t['quantize_s']:.2f} s, VLAD {r_t['vlad_s']:.2f} s")
     g_t, g = run(L, model=r[:3] if r is not None else None)
-    log(f"dropin: GPU library: EM {g_t['em_s']:.2f} s, Fisher {g_t['fisher_s']:.2f} s, "
+    log(f"dropin: GPU library: EM {g_t['em_s']:.2f} s, k-means {g_t['kmeans_s']:.2f} s, Fisher {g_t['fisher_s']:.2f} s, "
         f"quantize {g_t['quantize_s']:.2f} s, VLAD {g_t['vlad_s']:.2f} s")
     result = {
         "metric": "FP32 drop-in at SIFT scale (config 2): host C API, GPU library vs reference",
@@ -594,7 +605,8 @@ def dropin_config(args) -> int:
                   "priors_rel_l2": rel(g[2], r[2]), "loglikelihood_rel": abs(g[3] - r[3]) / abs(r[3]),
                   "fisher_max_rel_l2": float(fv_err.max()), "fisher_median_rel_l2": float(np.median(fv_err)),
                   "assignments_equal": int((g[5] == r[5]).sum()), "assignments_total": int(g[5].size),
-                  "vlad_max_rel_l2": float(vl_err.max())}
+                  "vlad_max_rel_l2": float(vl_err.max()),
+                  "kmeans_centers_equal": bool(np.array_equal(g[7], r[7])), "kmeans_energy_equal": g[8] == r[8]}
         log("dropin: parity " + ", ".join(f"{k}={v:.3g}" for k, v in parity.items()))
         result["reference_s"] = r_t
         result["parity_vs_reference"] = parity
