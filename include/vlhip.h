@@ -42,6 +42,10 @@ VLHIP_API void   vlhip_free (void * ptr) ;
 VLHIP_API void   vlhip_memcpy_h2d (void * dst, void const * src_host, size_t bytes) ;
 VLHIP_API void   vlhip_memcpy_d2h (void * dst_host, void const * src, size_t bytes) ;
 VLHIP_API void   vlhip_memcpy_d2d (void * dst, void const * src, size_t bytes) ;  /* stream-ordered */
+/* Stream-ordered copies that return without waiting: the host buffer must stay
+ * valid (h2d) or untouched (d2h) until vlhip_synchronize or a synchronous call. */
+VLHIP_API void   vlhip_memcpy_h2d_async (void * dst, void const * src_host, size_t bytes) ;
+VLHIP_API void   vlhip_memcpy_d2h_async (void * dst_host, void const * src, size_t bytes) ;
 VLHIP_API void   vlhip_memset (void * dst, int value, size_t bytes) ;
 VLHIP_API void   vlhip_synchronize (void) ;
 VLHIP_API int    vlhip_num_compute_units (void) ;

@@ -1078,8 +1078,9 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
 
 // Bytes of group records materialised at once (batches of whole images).
 constexpr size_t kRecordBudget = size_t(8) << 30;
-// Rows of a batch on which the learned gate is tried before it runs on the rest.
-constexpr size_t kGateProbeRows = 262144;
+// Rows of a batch on which the learned gate is tried before it runs on the rest
+// (2048 groups of 32 descriptors decide whether it serves at least 1% of them).
+constexpr size_t kGateProbeRows = 65536;
 
 // Device tables of one encode, carved from one buffer:
 // [scales][g Dp][bias Kp][mup, isg, sinv Kp*Dp][Whi, Wlo Kp*2Dp].
@@ -1191,6 +1192,7 @@ static void run_mfma(float* enc, const float* means, const float* covs, const fl
   long long* gstart = (long long*)scratch(SLOT_GROUPS, (numImages + 1) * sizeof(long long));
   upload(gstart, gs.data(), (numImages + 1) * sizeof(long long));
   sync();  // gs is a local buffer
+  bool gateOff = false;  // a probe switched the gate off for the rest of this call
   size_t b0 = 0;
   while (b0 < numImages) {
     size_t b1 = b0 + 1;
@@ -1207,7 +1209,8 @@ static void run_mfma(float* enc, const float* means, const float* covs, const fl
       gtab = (int4*)scratch(SLOT_GROUPTAB, nG * sizeof(int4));
       VLHIP_LAUNCH("mfma_group_table", mfma_group_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                    offsets, gstart, (int)b0, n, gtab);
-      if (gu) {
+      if (gu && gateOff) ctx().gate_groups += nG;
+      if (gu && !gateOff) {
         // gate: candidates and confidence per row, then the groups it serves
         int* cand = (int*)scratch(SLOT_GATE, (size_t)rows * gu->M * sizeof(int));
         unsigned char* gb = (unsigned char*)scratch(SLOT_GATE2, al256(rows * sizeof(float)) + al256(rows) + al256(nG));
@@ -1235,6 +1238,7 @@ static void run_mfma(float* enc, const float* means, const float* covs, const fl
             }
           }
           if (inProbe >= 64 && confident * 100 < inProbe) {
+            gateOff = true;
             VLHIP_CHECK(hipMemsetAsync(conf + probe, 0, rows - probe, ctx().stream));
           } else {
             vlhip_gate_select(gu->gate, x + (rowBase + probe) * D, rows - probe, gu->M, gu->need,
@@ -1251,6 +1255,12 @@ static void run_mfma(float* enc, const float* means, const float* covs, const fl
         for (size_t i = 0; i < nG; ++i) served += hf[i];
         ctx().gate_groups += nG;
         ctx().gate_served += served;
+        if (served == 0) {
+          // every group of the batch is dense: the plain dense kernels, which give
+          // the same encodings without reading a gate flag per group
+          gflag = nullptr;
+          list = nullptr;
+        }
       }
     }
     mfma_batch_launch<H, NS, CT, WAVES>(t, x + rowBase * D, (int)D, (int)K, Kp, flags, rec, nG, gtab, offsets, gstart,

@@ -171,6 +171,19 @@ void vlhip_memcpy_d2h(void* dst, void const* src, size_t bytes) {
   download(dst, src, bytes);
 }
 
+void vlhip_memcpy_h2d_async(void* dst, void const* src, size_t bytes) {
+  ensure_init();
+  std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+  upload(dst, src, bytes);
+}
+
+void vlhip_memcpy_d2h_async(void* dst, void const* src, size_t bytes) {
+  ensure_init();
+  std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+  if (bytes == 0) return;
+  VLHIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx().stream));
+}
+
 void vlhip_memcpy_d2d(void* dst, void const* src, size_t bytes) {
   ensure_init();
   std::lock_guard<std::recursive_mutex> lk(ctx().mu);

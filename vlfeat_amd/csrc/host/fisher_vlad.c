@@ -68,20 +68,24 @@ fisher_run (void * enc, vl_type dataType, void const * means, vl_size dimension,
   dEnc = vlhip_buffer(VLHIP_BUF_ENC, es * 2 * KD * numImages) ;
   dOff = (vl_uint64*) vlhip_buffer(VLHIP_BUF_AUX, sizeof(vl_uint64) * (numImages + 1)) ;
   dTerms = (unsigned long long*) vlhip_buffer(VLHIP_BUF_AUX2, sizeof(unsigned long long) * numImages) ;
-  vlhip_memcpy_h2d(params, means, es * KD) ;
-  vlhip_memcpy_h2d(params + es * KD, covariances, es * KD) ;
-  vlhip_memcpy_h2d(params + 2 * es * KD, priors, es * numClusters) ;
-  if (numData) vlhip_memcpy_h2d(dData, data, es * dimension * numData) ;
-  vlhip_memcpy_h2d(dOff, offsets, sizeof(vl_uint64) * (numImages + 1)) ;
+  /* stream-ordered copies; the caller's arrays stay valid for the whole call,
+   * and the last download waits for everything before it */
+  vlhip_memcpy_h2d_async(params, means, es * KD) ;
+  vlhip_memcpy_h2d_async(params + es * KD, covariances, es * KD) ;
+  vlhip_memcpy_h2d_async(params + 2 * es * KD, priors, es * numClusters) ;
+  if (numData) vlhip_memcpy_h2d_async(dData, data, es * dimension * numData) ;
+  vlhip_memcpy_h2d_async(dOff, offsets, sizeof(vl_uint64) * (numImages + 1)) ;
   vlhip_fisher_encode_batch((int)dataType, dEnc, params, params + es * KD, params + 2 * es * KD,
                             dimension, numClusters, dData, (uint64_t const*)dOff, numImages, flags,
                             dataType == VL_TYPE_FLOAT ? g_precision : VLHIP_MODE_FP32, dTerms) ;
-  vlhip_memcpy_d2h(enc, dEnc, es * 2 * KD * numImages) ;
   if (numTermsOut) {
     unsigned long long * h = (unsigned long long*) malloc(sizeof(unsigned long long) * numImages) ;
+    vlhip_memcpy_d2h_async(enc, dEnc, es * 2 * KD * numImages) ;
     vlhip_memcpy_d2h(h, dTerms, sizeof(unsigned long long) * numImages) ;
     for (i = 0 ; i < numImages ; ++i) { numTermsOut[i] = h[i] ; total += h[i] ; }
     free(h) ;
+  } else {
+    vlhip_memcpy_d2h(enc, dEnc, es * 2 * KD * numImages) ;
   }
   vlhip_unlock() ;
   return total ;
@@ -187,13 +191,15 @@ vlad_run (void * enc, vl_type dataType, void const * means, vl_size dimension, v
   dEnc = vlhip_buffer(VLHIP_BUF_ENC, es * KD) ;
   dOff = (vl_uint64*) vlhip_buffer(VLHIP_BUF_AUX, sizeof(offsets)) ;
   dAssign = vlhip_buffer(VLHIP_BUF_POST, (soft ? es * numClusters : sizeof(vl_uint32)) * (numData ? numData : 1)) ;
-  vlhip_memcpy_h2d(dMeans, means, es * KD) ;
+  /* stream-ordered copies (offsets and indices stay valid until the final
+   * download, which waits for everything before it) */
+  vlhip_memcpy_h2d_async(dMeans, means, es * KD) ;
   if (numData) {
-    vlhip_memcpy_h2d(dData, data, es * dimension * numData) ;
-    if (soft) vlhip_memcpy_h2d(dAssign, soft, es * numClusters * numData) ;
-    else vlhip_memcpy_h2d(dAssign, hard, sizeof(vl_uint32) * numData) ;
+    vlhip_memcpy_h2d_async(dData, data, es * dimension * numData) ;
+    if (soft) vlhip_memcpy_h2d_async(dAssign, soft, es * numClusters * numData) ;
+    else vlhip_memcpy_h2d_async(dAssign, hard, sizeof(vl_uint32) * numData) ;
   }
-  vlhip_memcpy_h2d(dOff, offsets, sizeof(offsets)) ;
+  vlhip_memcpy_h2d_async(dOff, offsets, sizeof(offsets)) ;
   vlhip_vlad_encode_batch((int)dataType, dEnc, dMeans, dimension, numClusters, dData, (uint64_t const*)dOff, 1,
                           soft ? dAssign : NULL, soft ? NULL : (uint32_t const*)dAssign, flags) ;
   vlhip_memcpy_d2h(enc, dEnc, es * KD) ;
