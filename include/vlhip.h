@@ -182,6 +182,16 @@ VLHIP_API void   vlhip_encode_plan_delete (void * plan) ;
 VLHIP_API void vlhip_project (void * out, void const * data, size_t numData, size_t inDimension,
                               size_t outDimension, void const * projection, void const * center, int renormalize) ;
 
+/* ---- homogeneous kernel map (vl/homkermap.c) -------------------------------- */
+
+/* The map of every value of in (n values, dtype) into out (n x (2*order+1)
+ * values, row i for in[i]), homkermap.c:532-571: table holds the map object's
+ * rows of 2*order+1 doubles for numSubdivisions mantissas of every exponent
+ * minExponent .. maxExponent (vl_homogeneouskernelmap_new).  Bitwise the
+ * reference's vl_homogeneouskernelmap_evaluate_{f,d}. */
+VLHIP_API void vlhip_homkermap_evaluate (int dtype, void * out, void const * in, size_t n, double const * table,
+                                         int order, int numSubdivisions, int minExponent, int maxExponent) ;
+
 /* ---- VLAD (vl/vlad.c) ---------------------------------------------------- */
 
 /* VLAD of numImages images, vlad.c:163-242 per image.  Exactly one of

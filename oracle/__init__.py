@@ -66,6 +66,7 @@ class Oracle:
             "or_vlad_encode": (None, [_p, _p, _sz, _sz, _p, _sz, _p, C.c_int]),
             "or_kmeans_quantize": (None, [_p, _p, _p, _p, _sz, _sz, _sz]),
             "or_mlp_loss_grads": (C.c_double, [_p, _sz, _sz, _sz, _sz, _p, _p, _sz, _p]),
+            "or_homkermap": (None, [_p, _p, _sz, C.c_int, C.c_double, _sz, C.c_double, C.c_int]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -123,6 +124,13 @@ class Oracle:
         loss = self.L.or_mlp_loss_grads(params.ctypes.data, D, H1, H2, K, xin.ctypes.data, q.ctypes.data,
                                         xin.shape[0], None if g is None else g.ctypes.data)
         return loss, g
+
+    def homkermap(self, x, kernel=1, gamma=1.0, order=1, period=-1.0, window=0):
+        """Homogeneous kernel map of every value of x: x.shape + (2*order+1,), FP64."""
+        xf = _f64(x).ravel()
+        out = np.empty((xf.size, 2 * order + 1))
+        self.L.or_homkermap(out.ctypes.data, xf.ctypes.data, xf.size, kernel, gamma, order, period, window)
+        return out.reshape(np.shape(x) + (2 * order + 1,))
 
     def quantize(self, x, centers):
         x, centers = _f64(x), _f64(centers)

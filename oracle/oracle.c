@@ -1,5 +1,6 @@
-/* oracle.c -- FP64 CPU restatement of VLFeat 0.9.21's GMM-EM, Fisher, VLAD
- * and exact k-means quantizer, used ONLY as the checker of the GPU library
+/* oracle.c -- FP64 CPU restatement of VLFeat 0.9.21's GMM-EM, Fisher, VLAD,
+ * exact k-means quantizer and homogeneous kernel map, used ONLY as the
+ * checker of the GPU library
  * (tests, smoke test and the CPU leg of the benchmark).  Nothing in the
  * product links or calls it.
  *
@@ -401,4 +402,103 @@ or_mlp_loss_grads (double const * params, size_t D, size_t H1, size_t H2, size_t
   }
   free(a1) ; free(h1) ; free(a2) ; free(h2) ; free(z) ; free(dz) ; free(dh2) ; free(dh1) ;
   return rows ? loss / (double)rows : 0.0 ;
+}
+
+/* ---- homogeneous kernel map (homkermap.c:239-426, 532-571) ----------------
+ * or_homkermap maps n doubles into n x (2*order+1) doubles: the reference's
+ * table of the windowed, periodised spectrum (kernel 0 intersection, 1 chi2,
+ * 2 Jensen-Shannon; window 0 uniform, 1 rectangular; period < 0: default),
+ * then linear interpolation between the rows bracketing each mantissa. */
+/* separate sin and cos calls, as in the reference's build (a merged sincos
+ * call can differ in the last bit) */
+static double (*volatile or_sin) (double) = sin ;
+static double (*volatile or_cos) (double) = cos ;
+
+static double
+or_hkm_spectrum (int kernel, double w)
+{
+  double sech = 2.0 / (exp(kPi * w) + exp(-kPi * w)) ;
+  if (kernel == 0) return (2.0 / kPi) / (1 + 4 * w * w) ;
+  if (kernel == 1) return sech ;
+  return (2.0 / log(4.0)) * 2.0 / (exp(kPi * w) + exp(-kPi * w)) / (1 + 4 * w * w) ;
+}
+
+static double
+or_hkm_window (int kernel, int window, double period, double w)
+{
+  double range, step, acc = 0, u ;
+  if (window == 0) return or_hkm_spectrum(kernel, w) ;
+  range = 2.0 / (period * 1e-2) ;
+  step = 2 * range / (2 * 1024.0 + 1) ;
+  for (u = -range ; u <= range ; u += step) {
+    double a = (period / 2.0) * u ;
+    double s = (a == 0.0) ? 1.0 : or_sin(a) / a ;
+    s *= (period / (2.0 * kPi)) ;
+    acc += s * or_hkm_spectrum(kernel, u + w) ;
+  }
+  acc *= step ;
+  return acc > 0.0 ? acc : 0.0 ;
+}
+
+OR_API void
+or_homkermap (double * out, double const * x, size_t n, int kernel, double gamma, size_t order,
+              double period, int window)
+{
+  size_t const F = 2 * order + 1, S = 8 + 8 * order ;
+  int const e0 = -20, e1 = 8 ;
+  double const sub = 1.0 / S ;
+  double * tab = (double*) malloc(sizeof(double) * F * S * (e1 - e0 + 1)) ;
+  double * kap = (double*) malloc(sizeof(double) * (order + 1)) ;
+  double * frq = (double*) malloc(sizeof(double) * (order + 1)) ;
+  double L, m, *row ;
+  size_t i, j, s ;
+  int e ;
+  if (period < 0) {
+    if (window == 0) period = kernel == 1 ? 5.86 * sqrt(order + 0) + 3.65
+                            : kernel == 2 ? 6.64 * sqrt(order + 0) + 7.24 : 2.38 * log(order + 0.8) + 5.6 ;
+    else period = kernel == 1 ? 8.80 * sqrt(order + 4.44) - 12.6
+                : kernel == 2 ? 9.63 * sqrt(order + 1.00) - 2.93 : 2.00 * log(order + 0.99) + 3.52 ;
+    if (period < 1.0) period = 1.0 ;
+  }
+  L = 2.0 * kPi / period ;
+  for (i = 0, j = 0 ; i <= order ; ) {
+    frq[i] = (double) j ;
+    kap[i] = or_hkm_window(kernel, window, period, j * L) ;
+    ++ j ;
+    if (kap[i] > 0 || j >= 3 * i) ++ i ;
+  }
+  row = tab ;
+  for (e = e0 ; e <= e1 ; ++e) {
+    for (s = 0, m = 1.0 ; s < S ; ++s, m += sub) {
+      double xv = ldexp(m, e), lx = L * pow(xv, gamma), ll = L * log(xv) ;
+      *row++ = sqrt(lx * kap[0]) ;
+      for (j = 1 ; j <= order ; ++j) {
+        double a = sqrt(2.0 * lx * kap[j]) ;
+        *row++ = a * or_cos(frq[j] * ll) ;
+        *row++ = a * or_sin(frq[j] * ll) ;
+      }
+    }
+  }
+  for (i = 0 ; i < n ; ++i) {
+    double * o = out + i * F ;
+    double const * lo, * hi ;
+    double sg ;
+    int ex ;
+    m = frexp(x[i], &ex) ;
+    sg = (m >= 0.0) ? 1.0 : -1.0 ;
+    m *= 2 * sg ;
+    ex -- ;
+    if (m == 0 || ex <= e0 || ex >= e1) {
+      for (j = 0 ; j < F ; ++j) o[j] = 0.0 ;
+      continue ;
+    }
+    lo = tab + (size_t)(ex - e0) * S * F ;
+    m -= 1.0 ;
+    while (m >= sub) { m -= sub ; lo += F ; }
+    hi = lo + F ;
+    for (j = 0 ; j < F ; ++j) o[j] = sg * ((hi[j] - lo[j]) * (S * m) + lo[j]) ;
+  }
+  free(tab) ;
+  free(kap) ;
+  free(frq) ;
 }

@@ -1,0 +1,54 @@
+"""Homogeneous kernel map of whole arrays on the GPU
+(vl_homogeneouskernelmap_evaluate_array -> vlhip_homkermap_evaluate) against
+the reference's scalar entry points, compiled from source: bitwise equal for
+float and double data, three kernels, both windows, orders 1 and 3, gamma 1
+and 0.5, on values spanning the table's exponents, negatives, zeros and values
+outside the table; ragged and empty arrays."""
+import numpy as np
+import pytest
+
+import vlfeat_amd as va
+from oracle import oracle, reference
+
+pytestmark = pytest.mark.gpu
+needs_ref = pytest.mark.skipif(reference() is None, reason="reference library not built")
+
+
+def sample(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.lognormal(-2, 2.5, n)
+    x[::7] *= -1
+    x[::11] = 0.0
+    x[:8] = [1e-7, 3e-6, 255.9, 256.0, 300.0, 1e4, 2.0 ** -20, 2.0 ** -19]
+    return x
+
+
+@needs_ref
+@pytest.mark.parametrize("kernel", [0, 1, 2])
+@pytest.mark.parametrize("window", [0, 1])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_array_map_matches_reference(kernel, window, dtype):
+    R, P = reference(), va.lib()
+    x = sample(3001, kernel * 2 + window).astype(dtype)
+    for order, gamma in ((1, 1.0), (3, 0.5)):
+        ref = R.homkermap(x, kernel, gamma, order, -1.0, window)
+        got = P.homkermap(x, kernel, gamma, order, -1.0, window, array=True)
+        assert got.dtype == dtype and got.shape == x.shape + (2 * order + 1,)
+        assert np.array_equal(got, ref), (order, gamma)
+
+
+def test_array_map_of_encodings_and_edge_sizes():
+    # signed-square-rooted Fisher-vector-like values (as the recognition
+    # application maps its encodings, traintest.m:99: order 1, chi2,
+    # rectangular window): GPU array path == FP64 oracle rounded to float
+    P, O = va.lib(), oracle()
+    rng = np.random.default_rng(5)
+    enc = (np.sign(rng.standard_normal((64, 2048))) * np.sqrt(np.abs(rng.standard_normal((64, 2048))) * 1e-2))
+    enc = enc.astype(np.float32)
+    got = P.homkermap(enc, 1, 1.0, 1, -1.0, 1, array=True)
+    assert np.array_equal(got, O.homkermap(enc.astype(np.float64), 1, 1.0, 1, -1.0, 1).astype(np.float32))
+    for n in (0, 1, 255, 257):
+        v = enc.ravel()[:n]
+        out = P.homkermap(v, 1, 1.0, 2, -1.0, 0, array=True)
+        assert out.shape == (n, 5)
+        assert np.array_equal(out, O.homkermap(v.astype(np.float64), 1, 1.0, 2, -1.0, 0).astype(np.float32))

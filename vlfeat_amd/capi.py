@@ -103,6 +103,15 @@ _SIGS = {
     "vl_kmeans_init_centers_with_rand_data": (None, [_p, _p, _sz, _sz, _sz]),
     "vl_kmeans_init_centers_plus_plus": (None, [_p, _p, _sz, _sz, _sz]),
     "vl_kmeans_refine_centers": (C.c_double, [_p, _p, _sz]),
+    # homkermap.h
+    "vl_homogeneouskernelmap_new": (_p, [C.c_int, C.c_double, _sz, C.c_double, C.c_int]),
+    "vl_homogeneouskernelmap_delete": (None, [_p]),
+    "vl_homogeneouskernelmap_evaluate_d": (None, [_p, _p, _sz, C.c_double]),
+    "vl_homogeneouskernelmap_evaluate_f": (None, [_p, _p, _sz, C.c_double]),
+    "vl_homogeneouskernelmap_get_order": (_sz, [_p]),
+    "vl_homogeneouskernelmap_get_dimension": (_sz, [_p]),
+    "vl_homogeneouskernelmap_get_kernel_type": (C.c_int, [_p]),
+    "vl_homogeneouskernelmap_get_window_type": (C.c_int, [_p]),
 }
 
 # Entry points present only in this library (batched forms and the device ABI).
@@ -144,6 +153,10 @@ _EXT_SIGS = {
     "vlhip_encode_plan_info": (None, [_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "vlhip_encode_plan_delete": (None, [_p]),
     "vlhip_encode_plan_set_projection": (None, [_p, _sz, _p, _p, C.c_int]),
+    "vl_homogeneouskernelmap_evaluate_array": (None, [_p, C.c_uint32, _p, _p, _sz]),
+    "vlhip_homkermap_evaluate": (None, [C.c_int, _p, _p, _sz, _p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "vlhip_memcpy_h2d_async": (None, [_p, _p, C.c_size_t]),
+    "vlhip_memcpy_d2h_async": (None, [_p, _p, C.c_size_t]),
     "vlhip_project": (None, [_p, _p, _sz, _sz, _sz, _p, _p, C.c_int]),
     "vlhip_memcpy_d2d": (None, [_p, _p, C.c_size_t]),
     "vlhip_kmeans_update_centers": (None, [C.c_int, _p, _p, _p, _p, _p, _sz, _sz]),
@@ -282,6 +295,29 @@ class CLib:
         s = C.cast(km, C.POINTER(VlKMeans)).contents
         ct = np.ctypeslib.as_ctypes_type(np.dtype(dtype))
         return np.ctypeslib.as_array(C.cast(s.centers, C.POINTER(ct)), shape=(s.numCenters, s.dimension)).copy()
+
+    def homkermap(self, data, kernel=1, gamma=1.0, order=1, period=-1.0, window=0, array=False):
+        """Map every value of data (float32 or float64) with a homogeneous kernel
+        map; returns data.shape + (2*order+1,).  array=True uses the array entry
+        point (this library only), otherwise the scalar one per value."""
+        d = np.ascontiguousarray(data)
+        h = self.vl_homogeneouskernelmap_new(kernel, gamma, order, period, window)
+        try:
+            F = int(self.vl_homogeneouskernelmap_get_dimension(h))
+            out = np.empty(d.shape + (F,), dtype=d.dtype)
+            flat = d.ravel()
+            if array:
+                self.vl_homogeneouskernelmap_evaluate_array(h, dtype_code(d), out.ctypes.data, flat.ctypes.data,
+                                                            flat.size)
+            else:
+                fn = self.vl_homogeneouskernelmap_evaluate_f if d.dtype == np.float32 else \
+                    self.vl_homogeneouskernelmap_evaluate_d
+                base = out.ctypes.data
+                for i, v in enumerate(flat.tolist()):
+                    fn(h, base + i * F * d.itemsize, 1, v)
+            return out
+        finally:
+            self.vl_homogeneouskernelmap_delete(h)
 
     def kmeans_quantize(self, km, data):
         N = data.shape[0]
