@@ -515,7 +515,8 @@ def dropin_config(args) -> int:
     descriptors projected to 80-d (seed 1), K = 256, 30 EM iterations from a
     seeded custom start (random data points, data variance, uniform priors);
     vl_kmeans_cluster (Lloyd, 10 iterations) on the same data; then
-    vl_fisher_encode (improved) and vl_vlad_encode (one-hot assignments from
+    vl_fisher_encode (improved), the homogeneous kernel map of the encodings,
+    and vl_vlad_encode (one-hot assignments from
     vl_kmeans_quantize against the GMM means) of 1,000 images x 3,000 descriptors
     (seed 2).  The same calls run on the reference (oracle/_ref, OpenMP over the
     host cores) on the same inputs: wall-clock of both, and the parity of every
@@ -587,6 +588,37 @@ def dropin_config(args) -> int:
     g_t, g = run(L, model=r[:3] if r is not None else None)
     log(f"dropin: GPU library: EM {g_t['em_s']:.2f} s, k-means {g_t['kmeans_s']:.2f} s, Fisher {g_t['fisher_s']:.2f} s, "
         f"quantize {g_t['quantize_s']:.2f} s, VLAD {g_t['vlad_s']:.2f} s")
+    # homogeneous kernel map of the encodings, as the recognition application
+    # applies it before its linear SVM (traintest.m:99: order 1, chi2, the
+    # MEX front-end's rectangular window): the library's array form (host
+    # arrays, PCIe included) vs the reference's scalar entry point looped over
+    # the values in C (toolbox/misc/vl_homkermap.c:170-184, one thread)
+    fvs = np.ascontiguousarray(g[4], np.float32)
+    L.homkermap(fvs, 1, 1.0, 1, -1.0, 1, array=True)  # warm-up: buffers, table, first launch
+    L.vlhip_timing_reset()
+    L.vlhip_timing_enable(1)
+    t0 = time.perf_counter()
+    hk = L.homkermap(fvs, 1, 1.0, 1, -1.0, 1, array=True)
+    hk_s = time.perf_counter() - t0
+    L.vlhip_timing_enable(0)
+    hk_ms, _ = va.kernel_timings().get("homkermap", (float("nan"), 0))
+    n_hk = fvs.size
+    homk = {"values": n_hk, "gpu_library_s": hk_s, "kernel_ms": hk_ms,
+            "kernel_GB_per_s": n_hk * 16 / hk_ms / 1e6 if hk_ms == hk_ms else None}
+    if R is not None:
+        import ctypes as C
+        f = R.lib.ref_homkermap_array_f
+        f.restype, f.argtypes = None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+        hmap = R.vl_homogeneouskernelmap_new(1, 1.0, 1, -1.0, 1)
+        hr = np.empty_like(hk)
+        t0 = time.perf_counter()
+        f(hmap, hr.ctypes.data, fvs.ctypes.data, n_hk)
+        homk["reference_s"] = time.perf_counter() - t0
+        R.vl_homogeneouskernelmap_delete(hmap)
+        homk["bit_identical"] = bool(np.array_equal(hk, hr))
+    log("dropin: homogeneous kernel map of the encodings: " + ", ".join(f"{k}={v}" for k, v in homk.items()))
+    del hk
+
     result = {
         "metric": "FP32 drop-in at SIFT scale (config 2): host C API, GPU library vs reference",
         "value": n_img * per / (g_t["fisher_s"] + g_t["vlad_s"] + g_t["quantize_s"]),
@@ -597,6 +629,7 @@ def dropin_config(args) -> int:
         "config": {"workload": "vl_gmm_cluster N=1M K=256 D=80 30 iterations; improved FV + VLAD of "
                                f"{n_img} images x {per}", "N": N, "K": K, "D": D, "images": n_img},
         "gpu_library_s": g_t,
+        "homkermap": homk,
     }
     if R is not None:
         rel = lambda a, b: float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(np.asarray(b, np.float64)))

@@ -30,7 +30,7 @@ def sample(n, seed):
 def test_array_map_matches_reference(kernel, window, dtype):
     R, P = reference(), va.lib()
     x = sample(3001, kernel * 2 + window).astype(dtype)
-    for order, gamma in ((1, 1.0), (3, 0.5)):
+    for order, gamma in ((1, 1.0), (3, 0.5), (6, 1.0)):
         ref = R.homkermap(x, kernel, gamma, order, -1.0, window)
         got = P.homkermap(x, kernel, gamma, order, -1.0, window, array=True)
         assert got.dtype == dtype and got.shape == x.shape + (2 * order + 1,)

@@ -14,14 +14,17 @@
 
 namespace vlhip {
 
-template <typename T>
+// ORDER > 0: the order is known at compile time and the 2F table reads of a
+// value are issued together; ORDER == 0: any order (runtime).
+template <typename T, int ORDER>
 __global__ __launch_bounds__(256) void homkermap_kernel(const T* __restrict__ in, long long n,
                                                         const double* __restrict__ table, int order,
-                                                        int numSub, int minExp, int maxExp, T* __restrict__ out) {
+                                                        int numSub, double sub, int minExp, int maxExp,
+                                                        T* __restrict__ out) {
 #pragma clang fp contract(off)
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int F = 2 * order + 1;
+  const int F = ORDER > 0 ? 2 * ORDER + 1 : 2 * order + 1;
   T* dst = out + i * F;
   int exponent;
   double mantissa = frexp((double)in[i], &exponent);
@@ -32,7 +35,6 @@ __global__ __launch_bounds__(256) void homkermap_kernel(const T* __restrict__ in
     for (int j = 0; j < F; ++j) dst[j] = T(0);
     return;
   }
-  const double sub = 1.0 / (double)numSub;
   const double* lo = table + (size_t)(exponent - minExp) * (size_t)numSub * F;
   mantissa -= 1.0;
   while (mantissa >= sub) {
@@ -41,7 +43,33 @@ __global__ __launch_bounds__(256) void homkermap_kernel(const T* __restrict__ in
   }
   const double* hi = lo + F;
   const double scale = (double)numSub * mantissa;
-  for (int j = 0; j < F; ++j) dst[j] = (T)((T)sign * ((hi[j] - lo[j]) * scale + lo[j]));
+  if constexpr (ORDER > 0) {
+    double l[2 * ORDER + 1], h[2 * ORDER + 1];
+#pragma unroll
+    for (int j = 0; j < 2 * ORDER + 1; ++j) { l[j] = lo[j]; h[j] = hi[j]; }
+#pragma unroll
+    for (int j = 0; j < 2 * ORDER + 1; ++j) dst[j] = (T)((T)sign * ((h[j] - l[j]) * scale + l[j]));
+  } else {
+    for (int j = 0; j < F; ++j) dst[j] = (T)((T)sign * ((hi[j] - lo[j]) * scale + lo[j]));
+  }
+}
+
+template <typename T>
+static void homkermap_launch(const T* in, size_t n, const double* table, int order, int numSub, double sub, int minExp,
+                             int maxExp, T* out) {
+  const dim3 grid((unsigned)((n + 255) / 256));
+  switch (order) {
+#define HK_CASE(O)                                                                                              \
+  case O:                                                                                                       \
+    VLHIP_LAUNCH("homkermap", (homkermap_kernel<T, O>), grid, dim3(256), 0, in, (long long)n, table, order,      \
+                 numSub, sub, minExp, maxExp, out);                                                             \
+    break;
+    HK_CASE(1) HK_CASE(2) HK_CASE(3) HK_CASE(4)
+#undef HK_CASE
+    default:
+      VLHIP_LAUNCH("homkermap", (homkermap_kernel<T, 0>), grid, dim3(256), 0, in, (long long)n, table, order, numSub,
+                   sub, minExp, maxExp, out);
+  }
 }
 
 }  // namespace vlhip
@@ -56,13 +84,13 @@ void vlhip_homkermap_evaluate(int dtype, void* out, void const* in, size_t n, do
   std::lock_guard<std::recursive_mutex> lk(ctx().mu);
   VLHIP_REQUIRE(order >= 0 && numSubdivisions >= 1 && minExponent < maxExponent, "invalid homogeneous kernel map");
   if (n == 0) return;
-  const dim3 grid((unsigned)((n + 255) / 256));
+  const double sub = 1.0 / (double)numSubdivisions;  // the map's subdivision, as the reference computes it
   if (dtype == 1)
-    VLHIP_LAUNCH("homkermap", homkermap_kernel<float>, grid, dim3(256), 0, (const float*)in, (long long)n, table,
-                 order, numSubdivisions, minExponent, maxExponent, (float*)out);
+    homkermap_launch<float>((const float*)in, n, table, order, numSubdivisions, sub, minExponent, maxExponent,
+                            (float*)out);
   else if (dtype == 2)
-    VLHIP_LAUNCH("homkermap", homkermap_kernel<double>, grid, dim3(256), 0, (const double*)in, (long long)n, table,
-                 order, numSubdivisions, minExponent, maxExponent, (double*)out);
+    homkermap_launch<double>((const double*)in, n, table, order, numSubdivisions, sub, minExponent, maxExponent,
+                             (double*)out);
   else
     fatal("vlhip_homkermap_evaluate", __FILE__, __LINE__, "unknown data type");
 }
