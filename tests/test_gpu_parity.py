@@ -257,6 +257,79 @@ def test_vlad_known_answers():
     assert np.array_equal(va.vl_vlad(np.concatenate([x, x]), mu, np.concatenate([eye, eye]), 4), [2, 2, 2])
 
 
+@pytest.mark.parametrize("n,D,K,dt", [(5000, 80, 256, np.float32), (4500, 13, 37, np.float32),
+                                      (2049, 128, 300, np.float64), (6200, 96, 1024, np.float32),
+                                      (1, 7, 3, np.float32)])
+def test_vlad_hard_chunks_blocks_and_norm_buffers(n, D, K, dt):
+    # rows beyond one staged chunk of cluster indices (2048), cluster blocks
+    # of 4..32 (chosen from the number of workgroups), encodings longer than
+    # one LDS buffer of the global norm with a ragged tail (K*D not a multiple
+    # of the 16-byte reads): bitwise the reference's dense loop
+    rng = np.random.default_rng(n + D + K)
+    x = rng.standard_normal((n, D)).astype(dt)
+    mu = rng.standard_normal((K, D)).astype(dt)
+    a = rng.integers(0, K, n).astype(np.uint32)
+    a[::53] = 0xFFFFFFFF
+    dense = np.zeros((n, K), dt)
+    rows = np.nonzero(a != 0xFFFFFFFF)[0]
+    dense[rows, a[rows]] = 1
+    R = reference()
+    for flags in (0, 1, 3, 8, 15):
+        enc = va.vl_vlad_hard(x, mu, a, flags)
+        if R is not None:
+            assert np.array_equal(enc, R.vlad(x, mu, dense, flags)), flags
+        # float sums of a few rows minus mass * mean cancel: 5e-5 from FP64
+        assert rel(enc, oracle().vlad(x, mu, dense, flags)) < (5e-5 if dt == np.float32 else 1e-12)
+
+
+def _f32_global_norm(z):
+    """vlad.c:230-241 in float32, term by term: n = n + z*z, sqrt, max 1e-12."""
+    n = np.float32(0)
+    for v in z.astype(np.float32):
+        n = np.float32(n + np.float32(v * v))
+    n = np.float32(np.sqrt(n))
+    n = n if n > 1e-12 else np.float32(1e-12)
+    return (z.astype(np.float32) / n).astype(np.float32)
+
+
+@pytest.mark.parametrize("case", ["ties", "subnormal", "overflow", "nan", "zeros", "mixed"])
+def test_vlad_global_norm_exact_sum(case):
+    # one descriptor per cluster, zero means, flags 0: the encoding before the
+    # global normalisation is the data itself, so the squared norm's float
+    # sum sees chosen terms: exact ties (2^-24 on top of 1), subnormal squares
+    # and a start below the normal range, squares that overflow to inf, a
+    # NaN, long runs of zeros, and sums over many powers of two
+    K, D = 96, 64
+    rng = np.random.default_rng(len(case))
+    z = rng.standard_normal(K * D).astype(np.float32)
+    if case == "ties":
+        z[:] = np.float32(2.0 ** -12)
+        z[::7] = 1.0
+        z[::13] = np.float32(3 * 2.0 ** -13)
+        z[0] = 1.0
+    elif case == "subnormal":
+        z[: K * D // 2] = (rng.uniform(0.5, 2.0, K * D // 2) * 1e-21).astype(np.float32)
+        z[K * D // 2::5] = np.float32(1e-19)
+    elif case == "overflow":
+        z[100] = np.float32(1.5e19)
+        z[200] = np.float32(1.5e19)
+    elif case == "nan":
+        z[3000] = np.nan
+    elif case == "zeros":
+        z[:2000] = 0
+        z[5000:] = 0
+    else:
+        z *= np.exp2(rng.integers(-30, 30, K * D)).astype(np.float32)
+    x = z.reshape(K, D)
+    mu = np.zeros((K, D), np.float32)
+    a = np.arange(K, dtype=np.uint32)
+    enc = va.vl_vlad_hard(x, mu, a, 0)
+    np.testing.assert_array_equal(enc, _f32_global_norm(z))
+    R = reference()
+    if R is not None:
+        np.testing.assert_array_equal(enc, R.vlad(x, mu, np.eye(K, dtype=np.float32), 0))
+
+
 def test_vlad_hard_equals_one_hot_and_double(cfg2):
     x, m, c, p = cfg2
     a, _ = va.vl_kmeans_quantize(x, m)

@@ -136,64 +136,264 @@ __global__ __launch_bounds__(256) void vlad_accum_kernel(
 }
 
 // vlad.c:230-241: enc /= max(||enc||, 1e-12), the squared norm summed in
-// index order by one lane (bitwise the reference's sequential loop).  The
-// workgroup stages the squares z*z (each rounded as the reference rounds it)
-// through LDS in chunks, so that the lane's only work is the chain of
-// dependent additions, fed by 16-byte LDS reads.
-constexpr int VL_NCHUNK = 4096;
+// index order (bitwise the reference's sequential loop).  The squares z*z
+// (each rounded as the reference rounds it) pass through two LDS buffers:
+// while wave 0 adds up one buffer, waves 1-3 fill the other.  Float sums take
+// the exact block form below (vlad_exact_chain); double sums run as one
+// lane's chain of dependent additions, fed by 16-byte LDS reads issued 8
+// ahead.  The division then reads the encoding again with independent loads.
+template <typename T>
+struct VladNorm {
+  static constexpr int CH = 16384 / sizeof(T);  // values per LDS buffer
+  static constexpr int V = 16 / sizeof(T);      // values per 16-byte LDS read
+};
+
+// Threads t = 0..NT-1 of the workgroup write the squares of e[0..m) into sq:
+// CH / NT values each, every load issued before the first square is stored.
+template <typename T, int NT>
+__device__ __forceinline__ void vlad_norm_stage(const T* __restrict__ e, int m, T* __restrict__ sq, int t) {
+#pragma clang fp contract(off)
+  constexpr int PER = (VladNorm<T>::CH + NT - 1) / NT;
+  T v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = t + NT * j;
+    v[j] = i < m ? e[i] : T(0);
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = t + NT * j;
+    if (i < VladNorm<T>::CH) sq[i] = v[j] * v[j];
+  }
+}
+
+// n + sq[0] + sq[1] + ... + sq[m-1], in that order, one lane.
+template <typename T>
+__device__ __forceinline__ T vlad_norm_chain(T n, const T* __restrict__ sq, int m) {
+#pragma clang fp contract(off)
+  constexpr int V = VladNorm<T>::V, G = 8;
+  typedef T vec __attribute__((ext_vector_type(V)));
+  const vec* q = reinterpret_cast<const vec*>(sq);
+  const int mv = m / V;
+  const int full = (mv / (2 * G)) * (2 * G);
+  vec a[G], b[G];
+  if (full > 0) {
+#pragma unroll
+    for (int j = 0; j < G; ++j) a[j] = q[j];
+  }
+  for (int i = 0; i < full; i += 2 * G) {
+#pragma unroll
+    for (int j = 0; j < G; ++j) b[j] = q[i + G + j];
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+#pragma unroll
+      for (int t = 0; t < V; ++t) n = n + a[j][t];
+    if (i + 2 * G < full) {
+#pragma unroll
+      for (int j = 0; j < G; ++j) a[j] = q[i + 2 * G + j];
+    }
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+#pragma unroll
+      for (int t = 0; t < V; ++t) n = n + b[j][t];
+  }
+  for (int i = full * V; i < m; ++i) n = n + sq[i];
+  return n;
+}
+
+// The same sum for float terms q >= 0 (or inf / NaN), bit for bit, a block
+// of 64 x VN_J terms at a time on one wave.  While the running sum S stays
+// below 2^(e+1), where 2^e <= S (e = -126 for S below the normal range),
+// every float it takes is a multiple of u = 2^(e-23), and adding a term q
+// rounds S + q to S + u * k with k = q/u rounded to the nearest integer --
+// independent of S, unless q/u is an exact half (a tie, broken towards the
+// even result, which depends on S).  So the sequential sum over a block equals
+// S + u * (sum of the k) when no term ties, no term is inf or NaN or
+// >= 2^(e+1), and the integer total stays below 2^24 (S remains in the
+// binade).  Otherwise the prefix before the first term that breaks one of
+// these is added the same way, that term is added as the reference adds it,
+// and the next block starts after it (the rare cases: a tie, and the few
+// times S crosses a power of two).
+constexpr int VN_J = 8;
+
+// Inclusive prefix sum over the 64 lanes of a wave: within each row of 16
+// lanes by shifts of 1, 2, 4 and 8, then row 15's total into rows 1 and 3 and
+// lane 31's into rows 2 and 3 (DPP row_shr / row_bcast; lanes without a
+// source add 0).
+__device__ __forceinline__ unsigned vn_wave_scan(unsigned v) {
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
+  v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
+  return v;
+}
+
+// The first `head` terms are added one by one: at the start of the sum S is
+// no larger than single terms and would leave its binade every few terms.
+__device__ __forceinline__ float vlad_exact_chain(float S, const float* __restrict__ sq, int m, int head, int lane) {
+#pragma clang fp contract(off)
+#pragma unroll 8
+  for (int i = 0; i < head; ++i) S = S + sq[i];
+  int p = head;
+  while (p < m) {
+    if (!(S < INFINITY)) {
+      // S is +inf or NaN: NaN if S or any later term is NaN, else +inf
+      bool nan = S != S;
+      for (int i = p + lane; i < m; i += 64) nan |= sq[i] != sq[i];
+      return __ballot(nan) ? __builtin_nanf("") : S;
+    }
+    int e = S == 0.f ? -126 : __builtin_amdgcn_frexp_expf(S) - 1;
+    if (e < -126) e = -126;
+    const float lim = ldexpf(1.f, e + 1);
+    const unsigned Sm = (unsigned)ldexpf(S, 23 - e);  // S / u, below 2^24
+    unsigned k[VN_J];
+    int fj = VN_J;  // first term of this lane that cannot take the block path
+#pragma unroll
+    for (int j = 0; j < VN_J; ++j) {
+      const int i = p + lane * VN_J + j;
+      const float q = i < m ? sq[i] : 0.f;
+      const bool ok = q < lim;  // false for inf, NaN and terms >= 2^(e+1)
+      const float sc = ldexpf(ok ? q : 0.f, 23 - e);
+      const float fl = floorf(sc);
+      const float fr = sc - fl;
+      k[j] = (unsigned)fl + (fr > 0.5f ? 1u : 0u);
+      if ((!ok || fr == 0.5f) && fj == VN_J) fj = j;
+    }
+    unsigned L = 0;
+#pragma unroll
+    for (int j = 0; j < VN_J; ++j) L += k[j];
+    const unsigned Lc = L < (1u << 25) ? L : (1u << 25);
+    const unsigned incl = vn_wave_scan(Lc);
+    const unsigned excl = incl - Lc;
+    // first term at which the running integer reaches 2^24 (S leaves the
+    // binade), and the prefix before this lane's first stop
+    unsigned run = Sm + (excl < (1u << 25) ? excl : (1u << 25));
+    int f = fj;
+    unsigned pre = 0;
+#pragma unroll
+    for (int j = 0; j < VN_J; ++j) {
+      if (j < f && run + k[j] >= (1u << 24)) f = j;
+      if (j < f) {
+        run += k[j];
+        pre += k[j];
+      }
+    }
+    const unsigned long long stop = __ballot(f < VN_J);
+    if (stop == 0) {
+      S = ldexpf((float)(Sm + (unsigned)__builtin_amdgcn_readlane((int)incl, 63)), e - 23);
+      p += 64 * VN_J;
+      continue;
+    }
+    const int fl_lane = __builtin_ctzll(stop);
+    const int fjx = __builtin_amdgcn_readlane(f, fl_lane);
+    const unsigned before = (unsigned)__builtin_amdgcn_readlane((int)(excl + pre), fl_lane);
+    S = ldexpf((float)(Sm + before), e - 23);
+    const int fi = p + fl_lane * VN_J + fjx;
+    S = S + sq[fi];
+    p = fi + 1;
+  }
+  return S;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void vlad_normalize_kernel(T* __restrict__ enc, size_t len) {
 #pragma clang fp contract(off)
+  constexpr int CH = VladNorm<T>::CH;
+  __shared__ __align__(16) T sq[2][CH];
   __shared__ T nsh;
-  __shared__ __align__(16) T buf[VL_NCHUNK];
   T* e = enc + (size_t)blockIdx.x * len;
+  const int tid = threadIdx.x;
+  const size_t nch = (len + CH - 1) / CH;
+  vlad_norm_stage<T, 256>(e, (int)min((size_t)CH, len), sq[0], tid);
+  __syncthreads();
   T n = T(0);
-  for (size_t c0 = 0; c0 < len; c0 += VL_NCHUNK) {
-    const int m = (int)min((size_t)VL_NCHUNK, len - c0);
-    __syncthreads();
-    for (int i = threadIdx.x; i < m; i += blockDim.x) {
-      const T z = e[c0 + i];
-      buf[i] = z * z;
+  for (size_t c = 0; c < nch; ++c) {
+    if (tid < 64) {
+      const int m = (int)min((size_t)CH, len - c * CH);
+      if constexpr (sizeof(T) == 4)
+        n = vlad_exact_chain(n, reinterpret_cast<const float*>(sq[c & 1]), m, c == 0 ? min(m, 64) : 0, tid);
+      else if (tid == 0)
+        n = vlad_norm_chain<T>(n, sq[c & 1], m);
+    } else if (c + 1 < nch) {
+      const size_t c0 = (c + 1) * CH;
+      vlad_norm_stage<T, 192>(e + c0, (int)min((size_t)CH, len - c0), sq[(c + 1) & 1], tid - 64);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      constexpr int V = 16 / sizeof(T);
-      typedef T vec __attribute__((ext_vector_type(V)));
-      const int mv = m / V;
-      for (int i = 0; i < mv; ++i) {
-        const vec q = reinterpret_cast<const vec*>(buf)[i];
-#pragma unroll
-        for (int t = 0; t < V; ++t) n = n + q[t];
-      }
-      for (int i = mv * V; i < m; ++i) n = n + buf[i];
-    }
   }
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     n = dev_sqrt<T>(n);
     nsh = (T)(((double)n > 1e-12) ? (double)n : 1e-12);
   }
   __syncthreads();
   const T nn = nsh;
-  for (size_t i = threadIdx.x; i < len; i += blockDim.x) e[i] = e[i] / nn;
+  constexpr int U = 16;
+  for (size_t i0 = 0; i0 < len; i0 += (size_t)256 * U) {
+    T v[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const size_t i = i0 + tid + 256 * j;
+      v[j] = i < len ? e[i] : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const size_t i = i0 + tid + 256 * j;
+      if (i < len) e[i] = v[j] / nn;
+    }
+  }
 }
 
 // Hard assignments (one cluster per descriptor): workgroup (image, block of
 // CB clusters), thread d owns dimension d of every cluster of the block and
-// walks the image's descriptors in index order, adding x_id to the LDS
-// accumulator of the descriptor's cluster when it is in the block -- the same
-// sequence of roundings per (cluster, dimension) as vlad_accum_kernel with a
-// one-hot q (vlad.c:181-188), without scanning every cluster for every row.
-// The rows of the block are found from a ballot over 64 cluster indices and
-// their coordinates loaded VH_RS at a time, so that a few memory round trips
-// serve 64 rows.
-// Finalisation as vlad_accum_kernel (vlad.c:191-227).  D <= 128.  Blocks of
-// 32 clusters: every workgroup walks all of its image's cluster indices, so
-// small blocks mean many short-lived workgroups (16 per image at K = 512)
-// that keep the device busy whatever the number of images.
-#ifndef VH_RS
-#define VH_RS 16
-#endif
-constexpr int VH_COMPS = 32;
+// adds the block's descriptors in index order -- the same sequence of
+// roundings per (cluster, dimension) as vlad_accum_kernel with a one-hot q
+// (vlad.c:181-188), without scanning every cluster for every row.  With
+// q = 1 the reference's step enc = (TYPE)(enc + q * x), formed in double, is
+// the plain TYPE sum: for float, rounding the exact double sum of two floats
+// to float gives the correctly rounded float sum (53 >= 2 * 24 + 2 bits, so
+// the double rounding is innocuous for addition).
+// Per chunk of VH_CHUNK rows: the cluster indices are staged in LDS and each
+// wave compacts the rows of its half that fall in the block into an LDS list
+// (ballots, index order kept) and counts them per cluster (the mass: the
+// reference adds q = 1 in double per row, which is the count exactly).  For
+// blocks of up to 8 clusters the list is sorted by cluster (index order
+// within a cluster), so that each cluster's sum is a plain chain of adds
+// into a register; larger blocks add in index order into LDS accumulators.
+// The listed rows are gathered into LDS in stages, every 16-byte load of a
+// stage in flight together.  Real assignments are skewed (a few clusters
+// hold many rows), so the workgroup of the largest clusters sets the time:
+// it pays one memory round trip per stage, not per row.
+// Finalisation as vlad_accum_kernel (vlad.c:191-227).  D <= 128, CB <= 256.
+// The host picks CB so that even one image spreads over enough workgroups.
+constexpr int VH_CHUNK = 2048;
+
+// Stage rows s0.. of the block's list (list0[0..n0) then list1) into xs as
+// pieces of P (16 bytes, or one value): 16 loads per thread issued together.
+template <typename T, typename P>
+__device__ __forceinline__ void vh_gather(const T* __restrict__ x, long long c0, int D, const uint32_t* list0, int n0,
+                                          const uint32_t* list1, int s0, int npieces, T* xs, int tid) {
+  constexpr int GU = 16;
+  const int DW = D / (int)(sizeof(P) / sizeof(T));
+  for (int q0 = 0; q0 < npieces; q0 += 128 * GU) {
+    P v[GU];
+#pragma unroll
+    for (int g = 0; g < GU; ++g) {
+      const int q = q0 + tid + 128 * g;
+      if (q < npieces) {
+        const int r = q / DW, c = q - r * DW;
+        const int p = s0 + r;
+        const uint32_t ent = p < n0 ? list0[p] : list1[p - n0];
+        v[g] = reinterpret_cast<const P*>(x + (c0 + (long long)(ent >> 8)) * D)[c];
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < GU; ++g) {
+      const int q = q0 + tid + 128 * g;
+      if (q < npieces) reinterpret_cast<P*>(xs)[q] = v[g];
+    }
+  }
+}
 
 template <typename T, int CB>
 __global__ __launch_bounds__(128) void vlad_hard_kernel(const T* __restrict__ x, const uint64_t* __restrict__ offsets,
@@ -201,10 +401,21 @@ __global__ __launch_bounds__(128) void vlad_hard_kernel(const T* __restrict__ x,
                                                         const uint32_t* __restrict__ hard, int flags,
                                                         T* __restrict__ enc, int numKB) {
 #pragma clang fp contract(off)
+  static_assert(CB <= 256, "cluster block must fit the 8-bit list field");
+  constexpr bool REG = CB <= 8;                          // accumulate in registers
+  constexpr int XS = (REG ? 32768 : 8192) / sizeof(T);  // staged values
+  typedef unsigned int piece16 __attribute__((ext_vector_type(4)));
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* acc = reinterpret_cast<T*>(smem_raw);                             // [CB][D]
   double* mass = reinterpret_cast<double*>(smem_raw + (((size_t)CB * D * sizeof(T) + 15) & ~size_t(15)));
   T* nrm = reinterpret_cast<T*>(mass + CB);                           // [CB]
+  __shared__ uint32_t idxs[VH_CHUNK];
+  __shared__ uint32_t list[2][VH_CHUNK / 2];  // (row in chunk) << 8 | cluster in block
+  uint32_t* const lflat = &list[0][0];        // REG: one sorted list of VH_CHUNK
+  __shared__ int cnt[2];
+  __shared__ unsigned counts[CB];
+  __shared__ int wcnt[2][REG ? CB : 1];  // rows per (wave, cluster) of the chunk
+  __shared__ __align__(16) T xs[XS];
   const int b = blockIdx.x;
   int img, kb;
   if (numImages >= 8) {
@@ -216,48 +427,135 @@ __global__ __launch_bounds__(128) void vlad_hard_kernel(const T* __restrict__ x,
     kb = b % numKB;
   }
   if (img >= numImages) return;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int k0 = kb * CB;
-  for (int i = tid; i < CB * D; i += blockDim.x) acc[i] = T(0);
-  for (int i = tid; i < CB; i += blockDim.x) mass[i] = 0.0;
-  __syncthreads();
+  if (!REG)
+    for (int i = tid; i < CB * D; i += blockDim.x) acc[i] = T(0);
+  for (int i = tid; i < CB; i += blockDim.x) counts[i] = 0u;
+  T a[REG ? CB : 1];
+#pragma unroll
+  for (int c = 0; c < (REG ? CB : 1); ++c) a[c] = T(0);
   const long long rBegin = (long long)offsets[img], rEnd = (long long)offsets[img + 1];
-  const int d = tid, lane = tid & 63;
-  // 64 rows at a time: each wave reads their cluster indices (one per lane),
-  // the rows of this block are picked from the ballot in index order, their
-  // coordinates fetched VH_RS at a time, then added in order.  The two waves
-  // own disjoint dimensions and walk the rows independently.
-  for (long long r0 = rBegin; r0 < rEnd; r0 += 64) {
-    const long long rr = r0 + lane;
-    const uint32_t kk = rr < rEnd ? hard[rr] : 0xffffffffu;
-    const int kl = (int)kk - k0;
-    unsigned long long mask = __ballot(kk != 0xffffffffu && kl >= 0 && kl < CB);
-    while (mask) {
-      int us[VH_RS], ks[VH_RS];
+  const int d = tid;
+  const bool vec16 = (D * (int)sizeof(T)) % 16 == 0 && ((uintptr_t)x & 15) == 0;
+  const int SR = XS / D;  // rows per stage
+  const unsigned long long lt = (1ull << lane) - 1;
+  for (long long c0 = rBegin; c0 < rEnd; c0 += VH_CHUNK) {
+    const int nr = (int)min((long long)VH_CHUNK, rEnd - c0);
+    __syncthreads();  // the previous chunk's lists are consumed
+    {
+      constexpr int PER = VH_CHUNK / 128;
+      uint32_t v[PER];
 #pragma unroll
-      for (int q = 0; q < VH_RS; ++q) {
-        if (mask) {
-          const int u = __builtin_ctzll(mask);
-          mask &= mask - 1;
-          us[q] = u;
-          ks[q] = __shfl(kl, u, 64);
+      for (int j = 0; j < PER; ++j) {
+        const int i = tid + 128 * j;
+        v[j] = i < nr ? hard[c0 + i] : 0xffffffffu;
+      }
+#pragma unroll
+      for (int j = 0; j < PER; ++j) idxs[tid + 128 * j] = v[j];
+    }
+    __syncthreads();
+    int cst[REG ? CB + 1 : 1];  // cluster c's rows: sorted list [cst[c], cst[c+1])
+    int ntot;
+    if constexpr (REG) {
+      // counts per (wave, cluster), then each wave writes its rows of cluster
+      // c after the rows of c of the clusters before and of wave 0
+      int m[CB];
+#pragma unroll
+      for (int c = 0; c < CB; ++c) m[c] = 0;
+      for (int g = 0; g < VH_CHUNK / 128; ++g) {
+        const uint32_t kk = idxs[wave * (VH_CHUNK / 2) + g * 64 + lane];
+        const int kl = (int)kk - k0;
+        const bool in = kk != 0xffffffffu && kl >= 0 && kl < CB;
+#pragma unroll
+        for (int c = 0; c < CB; ++c) m[c] += __popcll(__ballot(in && kl == c));
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < CB; ++c) wcnt[wave][c] = m[c];
+      }
+      __syncthreads();
+      int base[CB];
+      cst[0] = 0;
+#pragma unroll
+      for (int c = 0; c < CB; ++c) {
+        const int w0 = wcnt[0][c], w1 = wcnt[1][c];
+        base[c] = cst[c] + (wave ? w0 : 0);
+        cst[c + 1] = cst[c] + w0 + w1;
+      }
+      ntot = cst[CB];
+      for (int g = 0; g < VH_CHUNK / 128; ++g) {
+        const int r = wave * (VH_CHUNK / 2) + g * 64 + lane;
+        const uint32_t kk = idxs[r];
+        const int kl = (int)kk - k0;
+        const bool in = kk != 0xffffffffu && kl >= 0 && kl < CB;
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+          const unsigned long long mk = __ballot(in && kl == c);
+          if (in && kl == c) lflat[base[c] + __popcll(mk & lt)] = (uint32_t)r << 8 | (uint32_t)c;
+          base[c] += __popcll(mk);
+        }
+      }
+      if (tid < CB) counts[tid] += (unsigned)(wcnt[0][tid] + wcnt[1][tid]);
+    } else {
+      int n = 0;
+      for (int g = 0; g < VH_CHUNK / 128; ++g) {
+        const int r = wave * (VH_CHUNK / 2) + g * 64 + lane;
+        const uint32_t kk = idxs[r];
+        const int kl = (int)kk - k0;
+        const bool in = kk != 0xffffffffu && kl >= 0 && kl < CB;
+        const unsigned long long mask = __ballot(in);
+        if (in) {
+          list[wave][n + __popcll(mask & lt)] = ((uint32_t)r << 8) | (uint32_t)kl;
+          atomicAdd(&counts[kl], 1u);
+        }
+        n += __popcll(mask);
+      }
+      if (lane == 0) cnt[wave] = n;
+      cst[0] = 0;
+      ntot = 0;
+    }
+    __syncthreads();
+    const int n0 = REG ? ntot : cnt[0];
+    if (!REG) ntot = n0 + cnt[1];
+    const uint32_t* l0 = REG ? lflat : list[0];
+    const uint32_t* l1 = REG ? lflat : list[1];  // REG: every listed row is in l0
+    for (int s0 = 0; s0 < ntot; s0 += SR) {
+      const int ns = min(SR, ntot - s0);
+      if (vec16)
+        vh_gather<T, piece16>(x, c0, D, l0, n0, l1, s0, ns * D / (16 / (int)sizeof(T)), xs, tid);
+      else
+        vh_gather<T, T>(x, c0, D, l0, n0, l1, s0, ns * D, xs, tid);
+      __syncthreads();
+      if (d < D) {
+        if constexpr (REG) {
+#pragma unroll
+          for (int c = 0; c < CB; ++c) {
+            const int lo = max(cst[c], s0), hi = min(cst[c + 1], s0 + ns);
+            T t = a[c];
+#pragma unroll 8
+            for (int r = lo; r < hi; ++r) t = t + xs[(r - s0) * D + d];
+            a[c] = t;
+          }
         } else {
-          us[q] = -1;
-          ks[q] = -1;
+          for (int r = 0; r < ns; ++r) {
+            const int p = s0 + r;
+            const uint32_t ent = p < n0 ? l0[p] : l1[p - n0];
+            const int kl = (int)(ent & 255u);
+            acc[kl * D + d] = acc[kl * D + d] + xs[r * D + d];
+          }
         }
       }
-      T xv[VH_RS];
-#pragma unroll
-      for (int q = 0; q < VH_RS; ++q) xv[q] = (us[q] >= 0 && d < D) ? x[(r0 + us[q]) * D + d] : T(0);
-#pragma unroll
-      for (int q = 0; q < VH_RS; ++q) {
-        if (ks[q] >= 0) {
-          if (d < D) acc[ks[q] * D + d] = (T)((double)acc[ks[q] * D + d] + (double)xv[q]);
-          if (tid == 0) mass[ks[q]] += 1.0;
-        }
-      }
+      __syncthreads();
     }
   }
+  if constexpr (REG) {
+    if (d < D) {
+#pragma unroll
+      for (int c = 0; c < CB; ++c) acc[c * D + d] = a[c];
+    }
+  }
+  for (int i = tid; i < CB; i += blockDim.x) mass[i] = (double)counts[i];
   __syncthreads();
   // vlad.c:191-213 per (cluster, dimension), then the per-cluster norms
   for (int i = tid; i < CB * D; i += blockDim.x) {
@@ -301,8 +599,8 @@ __global__ __launch_bounds__(128) void vlad_hard_kernel(const T* __restrict__ x,
 }
 
 template <typename T, int CB>
-static void launch_vlad_hard(T* enc, const T* means, int D, int K, const T* x, const uint64_t* offsets, int numImages,
-                             const uint32_t* hard, int flags) {
+static void launch_vlad_hard_cb(T* enc, const T* means, int D, int K, const T* x, const uint64_t* offsets,
+                                int numImages, const uint32_t* hard, int flags) {
   const int nkb = (K + CB - 1) / CB;
   const int padImages = numImages >= 8 ? ((numImages + 7) / 8) * 8 : numImages;
   const size_t sh = (((size_t)CB * D * sizeof(T) + 15) & ~size_t(15)) + CB * sizeof(double) + CB * sizeof(T);
@@ -310,6 +608,22 @@ static void launch_vlad_hard(T* enc, const T* means, int D, int K, const T* x, c
   VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
   VLHIP_LAUNCH("vlad_hard", kern, dim3((unsigned)((size_t)padImages * nkb)), dim3(128), sh, x, offsets, numImages, D,
                K, means, hard, flags, enc, nkb);
+}
+
+// Blocks of 32 clusters when the images alone give the device >= 1024
+// workgroups; fewer clusters per block (down to 4) for small batches, since
+// every workgroup walks all of its image's cluster indices.
+template <typename T>
+static void launch_vlad_hard(T* enc, const T* means, int D, int K, const T* x, const uint64_t* offsets, int numImages,
+                             const uint32_t* hard, int flags) {
+  int cb = 32;
+  while (cb > 4 && (size_t)numImages * (size_t)((K + cb - 1) / cb) < 1024) cb >>= 1;
+  switch (cb) {
+    case 32: launch_vlad_hard_cb<T, 32>(enc, means, D, K, x, offsets, numImages, hard, flags); break;
+    case 16: launch_vlad_hard_cb<T, 16>(enc, means, D, K, x, offsets, numImages, hard, flags); break;
+    case 8: launch_vlad_hard_cb<T, 8>(enc, means, D, K, x, offsets, numImages, hard, flags); break;
+    default: launch_vlad_hard_cb<T, 4>(enc, means, D, K, x, offsets, numImages, hard, flags); break;
+  }
 }
 
 template <typename T, int DPT>
@@ -320,7 +634,7 @@ static void launch_vlad(T* enc, const T* means, int D, int K, const T* x, const 
   const int padImages = numImages >= 8 ? ((numImages + 7) / 8) * 8 : numImages;
   const unsigned blocks = (unsigned)((size_t)padImages * numKB);
   if (hard && D <= 128) {
-    launch_vlad_hard<T, VH_COMPS>(enc, means, D, K, x, offsets, numImages, hard, flags);
+    launch_vlad_hard<T>(enc, means, D, K, x, offsets, numImages, hard, flags);
   } else if (hard) {
     auto kern = vlad_accum_kernel<T, DPT, true>;
     VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
