@@ -129,24 +129,27 @@ vl_fisher_encode_batch (void * enc, vl_type dataType,
  * (vlad.c:181-188) and a weight of 1 adds the descriptor unscaled, so the
  * encoding from the cluster indices (0xffffffff for an empty row) is the same
  * number for number and the caller's N x K matrix need not be copied to the
- * GPU.  Returns 0 when some row is not of that form. */
+ * GPU.  One branch-free pass per row counts the positive entries, sums their
+ * indices and flags any positive entry other than 1 (vectorised; an AVX2
+ * clone is picked at run time where the host has it).  Returns 0 when some
+ * row is not of that form. */
 #define VLAD_ONEHOT_ROW(NAME, TYPE)                                                  \
-  __attribute__((optimize("O3", "tree-vectorize")))                                  \
+  __attribute__((optimize("O3", "tree-vectorize"), target_clones("avx2", "default"))) \
   static int NAME (TYPE const * row, vl_size numClusters, vl_uint32 * out)           \
   {                                                                                  \
-    unsigned positives = 0 ;                                                         \
+    unsigned positives = 0, other = 0 ;                                              \
+    vl_uint64 at = 0 ;                                                               \
     vl_uindex k ;                                                                    \
-    for (k = 0 ; k < numClusters ; ++k) positives += row[k] > 0 ;                    \
-    if (positives == 0) { *out = 0xffffffffu ; return 1 ; }                          \
-    if (positives > 1) return 0 ;                                                    \
     for (k = 0 ; k < numClusters ; ++k) {                                            \
-      if (row[k] > 0) {                                                              \
-        if (row[k] != 1) return 0 ;                                                  \
-        *out = (vl_uint32) k ;                                                       \
-        return 1 ;                                                                   \
-      }                                                                              \
+      unsigned pos = row[k] > 0 ;                                                    \
+      positives += pos ;                                                             \
+      at += pos ? k : 0 ;                                                            \
+      other |= pos & (row[k] != 1) ;                                                 \
     }                                                                                \
-    return 0 ;                                                                       \
+    if (positives == 0) { *out = 0xffffffffu ; return 1 ; }                          \
+    if (positives > 1 || other) return 0 ;                                           \
+    *out = (vl_uint32) at ;                                                          \
+    return 1 ;                                                                       \
   }
 VLAD_ONEHOT_ROW(vlad_onehot_row_f, float)
 VLAD_ONEHOT_ROW(vlad_onehot_row_d, double)
@@ -176,13 +179,6 @@ vlad_run (void * enc, vl_type dataType, void const * means, vl_size dimension, v
   vl_uint64 * dOff ;
   vl_uint32 * indices = NULL ;
 
-  if (soft && numData) {
-    indices = (vl_uint32*) malloc(sizeof(vl_uint32) * numData) ;
-    if (indices && vlad_onehot_indices(indices, dataType, soft, numData, numClusters)) {
-      hard = indices ;
-      soft = NULL ;
-    }
-  }
   offsets[0] = 0 ;
   offsets[1] = numData ;
   vlhip_lock() ;
@@ -190,16 +186,24 @@ vlad_run (void * enc, vl_type dataType, void const * means, vl_size dimension, v
   dData = vlhip_buffer(VLHIP_BUF_DATA, es * dimension * (numData ? numData : 1)) ;
   dEnc = vlhip_buffer(VLHIP_BUF_ENC, es * KD) ;
   dOff = (vl_uint64*) vlhip_buffer(VLHIP_BUF_AUX, sizeof(offsets)) ;
-  dAssign = vlhip_buffer(VLHIP_BUF_POST, (soft ? es * numClusters : sizeof(vl_uint32)) * (numData ? numData : 1)) ;
-  /* stream-ordered copies (offsets and indices stay valid until the final
-   * download, which waits for everything before it) */
+  /* stream-ordered copies (the caller's arrays, offsets and indices stay
+   * valid until the final download, which waits for everything before it);
+   * the means and data travel while the assignment matrix is examined */
   vlhip_memcpy_h2d_async(dMeans, means, es * KD) ;
+  if (numData) vlhip_memcpy_h2d_async(dData, data, es * dimension * numData) ;
+  vlhip_memcpy_h2d_async(dOff, offsets, sizeof(offsets)) ;
+  if (soft && numData) {
+    indices = (vl_uint32*) malloc(sizeof(vl_uint32) * numData) ;
+    if (indices && vlad_onehot_indices(indices, dataType, soft, numData, numClusters)) {
+      hard = indices ;
+      soft = NULL ;
+    }
+  }
+  dAssign = vlhip_buffer(VLHIP_BUF_POST, (soft ? es * numClusters : sizeof(vl_uint32)) * (numData ? numData : 1)) ;
   if (numData) {
-    vlhip_memcpy_h2d_async(dData, data, es * dimension * numData) ;
     if (soft) vlhip_memcpy_h2d_async(dAssign, soft, es * numClusters * numData) ;
     else vlhip_memcpy_h2d_async(dAssign, hard, sizeof(vl_uint32) * numData) ;
   }
-  vlhip_memcpy_h2d_async(dOff, offsets, sizeof(offsets)) ;
   vlhip_vlad_encode_batch((int)dataType, dEnc, dMeans, dimension, numClusters, dData, (uint64_t const*)dOff, 1,
                           soft ? dAssign : NULL, soft ? NULL : (uint32_t const*)dAssign, flags) ;
   vlhip_memcpy_d2h(enc, dEnc, es * KD) ;
