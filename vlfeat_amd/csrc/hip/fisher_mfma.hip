@@ -855,7 +855,7 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
   V8 ones;
 #pragma unroll
   for (int v = 0; v < 8; ++v) ones[v] = H(1.f);
-  unsigned terms = 0;
+  unsigned terms = 0;  // kept (row, component) pairs of this wave, counted from lane masks
   double massd[CT];  // EM: unthresholded posterior mass of component kw + 16c + lr
 #pragma unroll
   for (int c = 0; c < CT; ++c) massd[c] = 0.0;
@@ -958,7 +958,7 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
 #pragma unroll
             for (int c = 0; c < CT; ++c) {
               const bool hit = kw + 16 * c + lr == __float_as_int(ar4[t][r]) && bc[c] != -INFINITY;
-              terms += hit ? 1u : 0u;
+              terms += (unsigned)__popcll(__ballot(hit));
               aq[c][4 * t + r] = hit ? H(sc.sq) : H(0.f);
             }
       } else {
@@ -973,7 +973,7 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
             for (int c = 0; c < CT; ++c) {
               const float v = __builtin_amdgcn_exp2f(acc[t][c][r]) * rs4[t][r];
               const bool keep = v >= qminS;
-              terms += keep ? 1u : 0u;
+              terms += (unsigned)__popcll(__ballot(keep));
               // padding rows (rs4 = 0) may have 2^acc = inf: keep them out of the mass
               if (EM) msum[c] += rs4[t][r] > 0.f ? v : 0.f;
               aq[c][4 * t + r] = H(keep ? v : 0.f);
@@ -1069,7 +1069,8 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
   const double bn = block_sum_waves<WAVES>(nrm, red);
   if (tid == 0) normpart[(size_t)img * numKB + kb] = bn;
   if (numTerms) {
-    const double bt = block_sum_waves<WAVES>((double)terms, red);
+    // terms is the wave's count (lane masks), entered once per wave
+    const double bt = block_sum_waves<WAVES>(lane == 0 ? (double)terms : 0.0, red);
     if (tid == 0) atomicAdd(&numTerms[img], (unsigned long long)bt);
   }
 }
