@@ -16,28 +16,46 @@ namespace vlhip {
 //   cterm[k] = log pi_k - D/2 log(2 pi) - 1/2 sum_d log sigma2_kd
 // (-inf when pi_k < 1e-6, gmm.c:746-747), inv[k,d] = 1/sigma2 in T
 // (gmm.c:753), isig[k,d] = sqrt(1/sigma2) rounded from FP64 (fisher.c:388).
+// One workgroup per component: the threads take the dimensions (log, 1/sigma2
+// and sqrt(1/sigma2) of each), the logarithms pass through LDS in chunks and
+// thread 0 adds them in dimension order -- the same FP64 sum as a serial loop.
+constexpr int PREP_CHUNK = 1024;
 template <typename T>
-__global__ void gmm_prep_kernel(const T* __restrict__ priors, const T* __restrict__ covs,
-                                int D, int K, T* __restrict__ inv, T* __restrict__ isig,
-                                double* __restrict__ cterm) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void gmm_prep_kernel(const T* __restrict__ priors, const T* __restrict__ covs,
+                                                      int D, int K, T* __restrict__ inv, T* __restrict__ isig,
+                                                      double* __restrict__ cterm) {
+  __shared__ double lg[PREP_CHUNK];
+  const int k = blockIdx.x;
   if (k >= K) return;
   double logdet = 0.0;
-  for (int d = 0; d < D; ++d) {
-    const T c = covs[(size_t)k * D + d];
-    logdet += log((double)c);
-    inv[(size_t)k * D + d] = T(1) / c;
-    isig[(size_t)k * D + d] = (T)sqrt(1.0 / (double)c);
+  for (int d0 = 0; d0 < D; d0 += PREP_CHUNK) {
+    const int n = min(PREP_CHUNK, D - d0);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const size_t at = (size_t)k * D + d0 + i;
+      const T c = covs[at];
+      lg[i] = log((double)c);
+      inv[at] = T(1) / c;
+      isig[at] = (T)sqrt(1.0 / (double)c);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int i = 0; i < n; ++i) logdet += lg[i];
+    __syncthreads();
   }
-  const double p = (double)priors[k];
-  const double logw = (p < 1e-6) ? -(double)INFINITY : log(p);
-  cterm[k] = logw - 0.5 * (double)D * kLog2Pi - 0.5 * logdet;
+  if (threadIdx.x == 0) {
+    const double p = (double)priors[k];
+    const double logw = (p < 1e-6) ? -(double)INFINITY : log(p);
+    cterm[k] = logw - 0.5 * (double)D * kLog2Pi - 0.5 * logdet;
+  }
 }
 
 // ---------------------------------------------------------------------------
-// E-step, gmm.c:758-791.  One workgroup owns EM_ROWS descriptors and sweeps all
-// K components in tiles of EM_COLS; every thread keeps a 4x4 register tile of
-// squared Mahalanobis distances, accumulated over D in LDS-staged chunks.  The
+// E-step, gmm.c:758-791.  One workgroup owns 16R descriptors (R = 4: EM_ROWS)
+// and sweeps all K components in tiles of EM_COLS; every thread keeps an R x 4
+// register tile of squared Mahalanobis distances, accumulated over D in
+// LDS-staged chunks.  A row's arithmetic does not depend on R (same chunks,
+// tiles and merge order), so R = 1 -- four times the workgroups -- serves
+// small launches with the same posteriors bit for bit.  The
 // row log-sum-exp is kept online per thread and merged across the 16 threads
 // of a row at the end.  With WRITE_Q the logits are written to q during the
 // sweep and normalised in place afterwards by the same thread that wrote them.
@@ -46,42 +64,43 @@ __global__ void gmm_prep_kernel(const T* __restrict__ priors, const T* __restric
 // row_s = sum_k exp(logit - max), row_arg = first index of the max logit
 // (the FAST rule of fisher.c:399-417).  ll_part[block] = sum over the block's
 // rows of log(row_s) + row_m, in FP64 (gmm.c:786).
-template <typename T, bool WRITE_Q>
+template <typename T, bool WRITE_Q, int R = 4>
 __global__ __launch_bounds__(256) void estep_kernel(
     const T* __restrict__ x, long long N, int D, int K,
     const T* __restrict__ means, const T* __restrict__ inv, const double* __restrict__ cterm,
     T* __restrict__ q, T* __restrict__ row_m, T* __restrict__ row_s, int* __restrict__ row_arg,
     double* __restrict__ ll_part) {
-  __shared__ __align__(16) T xs[EM_DC][EM_ROWS + 4];
+  constexpr int ROWS = 16 * R;  // rows of the workgroup, R per thread
+  __shared__ __align__(16) T xs[EM_DC][ROWS + 4];
   __shared__ __align__(16) T ms[EM_DC][EM_COLS + 4];
   __shared__ __align__(16) T ss[EM_DC][EM_COLS + 4];
-  __shared__ T red_m[EM_ROWS][17];
-  __shared__ T red_s[EM_ROWS][17];
-  __shared__ T red_b[EM_ROWS][16];
-  __shared__ int red_a[EM_ROWS][16];
+  __shared__ T red_m[ROWS][17];
+  __shared__ T red_s[ROWS][17];
+  __shared__ T red_b[ROWS][16];
+  __shared__ int red_a[ROWS][16];
   __shared__ double red_ll[256];
 
   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-  const long long row0 = (long long)blockIdx.x * EM_ROWS;
+  const long long row0 = (long long)blockIdx.x * ROWS;
 
-  T m_run[4], s_run[4], b_val[4];
-  int b_idx[4];
+  T m_run[R], s_run[R], b_val[R];
+  int b_idx[R];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < R; ++i) {
     m_run[i] = neg_inf<T>(); s_run[i] = T(0); b_val[i] = neg_inf<T>(); b_idx[i] = 0;
   }
 
   for (int k0 = 0; k0 < K; k0 += EM_COLS) {
-    T acc[4][4];
+    T acc[R][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < R; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
 
     for (int d0 = 0; d0 < D; d0 += EM_DC) {
       const int dc = min(EM_DC, D - d0);
       __syncthreads();
-      for (int idx = tid; idx < EM_ROWS * EM_DC; idx += 256) {
+      for (int idx = tid; idx < ROWS * EM_DC; idx += 256) {
         const int r = idx / EM_DC, dd = idx % EM_DC;
         const long long row = row0 + r;
         xs[dd][r] = (dd < dc && row < N) ? x[row * D + d0 + dd] : T(0);
@@ -96,11 +115,11 @@ __global__ __launch_bounds__(256) void estep_kernel(
       __syncthreads();
       for (int dd = 0; dd < dc; ++dd) {
         T xv[4], mv[4], sv[4];
-        lds4(&xs[dd][ty * 4], xv);
+        if constexpr (R == 4) lds4(&xs[dd][ty * R], xv); else xv[0] = xs[dd][ty];
         lds4(&ms[dd][tx * 4], mv);
         lds4(&ss[dd][tx * 4], sv);
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < R; ++i)
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             const T t = xv[i] - mv[j];
@@ -111,8 +130,8 @@ __global__ __launch_bounds__(256) void estep_kernel(
 
     // logits and online log-sum-exp
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const long long row = row0 + ty * 4 + i;
+    for (int i = 0; i < R; ++i) {
+      const long long row = row0 + ty * R + i;
       T l[4];
       T lmax = neg_inf<T>();
 #pragma unroll
@@ -142,15 +161,15 @@ __global__ __launch_bounds__(256) void estep_kernel(
 
   // merge the 16 per-thread partial statistics of each row
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    red_m[ty * 4 + i][tx] = m_run[i];
-    red_s[ty * 4 + i][tx] = s_run[i];
-    red_b[ty * 4 + i][tx] = b_val[i];
-    red_a[ty * 4 + i][tx] = b_idx[i];
+  for (int i = 0; i < R; ++i) {
+    red_m[ty * R + i][tx] = m_run[i];
+    red_s[ty * R + i][tx] = s_run[i];
+    red_b[ty * R + i][tx] = b_val[i];
+    red_a[ty * R + i][tx] = b_idx[i];
   }
   __syncthreads();
   double ll = 0.0;
-  if (tid < EM_ROWS) {
+  if (tid < ROWS) {
     const int r = tid;
     T M = neg_inf<T>();
     for (int t = 0; t < 16; ++t) M = red_m[r][t] > M ? red_m[r][t] : M;
@@ -181,13 +200,13 @@ __global__ __launch_bounds__(256) void estep_kernel(
 
   if (WRITE_Q) {
     // each thread rescales exactly the entries it wrote (no cross-thread reads)
-    T M[4], S[4];
+    T M[R], S[R];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) { M[i] = red_m[ty * 4 + i][16]; S[i] = red_s[ty * 4 + i][16]; }
+    for (int i = 0; i < R; ++i) { M[i] = red_m[ty * R + i][16]; S[i] = red_s[ty * R + i][16]; }
     for (int k0 = 0; k0 < K; k0 += EM_COLS) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const long long row = row0 + ty * 4 + i;
+      for (int i = 0; i < R; ++i) {
+        const long long row = row0 + ty * R + i;
         if (row >= N) continue;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -373,8 +392,8 @@ GmmPrep<T> gmm_prepare(const T* priors, const T* covs, size_t D, size_t K) {
   p.inv = (T*)base;
   p.isig = (T*)(base + bInv);
   p.cterm = (double*)(base + bInv + bSig);
-  VLHIP_LAUNCH("gmm_prep", (gmm_prep_kernel<T>), dim3((unsigned)((K + 127) / 128)), dim3(128), 0,
-               priors, covs, (int)D, (int)K, p.inv, p.isig, p.cterm);
+  VLHIP_LAUNCH("gmm_prep", (gmm_prep_kernel<T>), dim3((unsigned)K), dim3(64), 0, priors, covs, (int)D, (int)K, p.inv,
+               p.isig, p.cterm);
   return p;
 }
 
@@ -383,6 +402,19 @@ double run_estep(const T* x, size_t N, size_t D, size_t K, const T* means, const
                  T* q, T* row_m, T* row_s, int* row_arg, const char* name, bool want_ll) {
   if (N == 0) return 0.0;
   const unsigned blocks = (unsigned)((N + EM_ROWS - 1) / EM_ROWS);
+  const int cus = ctx().num_cus > 0 ? ctx().num_cus : 256;
+  if (!want_ll && blocks < (unsigned)(2 * cus)) {
+    // few rows (one image of a drop-in call): 16-row workgroups; the
+    // log-likelihood partials (grouped by workgroup) are not produced here
+    const unsigned b1 = (unsigned)((N + 15) / 16);
+    if (q)
+      VLHIP_LAUNCH(name, (estep_kernel<T, true, 1>), dim3(b1), dim3(256), 0, x, (long long)N, (int)D, (int)K, means,
+                   prep.inv, prep.cterm, q, row_m, row_s, row_arg, (double*)nullptr);
+    else
+      VLHIP_LAUNCH(name, (estep_kernel<T, false, 1>), dim3(b1), dim3(256), 0, x, (long long)N, (int)D, (int)K, means,
+                   prep.inv, prep.cterm, q, row_m, row_s, row_arg, (double*)nullptr);
+    return 0.0;
+  }
   double* llp = want_ll ? (double*)scratch(SLOT_PARTIAL, (size_t)blocks * sizeof(double)) : nullptr;
   if (q) {
     VLHIP_LAUNCH(name, (estep_kernel<T, true>), dim3(blocks), dim3(256), 0, x, (long long)N, (int)D,
