@@ -282,6 +282,28 @@ def test_vlad_hard_chunks_blocks_and_norm_buffers(n, D, K, dt):
         assert rel(enc, oracle().vlad(x, mu, dense, flags)) < (5e-5 if dt == np.float32 else 1e-12)
 
 
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_vlad_hard_wide_descriptors(dt):
+    # cluster indices with D > 128 take the dense-assignment kernel (one
+    # (cluster, dimension) chain per thread, 16-dimension slices, the last
+    # one partial) and the separate per-cluster normalisation
+    rng = np.random.default_rng(200)
+    n, D, K = 2500, 200, 37
+    x = rng.standard_normal((n, D)).astype(dt)
+    mu = rng.standard_normal((K, D)).astype(dt)
+    a = rng.integers(0, K, n).astype(np.uint32)
+    a[::41] = 0xFFFFFFFF
+    dense = np.zeros((n, K), dt)
+    rows = np.nonzero(a != 0xFFFFFFFF)[0]
+    dense[rows, a[rows]] = 1
+    R = reference()
+    for flags in (0, 1, 3, 15):
+        enc = va.vl_vlad_hard(x, mu, a, flags)
+        if R is not None:
+            assert np.array_equal(enc, R.vlad(x, mu, dense, flags)), flags
+        assert rel(enc, oracle().vlad(x, mu, dense, flags)) < (5e-5 if dt == np.float32 else 1e-12)
+
+
 def _f32_global_norm(z):
     """vlad.c:230-241 in float32, term by term: n = n + z*z, sqrt, max 1e-12."""
     n = np.float32(0)

@@ -1,15 +1,16 @@
 // vlad_kernels.hip -- VLAD encoder (vl/vlad.c:163-242).
 //
-// Workgroup (image, block of VL_COMPS clusters) accumulates, for each of its
-// clusters k and every dimension d, enc[k,d] = sum_i q_ik x_id over the image's
-// descriptors in index order, with the reference's arithmetic: the product and
-// sum are formed in double and the running sum is rounded back to the data
-// type after every term (vlad.c:181-188, where q is double and enc is TYPE),
-// and only q > 0 contributes.  The per-cluster corrections, signed square root
-// and per-cluster L2 normalisation follow vlad.c:191-227 in the same order and
-// precision; the global normalisation (vlad.c:230-241) sums the squared
-// entries sequentially in index order, so for float data the result matches
-// the reference's float path element for element.
+// For each cluster k and dimension d, enc[k,d] = sum_i q_ik x_id over the
+// image's descriptors in index order, with the reference's arithmetic: the
+// product and sum are formed in double and the running sum is rounded back to
+// the data type after every term (vlad.c:181-188, where q is double and enc is
+// TYPE), and only q > 0 contributes.  The per-cluster corrections, signed
+// square root and per-cluster L2 normalisation follow vlad.c:191-227 in the
+// same order and precision; the global normalisation (vlad.c:230-241) gives
+// the reference's sequential sum of the squared entries, so for float data the
+// result matches the reference's float path element for element.  Dense
+// assignments: vlad_slice_kernel; hard assignments (cluster indices, D <= 128):
+// vlad_hard_kernel.
 #include "vlhip_runtime.h"
 #include "vlhip_device.h"
 #include "vlhip_kernels.h"
@@ -19,120 +20,124 @@
 
 namespace vlhip {
 
-constexpr int VL_COMPS = 16;
-constexpr int VL_ROWS = 64;
+// Dense (soft) assignments, and hard ones with D > 128: workgroup (image,
+// block of VS_COMPS clusters, slice of VS_DIMS dimensions); thread (c, l) owns
+// one (cluster, dimension) and adds q_ik x_id over the image's descriptors in
+// index order (vlad.c:181-188: the product and sum formed in double, the sum
+// rounded back to the data type after every term, only q > 0 counted).  One
+// chain per thread and the dimensions spread over workgroups, so that one
+// image fills K D / 256 workgroups.  Descriptors and assignments are staged
+// through LDS VS_ROWS rows at a time, the next chunk's loads in flight while
+// the current one is added up.  Mean subtraction and signed square root
+// (vlad.c:191-213) follow per element; the per-cluster normalisation
+// (vlad.c:215-227) needs every dimension of a cluster and runs afterwards
+// (vlad_component_norm_kernel).
+constexpr int VS_COMPS = 16;
+constexpr int VS_DIMS = 16;
+constexpr int VS_ROWS = 256;
 
-template <typename T, int DPT, bool HARD>
-__global__ __launch_bounds__(256) void vlad_accum_kernel(
+template <typename T, bool HARD>
+__global__ __launch_bounds__(256) void vlad_slice_kernel(
     const T* __restrict__ x, const uint64_t* __restrict__ offsets, int numImages, int D, int K,
     const T* __restrict__ means, const T* __restrict__ assign, const uint32_t* __restrict__ hard,
-    int flags, T* __restrict__ enc, int numKB) {
+    int flags, T* __restrict__ enc, int numKB, int numDS) {
 #pragma clang fp contract(off)
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* xs = reinterpret_cast<T*>(smem_raw);           // [VL_ROWS][D + 1]
-  T* qs = xs + (size_t)VL_ROWS * (D + 1);            // [VL_ROWS][VL_COMPS]
-  T* zs = qs + VL_ROWS * VL_COMPS;                   // [VL_COMPS][D] final values
-  __shared__ double massSh[VL_COMPS];
-
+  __shared__ T xs[VS_ROWS][VS_DIMS + 1];
+  __shared__ T qs[VS_ROWS][VS_COMPS + 1];
+  const int per = numKB * numDS;
   const int b = blockIdx.x;
-  int img, kb;
+  int img, rem;
   if (numImages >= 8) {
     const int xcd = b & 7, slot = b >> 3;
-    img = (slot / numKB) * 8 + xcd;
-    kb = slot % numKB;
+    img = (slot / per) * 8 + xcd;
+    rem = slot % per;
   } else {
-    img = b / numKB;
-    kb = b % numKB;
+    img = b / per;
+    rem = b % per;
   }
   if (img >= numImages) return;
-
-  const int tid = threadIdx.x;
-  const int c = tid >> 4, dg = tid & 15;
-  const int k = kb * VL_COMPS + c;
+  const int kb = rem / numDS, ds = rem % numDS;
+  const int tid = threadIdx.x, c = tid >> 4, l = tid & 15;
+  const int k = kb * VS_COMPS + c, d = ds * VS_DIMS + l;
   const long long rBegin = (long long)offsets[img], rEnd = (long long)offsets[img + 1];
-
-  T acc[DPT];
+  constexpr int PF = VS_ROWS * 16 / 256;  // staged values of each kind per thread
+  T px[PF], pq[PF];
+  auto load_chunk = [&](long long r0, int nr) {
 #pragma unroll
-  for (int j = 0; j < DPT; ++j) acc[j] = T(0);
-  double mass = 0.0;
-
-  for (long long r0 = rBegin; r0 < rEnd; r0 += VL_ROWS) {
-    const int nr = (int)min((long long)VL_ROWS, rEnd - r0);
-    __syncthreads();
-    for (int idx = tid; idx < nr * D; idx += 256) {
-      const int r = idx / D, d = idx - r * D;
-      xs[r * (D + 1) + d] = x[(r0 + r) * D + d];
-    }
-    for (int idx = tid; idx < VL_ROWS * VL_COMPS; idx += 256) {
-      const int r = idx / VL_COMPS, cc = idx - r * VL_COMPS;
-      const int kk = kb * VL_COMPS + cc;
+    for (int u = 0; u < PF; ++u) {
+      const int idx = tid + 256 * u;
+      const int r = idx >> 4, j = idx & 15;
+      const int dd = ds * VS_DIMS + j, kk = kb * VS_COMPS + j;
+      px[u] = (r < nr && dd < D) ? x[(r0 + r) * D + dd] : T(0);
       T qv = T(0);
       if (r < nr && kk < K) {
         if (HARD) qv = (hard[r0 + r] == (uint32_t)kk) ? T(1) : T(0);
         else qv = assign[(r0 + r) * K + kk];
       }
-      qs[idx] = qv;
+      pq[u] = qv;
+    }
+  };
+  T acc = T(0);
+  double mass = 0.0;
+  if (rBegin < rEnd) load_chunk(rBegin, (int)min((long long)VS_ROWS, rEnd - rBegin));
+  for (long long r0 = rBegin; r0 < rEnd; r0 += VS_ROWS) {
+    const int nr = (int)min((long long)VS_ROWS, rEnd - r0);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int idx = tid + 256 * u;
+      xs[idx >> 4][idx & 15] = px[u];
+      qs[idx >> 4][idx & 15] = pq[u];
     }
     __syncthreads();
-    if (k < K) {
+    if (r0 + VS_ROWS < rEnd) load_chunk(r0 + VS_ROWS, (int)min((long long)VS_ROWS, rEnd - r0 - VS_ROWS));
+    if (k < K && d < D) {
       for (int r = 0; r < nr; ++r) {
-        const T qv = qs[r * VL_COMPS + c];
+        const T qv = qs[r][c];
         if (qv > T(0)) {
           const double qd = (double)qv;
           mass += qd;
-#pragma unroll
-          for (int j = 0; j < DPT; ++j) {
-            const int d = dg + 16 * j;
-            if (d < D) acc[j] = (T)((double)acc[j] + qd * (double)xs[r * (D + 1) + d]);
-          }
+          acc = (T)((double)acc + qd * (double)xs[r][l]);
         }
       }
     }
   }
+  if (k < K && d < D) {
+    T e = acc;
+    const T mu = means[(size_t)k * D + d];
+    if (mass > 0) {
+      if (flags & 8) {
+        e = (T)((double)e / mass);
+        e = e - mu;
+      } else {
+        e = (T)((double)e - mass * (double)mu);
+      }
+    }
+    if (flags & 2) e = signed_sqrt(e);
+    enc[(size_t)img * K * D + (size_t)k * D + d] = e;
+  }
+}
 
-  // vlad.c:191-213: mean subtraction and signed square root
-  if (k < K) {
-#pragma unroll
-    for (int j = 0; j < DPT; ++j) {
-      const int d = dg + 16 * j;
-      if (d >= D) continue;
-      T e = acc[j];
-      const T mu = means[(size_t)k * D + d];
-      if (mass > 0) {
-        if (flags & 8) {
-          e = (T)((double)e / mass);
-          e = e - mu;
-        } else {
-          e = (T)((double)e - mass * (double)mu);
-        }
-      }
-      if (flags & 2) e = signed_sqrt(e);
-      zs[c * D + d] = e;
-    }
-  }
+// vlad.c:215-227: each cluster's D values divided by max(||.||, 1e-12), the
+// squared norm summed over d in order by one lane.  Workgroup per (image,
+// cluster), in encoding order.
+template <typename T>
+__global__ __launch_bounds__(64) void vlad_component_norm_kernel(T* __restrict__ enc, int D, int K) {
+#pragma clang fp contract(off)
+  __shared__ T z[512];
+  __shared__ T nsh;
+  T* e = enc + (size_t)blockIdx.x * D;
+  for (int i = threadIdx.x; i < D; i += 64) z[i] = e[i];
   __syncthreads();
-  // vlad.c:215-227: per-cluster L2 normalisation, sequential sum over d
-  if ((flags & 1) && dg == 0 && k < K) {
+  if (threadIdx.x == 0) {
     T n = T(0);
-    for (int d = 0; d < D; ++d) {
-      const T z = zs[c * D + d];
-      n = n + z * z;
-    }
+    for (int i = 0; i < D; ++i) n = n + z[i] * z[i];
     n = dev_sqrt<T>(n);
-    massSh[c] = ((double)n > 1e-12) ? (double)n : 1e-12;
+    nsh = (T)(((double)n > 1e-12) ? (double)n : 1e-12);
   }
   __syncthreads();
-  if (k < K) {
-    T* e = enc + (size_t)img * K * D + (size_t)k * D;
-    const T n = (flags & 1) ? (T)massSh[c] : T(1);
-#pragma unroll
-    for (int j = 0; j < DPT; ++j) {
-      const int d = dg + 16 * j;
-      if (d >= D) continue;
-      const T z = zs[c * D + d];
-      e[d] = (flags & 1) ? z / n : z;
-    }
-  }
+  const T n = nsh;
+  for (int i = threadIdx.x; i < D; i += 64) e[i] = z[i] / n;
 }
 
 // vlad.c:230-241: enc /= max(||enc||, 1e-12), the squared norm summed in
@@ -347,7 +352,7 @@ __global__ __launch_bounds__(256) void vlad_normalize_kernel(T* __restrict__ enc
 // Hard assignments (one cluster per descriptor): workgroup (image, block of
 // CB clusters), thread d owns dimension d of every cluster of the block and
 // adds the block's descriptors in index order -- the same sequence of
-// roundings per (cluster, dimension) as vlad_accum_kernel with a one-hot q
+// roundings per (cluster, dimension) as vlad_slice_kernel with a one-hot q
 // (vlad.c:181-188), without scanning every cluster for every row.  With
 // q = 1 the reference's step enc = (TYPE)(enc + q * x), formed in double, is
 // the plain TYPE sum: for float, rounding the exact double sum of two floats
@@ -364,7 +369,8 @@ __global__ __launch_bounds__(256) void vlad_normalize_kernel(T* __restrict__ enc
 // stage in flight together.  Real assignments are skewed (a few clusters
 // hold many rows), so the workgroup of the largest clusters sets the time:
 // it pays one memory round trip per stage, not per row.
-// Finalisation as vlad_accum_kernel (vlad.c:191-227).  D <= 128, CB <= 256.
+// Finalisation as vlad_slice_kernel and vlad_component_norm_kernel
+// (vlad.c:191-227), in one workgroup.  D <= 128, CB <= 256.
 // The host picks CB so that even one image spreads over enough workgroups.
 constexpr int VH_CHUNK = 2048;
 
@@ -626,25 +632,24 @@ static void launch_vlad_hard(T* enc, const T* means, int D, int K, const T* x, c
   }
 }
 
-template <typename T, int DPT>
+template <typename T>
 static void launch_vlad(T* enc, const T* means, int D, int K, const T* x, const uint64_t* offsets, int numImages,
                         const T* assign, const uint32_t* hard, int flags) {
-  const int numKB = (K + VL_COMPS - 1) / VL_COMPS;
-  const size_t shmem = ((size_t)VL_ROWS * (D + 1) + VL_ROWS * VL_COMPS + (size_t)VL_COMPS * D) * sizeof(T);
-  const int padImages = numImages >= 8 ? ((numImages + 7) / 8) * 8 : numImages;
-  const unsigned blocks = (unsigned)((size_t)padImages * numKB);
   if (hard && D <= 128) {
     launch_vlad_hard<T>(enc, means, D, K, x, offsets, numImages, hard, flags);
-  } else if (hard) {
-    auto kern = vlad_accum_kernel<T, DPT, true>;
-    VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-    VLHIP_LAUNCH("vlad_accum", kern, dim3(blocks), dim3(256), shmem, x, offsets, numImages, D, K, means, assign,
-                 hard, flags, enc, numKB);
   } else {
-    auto kern = vlad_accum_kernel<T, DPT, false>;
-    VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-    VLHIP_LAUNCH("vlad_accum", kern, dim3(blocks), dim3(256), shmem, x, offsets, numImages, D, K, means, assign,
-                 hard, flags, enc, numKB);
+    const int numKB = (K + VS_COMPS - 1) / VS_COMPS, numDS = (D + VS_DIMS - 1) / VS_DIMS;
+    const int padImages = numImages >= 8 ? ((numImages + 7) / 8) * 8 : numImages;
+    const unsigned blocks = (unsigned)((size_t)padImages * numKB * numDS);
+    if (hard)
+      VLHIP_LAUNCH("vlad_slice", (vlad_slice_kernel<T, true>), dim3(blocks), dim3(256), 0, x, offsets, numImages, D,
+                   K, means, assign, hard, flags, enc, numKB, numDS);
+    else
+      VLHIP_LAUNCH("vlad_slice", (vlad_slice_kernel<T, false>), dim3(blocks), dim3(256), 0, x, offsets, numImages, D,
+                   K, means, assign, hard, flags, enc, numKB, numDS);
+    if (flags & 1)
+      VLHIP_LAUNCH("vlad_component_norm", (vlad_component_norm_kernel<T>), dim3((unsigned)((size_t)K * numImages)),
+                   dim3(64), 0, enc, D, K);
   }
   if (!(flags & 4)) {
     VLHIP_LAUNCH("vlad_normalize", (vlad_normalize_kernel<T>), dim3((unsigned)numImages), dim3(256), 0, enc,
@@ -659,14 +664,7 @@ void vlad_t(T* enc, const T* means, size_t D, size_t K, const T* x, const uint64
   VLHIP_REQUIRE((assign != nullptr) != (hard != nullptr), "exactly one of assignments / hardAssignments");
   VLHIP_REQUIRE(D <= 512, "GPU VLAD encoder supports dimension <= 512");
   if (numImages == 0) return;
-  const int d = (int)D, k = (int)K, n = (int)numImages;
-  if (D <= 16) launch_vlad<T, 1>(enc, means, d, k, x, offsets, n, assign, hard, flags);
-  else if (D <= 32) launch_vlad<T, 2>(enc, means, d, k, x, offsets, n, assign, hard, flags);
-  else if (D <= 64) launch_vlad<T, 4>(enc, means, d, k, x, offsets, n, assign, hard, flags);
-  else if (D <= 80) launch_vlad<T, 5>(enc, means, d, k, x, offsets, n, assign, hard, flags);
-  else if (D <= 128) launch_vlad<T, 8>(enc, means, d, k, x, offsets, n, assign, hard, flags);
-  else if (D <= 256) launch_vlad<T, 16>(enc, means, d, k, x, offsets, n, assign, hard, flags);
-  else launch_vlad<T, 32>(enc, means, d, k, x, offsets, n, assign, hard, flags);
+  launch_vlad<T>(enc, means, (int)D, (int)K, x, offsets, (int)numImages, assign, hard, flags);
 }
 
 template void vlad_t<float>(float*, const float*, size_t, size_t, const float*, const uint64_t*, size_t,
