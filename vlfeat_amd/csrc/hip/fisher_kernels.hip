@@ -165,21 +165,48 @@ __global__ __launch_bounds__(256) void fisher_accum_kernel(
     for (long long r0 = sBegin; r0 < sEnd; r0 += chunk) {
       const int nr = (int)min((long long)chunk, sEnd - r0);
       __syncthreads();
-      for (int idx = tid; idx < chunk * D; idx += 256) {
-        const int r = idx / D, d = idx - r * D;
-        xs[d * XS + r] = (r < nr) ? x[(r0 + r) * D + d] : T(0);
-      }
-      for (int j = 0; j < chunk / 4; ++j) {
-        const int r = sr + 4 * j;
-        T p = T(0);
-        if (r < nr && ksok) {
-          const long long row = r0 + r;
-          if (fast) p = (ks == row_arg[row]) ? T(1) : T(0);
-          else p = q[(row - qRowBase) * K + ks];
-          if ((double)p < 1e-6) p = T(0); else terms += 1;
+      // staging: each thread's loads are issued together (16 at a time), then stored
+      for (int base = 0; base < chunk * D; base += 256 * 16) {
+        T xv[16];
+#pragma unroll
+        for (int u2 = 0; u2 < 16; ++u2) {
+          const int idx = base + tid + 256 * u2;
+          const int r = idx / D;
+          xv[u2] = (idx < chunk * D && r < nr) ? x[(r0 + r) * D + (idx - r * D)] : T(0);
         }
-        qs[r][sc] = p;
-        s0acc += p;
+#pragma unroll
+        for (int u2 = 0; u2 < 16; ++u2) {
+          const int idx = base + tid + 256 * u2;
+          if (idx < chunk * D) {
+            const int r = idx / D;
+            xs[(idx - r * D) * XS + r] = xv[u2];
+          }
+        }
+      }
+      {
+        T pv[FA_ROWS / 4];
+#pragma unroll
+        for (int j = 0; j < FA_ROWS / 4; ++j) {
+          const int r = sr + 4 * j;
+          T p = T(0);
+          if (j < chunk / 4 && r < nr && ksok) {
+            const long long row = r0 + r;
+            if (fast) p = (ks == row_arg[row]) ? T(1) : T(0);
+            else p = q[(row - qRowBase) * K + ks];
+          }
+          pv[j] = p;
+        }
+#pragma unroll
+        for (int j = 0; j < FA_ROWS / 4; ++j) {
+          if (j >= chunk / 4) break;
+          const int r = sr + 4 * j;
+          T p = pv[j];
+          if (r < nr && ksok) {
+            if ((double)p < 1e-6) p = T(0); else terms += 1;
+          }
+          qs[r][sc] = p;
+          s0acc += p;
+        }
       }
       __syncthreads();
       for (int r = 0; r < nr; ++r) {
@@ -295,18 +322,28 @@ __global__ __launch_bounds__(256) void fisher_split_combine_kernel(
 #pragma unroll
   for (int c = 0; c < 4; ++c)
 #pragma unroll
-    for (int j = 0; j < DPT; ++j) {
-      const int k = k0 + kg * 4 + c, d = dg + 16 * j;
-      T su = T(0), sv = T(0);
-      if (k < K && d < D) {
-        for (int sp = 0; sp < nseg; ++sp) {
-          su += P0[sp * PS + (size_t)k * D + d];
-          sv += P0[sp * PS + (size_t)K * D + (size_t)k * D + d];
-        }
+    for (int j = 0; j < DPT; ++j) { u[c][j] = T(0); v[c][j] = T(0); }
+  // segment by segment (every load of a segment issued together), each sum
+  // from zero in segment order
+  for (int sp = 0; sp < nseg; ++sp) {
+    T pu[4][DPT], pv[4][DPT];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int j = 0; j < DPT; ++j) {
+        const int k = k0 + kg * 4 + c, d = dg + 16 * j;
+        const bool ok = k < K && d < D;
+        pu[c][j] = ok ? P0[sp * PS + (size_t)k * D + d] : T(0);
+        pv[c][j] = ok ? P0[sp * PS + (size_t)K * D + (size_t)k * D + d] : T(0);
       }
-      u[c][j] = su;
-      v[c][j] = sv;
-    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int j = 0; j < DPT; ++j) {
+        u[c][j] += pu[c][j];
+        v[c][j] += pv[c][j];
+      }
+  }
   if (tid < FA_COMPS) {
     T s0 = T(0);
     if (k0 + tid < K)
