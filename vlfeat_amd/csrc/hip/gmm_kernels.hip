@@ -100,17 +100,39 @@ __global__ __launch_bounds__(256) void estep_kernel(
     for (int d0 = 0; d0 < D; d0 += EM_DC) {
       const int dc = min(EM_DC, D - d0);
       __syncthreads();
-      for (int idx = tid; idx < ROWS * EM_DC; idx += 256) {
-        const int r = idx / EM_DC, dd = idx % EM_DC;
-        const long long row = row0 + r;
-        xs[dd][r] = (dd < dc && row < N) ? x[row * D + d0 + dd] : T(0);
-      }
-      for (int idx = tid; idx < EM_COLS * EM_DC; idx += 256) {
-        const int c = idx / EM_DC, dd = idx % EM_DC;
-        const int k = k0 + c;
-        const bool ok = (dd < dc && k < K);
-        ms[dd][c] = ok ? means[(size_t)k * D + d0 + dd] : T(0);
-        ss[dd][c] = ok ? inv[(size_t)k * D + d0 + dd] : T(0);
+      {
+        // every load of the step issued together, then stored
+        constexpr int XN = (ROWS * EM_DC + 255) / 256, MN = (EM_COLS * EM_DC + 255) / 256;
+        T xv[XN], mv[MN], iv[MN];
+#pragma unroll
+        for (int u = 0; u < XN; ++u) {
+          const int idx = tid + 256 * u;
+          const int r = idx / EM_DC, dd = idx % EM_DC;
+          const long long row = row0 + r;
+          xv[u] = (idx < ROWS * EM_DC && dd < dc && row < N) ? x[row * D + d0 + dd] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < MN; ++u) {
+          const int idx = tid + 256 * u;
+          const int c = idx / EM_DC, dd = idx % EM_DC;
+          const int k = k0 + c;
+          const bool ok = (idx < EM_COLS * EM_DC && dd < dc && k < K);
+          mv[u] = ok ? means[(size_t)k * D + d0 + dd] : T(0);
+          iv[u] = ok ? inv[(size_t)k * D + d0 + dd] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < XN; ++u) {
+          const int idx = tid + 256 * u;
+          if (idx < ROWS * EM_DC) xs[idx % EM_DC][idx / EM_DC] = xv[u];
+        }
+#pragma unroll
+        for (int u = 0; u < MN; ++u) {
+          const int idx = tid + 256 * u;
+          if (idx < EM_COLS * EM_DC) {
+            ms[idx % EM_DC][idx / EM_DC] = mv[u];
+            ss[idx % EM_DC][idx / EM_DC] = iv[u];
+          }
+        }
       }
       __syncthreads();
       for (int dd = 0; dd < dc; ++dd) {
