@@ -295,9 +295,22 @@ __global__ __launch_bounds__(256) void moments_kernel(
       }
       qs[r * (MS_COMPS + 1) + cs] = p;
     }
-    for (int idx = tid; idx < nr * D; idx += 256) {
-      const int r = idx / D, d = idx - r * D;
-      xs[r * (D + 1) + d] = x[(r0 + r) * D + d];
+    // descriptors: 8 loads per thread issued together, then stored
+    for (int base = 0; base < nr * D; base += 256 * 8) {
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int idx = base + tid + 256 * u;
+        v[u] = idx < nr * D ? x[r0 * D + idx] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int idx = base + tid + 256 * u;
+        if (idx < nr * D) {
+          const int r = idx / D;
+          xs[r * (D + 1) + (idx - r * D)] = v[u];
+        }
+      }
     }
     __syncthreads();
     for (int r = 0; r < nr; ++r) {
