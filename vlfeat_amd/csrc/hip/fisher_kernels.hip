@@ -376,8 +376,32 @@ __global__ __launch_bounds__(256) void fisher_normalize_kernel(T* __restrict__ e
   __syncthreads();
   const T n = nsh;
   T* e = enc + (size_t)img * len;
-  const size_t end = min(len, (size_t)(blockIdx.y + 1) * FN_SLICE);
-  for (size_t i = (size_t)blockIdx.y * FN_SLICE + threadIdx.x; i < end; i += blockDim.x) e[i] = e[i] / n;
+  const size_t s0 = (size_t)blockIdx.y * FN_SLICE;
+  const size_t end = min(len, s0 + FN_SLICE);
+  constexpr int V = 16 / sizeof(T);
+  typedef T vec __attribute__((ext_vector_type(V)));
+  if (len % V == 0 && ((uintptr_t)e & 15) == 0) {
+    // 16-byte pieces, every load of the thread issued before the first store
+    constexpr int U = FN_SLICE / V / 256;
+    vec* ev = reinterpret_cast<vec*>(e);
+    vec v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const size_t i = s0 / V + threadIdx.x + 256 * u;
+      if (i * V < end) v[u] = ev[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const size_t i = s0 / V + threadIdx.x + 256 * u;
+      if (i * V < end) {
+#pragma unroll
+        for (int t = 0; t < V; ++t) v[u][t] = v[u][t] / n;
+        ev[i] = v[u];
+      }
+    }
+  } else {
+    for (size_t i = s0 + threadIdx.x; i < end; i += blockDim.x) e[i] = e[i] / n;
+  }
 }
 
 template <typename T>
