@@ -681,8 +681,8 @@ def main() -> int:
                     help="also run a 1 GiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU layout (images split across GPUs, "
-                          "one all-reduce of EM statistics per iteration) is described in DESIGN.md, not built"}))
+        print(json.dumps({"error": "this library runs on one GPU; a multi-GPU layout (images split across GPUs, "
+                          "one all-reduce of EM statistics per iteration) is not built"}))
         return 2
     if args.config == "batched":
         return batched_config(args)
