@@ -504,21 +504,52 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
   static_assert(LG * GR * XS * sizeof(float) <= sizeof(wsh), "staging must fit the weight buffers");
   const bool vecx = (D & 3) == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0;
   float* xsh = reinterpret_cast<float*>(&wsh[0][0][0][0]) + w * GR * XS;
-  for (int i = lane; i < GR * (Dp / 4); i += 64) {
-    const int r = i / (Dp / 4), c4 = (i - r * (Dp / 4)) * 4;
-    const long long row = row0 + r;
-    f32x4 v = f32x4{0.f, 0.f, 0.f, 0.f};
-    if (row < rowEnd) {
-      if (vecx && c4 + 4 <= D) {
-        v = *reinterpret_cast<const f32x4*>(x + row * D + c4);
-      } else {
+  {
+    // every load of the lane issued before the first is used
+    constexpr int XW = GR * (Dp / 4);  // 16-byte words of the wave's rows
+    constexpr int XN = (XW + 63) / 64;
+    // (the choice of load width is uniform and taken outside the loop: one
+    // load per word, so that no load waits for an earlier one)
+    f32x4 xv[XN], gv[XN];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) v[e] = c4 + e < D ? x[row * D + c4 + e] : 0.f;
-      }
-      const f32x4 gg = *reinterpret_cast<const f32x4*>(g + c4);  // g holds Dp entries, zero past D
-      v = v - gg;
+    for (int u = 0; u < XN; ++u) {
+      const int i = lane + 64 * u;
+      const int c4 = (i - (i / (Dp / 4)) * (Dp / 4)) * 4;
+      gv[u] = i < XW ? *reinterpret_cast<const f32x4*>(g + c4) : f32x4{0.f, 0.f, 0.f, 0.f};  // g: Dp entries, zero past D
     }
-    *reinterpret_cast<f32x4*>(xsh + r * XS + c4) = v;
+    if (vecx) {
+#pragma unroll
+      for (int u = 0; u < XN; ++u) {
+        const int i = lane + 64 * u;
+        const int r = i / (Dp / 4), c4 = (i - r * (Dp / 4)) * 4;
+        const long long row = row0 + r;
+        f32x4 v = f32x4{0.f, 0.f, 0.f, 0.f};
+        if (i < XW && row < rowEnd && c4 < D) v = *reinterpret_cast<const f32x4*>(x + row * D + c4);
+        xv[u] = v;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < XN; ++u) {
+        const int i = lane + 64 * u;
+        const int r = i / (Dp / 4), c4 = (i - r * (Dp / 4)) * 4;
+        const long long row = row0 + r;
+        f32x4 v = f32x4{0.f, 0.f, 0.f, 0.f};
+        if (i < XW && row < rowEnd) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) v[e] = c4 + e < D ? x[row * D + c4 + e] : 0.f;
+        }
+        xv[u] = v;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < XN; ++u) {
+      const int i = lane + 64 * u;
+      if (i >= XW) continue;
+      const int r = i / (Dp / 4), c4 = (i - r * (Dp / 4)) * 4;
+      f32x4 v = xv[u];
+      if (row0 + r < rowEnd) v = v - gv[u];
+      *reinterpret_cast<f32x4*>(xsh + r * XS + c4) = v;
+    }
   }
   __syncthreads();
   // A fragments of this wave's 32 rows, resident for the whole sweep
