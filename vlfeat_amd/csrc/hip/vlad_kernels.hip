@@ -563,24 +563,35 @@ __global__ __launch_bounds__(128) void vlad_hard_kernel(const T* __restrict__ x,
   }
   for (int i = tid; i < CB; i += blockDim.x) mass[i] = (double)counts[i];
   __syncthreads();
-  // vlad.c:191-213 per (cluster, dimension), then the per-cluster norms
-  for (int i = tid; i < CB * D; i += blockDim.x) {
-    const int c = i / D, dd = i - c * D;
-    const int k = k0 + c;
-    if (k >= K) continue;
-    T e = acc[i];
-    const T mu = means[(size_t)k * D + dd];
-    const double ms = mass[c];
-    if (ms > 0) {
-      if (flags & 8) {
-        e = (T)((double)e / ms);
-        e = e - mu;
-      } else {
-        e = (T)((double)e - ms * (double)mu);
-      }
+  // vlad.c:191-213 per (cluster, dimension), then the per-cluster norms; the
+  // block's means are contiguous and 8 of a thread's are loaded together
+  const int nval = min(CB, K - k0) * D;
+  for (int base = 0; base < nval; base += 128 * 8) {
+    T mv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = base + tid + 128 * u;
+      mv[u] = i < nval ? means[(size_t)k0 * D + i] : T(0);
     }
-    if (flags & 2) e = signed_sqrt(e);
-    acc[i] = e;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = base + tid + 128 * u;
+      if (i >= nval) continue;
+      const int c = i / D;
+      T e = acc[i];
+      const T mu = mv[u];
+      const double ms = mass[c];
+      if (ms > 0) {
+        if (flags & 8) {
+          e = (T)((double)e / ms);
+          e = e - mu;
+        } else {
+          e = (T)((double)e - ms * (double)mu);
+        }
+      }
+      if (flags & 2) e = signed_sqrt(e);
+      acc[i] = e;
+    }
   }
   __syncthreads();
   if (flags & 1) {
