@@ -304,6 +304,56 @@ def test_vlad_hard_wide_descriptors(dt):
         assert rel(enc, oracle().vlad(x, mu, dense, flags)) < (5e-5 if dt == np.float32 else 1e-12)
 
 
+def _vlad_batch_case(seed, n_img, N_max, D, K):
+    rng = np.random.default_rng(seed)
+    counts = rng.integers(1, N_max, n_img)
+    counts[1], counts[2] = 0, 1
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint64)
+    x = rng.standard_normal((int(off[-1]), D)).astype(np.float32)
+    mu = rng.standard_normal((K, D)).astype(np.float32)
+    return rng, off, x, mu
+
+
+@pytest.mark.parametrize("K,n_img,D", [(512, 300, 80), (256, 80, 64), (37, 20, 200)])
+def test_vlad_batch_hard_blocks_equal_single_images(K, n_img, D):
+    # batches large enough for 32- and 16-cluster blocks (LDS accumulators; a
+    # single image uses 4-cluster blocks and register chains), and cluster
+    # indices with D > 128 (the dense-assignment kernel), with an empty and a
+    # one-row image: each image's encoding bit for bit as when encoded alone
+    from vlfeat_amd.gate import DeviceArray
+    rng, off, x, mu = _vlad_batch_case(K + n_img, n_img, 400, D, K)
+    a = rng.integers(0, K, x.shape[0]).astype(np.uint32)
+    a[::29] = 0xFFFFFFFF
+    L = va.lib()
+    dx, dm, doff, da = DeviceArray(x), DeviceArray(mu), DeviceArray(off), DeviceArray(a)
+    denc = DeviceArray(nbytes=4 * n_img * K * D)
+    for flags in (0, 3):
+        L.vlhip_vlad_encode_batch(1, denc.ptr, dm.ptr, D, K, dx.ptr, doff.ptr, n_img, None, da.ptr, flags)
+        enc = denc.to_host(np.float32, (n_img, K * D))
+        for i in sorted(set(list(range(0, n_img, max(1, n_img // 12))) + [1, 2, n_img - 1])):
+            e1 = va.vl_vlad_hard(x[off[i]:off[i + 1]], mu, a[off[i]:off[i + 1]], flags)
+            assert np.array_equal(enc[i], e1), (i, flags)
+
+
+def test_vlad_batch_soft_equals_single_images():
+    # dense (soft) assignments over 40 images: the image-major workgroup order
+    # of many-image launches gives the same encodings as single-image calls
+    from vlfeat_amd.gate import DeviceArray
+    K, D, n_img = 48, 80, 40
+    rng, off, x, mu = _vlad_batch_case(9, n_img, 300, D, K)
+    q = rng.uniform(size=(x.shape[0], K)).astype(np.float32) ** 4
+    q /= q.sum(1, keepdims=True)
+    L = va.lib()
+    dx, dm, doff, dq = DeviceArray(x), DeviceArray(mu), DeviceArray(off), DeviceArray(q)
+    denc = DeviceArray(nbytes=4 * n_img * K * D)
+    for flags in (0, 1, 15):
+        L.vlhip_vlad_encode_batch(1, denc.ptr, dm.ptr, D, K, dx.ptr, doff.ptr, n_img, dq.ptr, None, flags)
+        enc = denc.to_host(np.float32, (n_img, K * D))
+        for i in (0, 1, 2, 17, n_img - 1):
+            e1 = va.vl_vlad(x[off[i]:off[i + 1]], mu, q[off[i]:off[i + 1]], flags)
+            assert np.array_equal(enc[i], e1), (i, flags)
+
+
 def _f32_global_norm(z):
     """vlad.c:230-241 in float32, term by term: n = n + z*z, sqrt, max 1e-12."""
     n = np.float32(0)
