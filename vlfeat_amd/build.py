@@ -67,7 +67,7 @@ def build(verbose: bool = False) -> Path:
         jobs.append([hipcc, *HIPCC_FLAGS, f"-I{INCLUDE}", f"-I{HIP_SRC}", "-c", str(src), "-o", str(obj)])
     for src in sorted(HOST_SRC.glob("*.c")):
         obj = BUILD / (src.stem + ".c.o")
-        jobs.append(["gcc", *CC_FLAGS, f"-I{INCLUDE}", f"-I{HOST_SRC}", "-c", str(src), "-o", str(obj)])
+        jobs.append(["gcc", *CC_FLAGS, "-fopenmp", f"-I{INCLUDE}", f"-I{HOST_SRC}", "-c", str(src), "-o", str(obj)])
     workers = min(len(jobs), max(1, min(8, os.cpu_count() or 1)))
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
         for cmd, fut in [(c, ex.submit(_run, c)) for c in jobs]:
@@ -77,7 +77,7 @@ def build(verbose: bool = False) -> Path:
     objs = [j[-1] for j in jobs]
     tmp = LIB.with_suffix(".so.tmp")
     _run([hipcc, f"--offload-arch={OFFLOAD_ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs,
-          "-Wl,-Bsymbolic", "-lpthread", "-lm"])
+          "-Wl,-Bsymbolic", "-lgomp", "-lpthread", "-lm"])
     os.replace(tmp, LIB)
     return LIB
 

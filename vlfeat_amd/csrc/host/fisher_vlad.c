@@ -154,18 +154,27 @@ vl_fisher_encode_batch (void * enc, vl_type dataType,
 VLAD_ONEHOT_ROW(vlad_onehot_row_f, float)
 VLAD_ONEHOT_ROW(vlad_onehot_row_d, double)
 
+/* Rows are checked in parallel host threads (vl_get_max_threads(), as the
+ * reference sizes its OpenMP loops) when the matrix is large; the outcome does
+ * not depend on the thread count.  A thread stops checking after its first
+ * row that is not one-hot. */
 static int
 vlad_onehot_indices (vl_uint32 * out, vl_type dataType, void const * soft, vl_size numData, vl_size numClusters)
 {
-  vl_uindex i ;
+  long long i ;
+  int ok = 1 ;
+  int nt = (int) vl_get_max_threads() ;
   if (numClusters >= 0xffffffffu) return 0 ;
-  for (i = 0 ; i < numData ; ++i) {
-    int ok = (dataType == VL_TYPE_FLOAT)
-      ? vlad_onehot_row_f((float const *) soft + i * numClusters, numClusters, out + i)
-      : vlad_onehot_row_d((double const *) soft + i * numClusters, numClusters, out + i) ;
-    if (!ok) return 0 ;
+  if (nt < 1) nt = 1 ;
+#pragma omp parallel for schedule(static) reduction(&&:ok) num_threads(nt) if (numData * numClusters >= (1u << 18))
+  for (i = 0 ; i < (long long) numData ; ++i) {
+    if (ok) {
+      ok = (dataType == VL_TYPE_FLOAT)
+        ? vlad_onehot_row_f((float const *) soft + i * numClusters, numClusters, out + i)
+        : vlad_onehot_row_d((double const *) soft + i * numClusters, numClusters, out + i) ;
+    }
   }
-  return 1 ;
+  return ok ;
 }
 
 static void
