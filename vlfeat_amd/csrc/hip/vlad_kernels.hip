@@ -142,11 +142,12 @@ __global__ __launch_bounds__(64) void vlad_component_norm_kernel(T* __restrict__
 
 // vlad.c:230-241: enc /= max(||enc||, 1e-12), the squared norm summed in
 // index order (bitwise the reference's sequential loop).  The squares z*z
-// (each rounded as the reference rounds it) pass through two LDS buffers:
-// while wave 0 adds up one buffer, waves 1-3 fill the other.  Float sums take
-// the exact block form below (vlad_exact_chain); double sums run as one
+// (each rounded as the reference rounds it) pass through two LDS buffers.
+// Float sums take the exact block form below (vlad_exact_chain4) on all four
+// waves, the next buffer's loads in flight meanwhile; double sums run as one
 // lane's chain of dependent additions, fed by 16-byte LDS reads issued 8
-// ahead.  The division then reads the encoding again with independent loads.
+// ahead, while waves 1-3 fill the other buffer.  The division then reads the
+// encoding again with independent loads.
 template <typename T>
 struct VladNorm {
   static constexpr int CH = 16384 / sizeof(T);  // values per LDS buffer
@@ -207,7 +208,7 @@ __device__ __forceinline__ T vlad_norm_chain(T n, const T* __restrict__ sq, int 
 }
 
 // The same sum for float terms q >= 0 (or inf / NaN), bit for bit, a block
-// of 64 x VN_J terms at a time on one wave.  While the running sum S stays
+// of terms at a time.  While the running sum S stays
 // below 2^(e+1), where 2^e <= S (e = -126 for S below the normal range),
 // every float it takes is a multiple of u = 2^(e-23), and adding a term q
 // rounds S + q to S + u * k with k = q/u rounded to the nearest integer --
@@ -218,7 +219,7 @@ __device__ __forceinline__ T vlad_norm_chain(T n, const T* __restrict__ sq, int 
 // binade).  Otherwise the prefix before the first term that breaks one of
 // these is added the same way, that term is added as the reference adds it,
 // and the next block starts after it (the rare cases: a tie, and the few
-// times S crosses a power of two).
+// times S crosses a power of two).  vlad_exact_chain4 below runs this.
 constexpr int VN_J = 8;
 
 // Inclusive prefix sum over the 64 lanes of a wave: within each row of 16
@@ -235,31 +236,47 @@ __device__ __forceinline__ unsigned vn_wave_scan(unsigned v) {
   return v;
 }
 
-// The first `head` terms are added one by one: at the start of the sum S is
-// no larger than single terms and would leave its binade every few terms.
-__device__ __forceinline__ float vlad_exact_chain(float S, const float* __restrict__ sq, int m, int head, int lane) {
+// The block form above on the four waves of a workgroup: each step takes
+// 4 x 512 terms, wave w the w-th 512 (8 per lane).  The waves exchange their
+// integer totals through LDS; each then looks for its first term that cannot
+// take the block path, counting from the totals of the waves before it, and
+// the first such term of the step (if any) ends the step: the prefix before
+// it is added as a block, the term as the reference adds it, and the next
+// step starts after it.  The first `head` terms are added one by one (at the
+// start of the sum S is no larger than single terms and would leave its
+// binade every few terms).  Every thread computes the same S.  Slots of the
+// exchange alternate between steps, so that two barriers per step suffice.
+struct VnExchange {
+  unsigned tot[2][4];
+  int stop[2][4];
+  unsigned before[2][4];
+};
+__device__ __forceinline__ float vlad_exact_chain4(float S, const float* __restrict__ sq, int m, int head, int tid,
+                                                   VnExchange& x, int& step) {
 #pragma clang fp contract(off)
+  const int lane = tid & 63, w = tid >> 6;
 #pragma unroll 8
   for (int i = 0; i < head; ++i) S = S + sq[i];
   int p = head;
   while (p < m) {
     if (!(S < INFINITY)) {
-      // S is +inf or NaN: NaN if S or any later term is NaN, else +inf
       bool nan = S != S;
-      for (int i = p + lane; i < m; i += 64) nan |= sq[i] != sq[i];
-      return __ballot(nan) ? __builtin_nanf("") : S;
+      for (int i = p + tid; i < m; i += 256) nan |= sq[i] != sq[i];
+      return __syncthreads_or(nan) ? __builtin_nanf("") : S;
     }
+    const int par = step & 1;
+    ++step;
     int e = S == 0.f ? -126 : __builtin_amdgcn_frexp_expf(S) - 1;
     if (e < -126) e = -126;
     const float lim = ldexpf(1.f, e + 1);
-    const unsigned Sm = (unsigned)ldexpf(S, 23 - e);  // S / u, below 2^24
+    const unsigned Sm = (unsigned)ldexpf(S, 23 - e);
     unsigned k[VN_J];
-    int fj = VN_J;  // first term of this lane that cannot take the block path
+    int fj = VN_J;
 #pragma unroll
     for (int j = 0; j < VN_J; ++j) {
-      const int i = p + lane * VN_J + j;
+      const int i = p + 512 * w + lane * VN_J + j;
       const float q = i < m ? sq[i] : 0.f;
-      const bool ok = q < lim;  // false for inf, NaN and terms >= 2^(e+1)
+      const bool ok = q < lim;
       const float sc = ldexpf(ok ? q : 0.f, 23 - e);
       const float fl = floorf(sc);
       const float fr = sc - fl;
@@ -272,9 +289,12 @@ __device__ __forceinline__ float vlad_exact_chain(float S, const float* __restri
     const unsigned Lc = L < (1u << 25) ? L : (1u << 25);
     const unsigned incl = vn_wave_scan(Lc);
     const unsigned excl = incl - Lc;
-    // first term at which the running integer reaches 2^24 (S leaves the
-    // binade), and the prefix before this lane's first stop
-    unsigned run = Sm + (excl < (1u << 25) ? excl : (1u << 25));
+    const unsigned wt = (unsigned)__builtin_amdgcn_readlane((int)incl, 63);
+    if (lane == 0) x.tot[par][w] = wt < (1u << 25) ? wt : (1u << 25);
+    __syncthreads();
+    unsigned base = 0;
+    for (int v = 0; v < w; ++v) base += x.tot[par][v];
+    unsigned run = Sm + (base + excl < (1u << 25) ? base + excl : (1u << 25));
     int f = fj;
     unsigned pre = 0;
 #pragma unroll
@@ -286,21 +306,54 @@ __device__ __forceinline__ float vlad_exact_chain(float S, const float* __restri
       }
     }
     const unsigned long long stop = __ballot(f < VN_J);
-    if (stop == 0) {
-      S = ldexpf((float)(Sm + (unsigned)__builtin_amdgcn_readlane((int)incl, 63)), e - 23);
-      p += 64 * VN_J;
-      continue;
+    const int sl = stop ? __builtin_ctzll(stop) : 0;
+    const int fsl = __builtin_amdgcn_readlane(f, sl);
+    const unsigned bsl = (unsigned)__builtin_amdgcn_readlane((int)(base + excl + pre), sl);
+    if (lane == 0) {
+      x.stop[par][w] = stop ? sl * VN_J + fsl : 512;
+      x.before[par][w] = bsl;
     }
-    const int fl_lane = __builtin_ctzll(stop);
-    const int fjx = __builtin_amdgcn_readlane(f, fl_lane);
-    const unsigned before = (unsigned)__builtin_amdgcn_readlane((int)(excl + pre), fl_lane);
-    S = ldexpf((float)(Sm + before), e - 23);
-    const int fi = p + fl_lane * VN_J + fjx;
-    S = S + sq[fi];
-    p = fi + 1;
+    __syncthreads();
+    int W = 4;
+    for (int v = 3; v >= 0; --v)
+      if (x.stop[par][v] < 512) W = v;
+    if (W == 4) {
+      unsigned tot = 0;
+      for (int v = 0; v < 4; ++v) tot += x.tot[par][v];
+      S = ldexpf((float)(Sm + tot), e - 23);
+      p += 4 * 512;
+    } else {
+      S = ldexpf((float)(Sm + x.before[par][W]), e - 23);
+      const int fi = p + 512 * W + x.stop[par][W];
+      S = S + sq[fi];
+      p = fi + 1;
+    }
   }
   return S;
 }
+
+// Loads of vlad_norm_stage (issued) and its stores (squares), apart, so that
+// the next chunk's loads are in flight while the current one is summed.
+template <typename T, int NT>
+struct VnPrefetch {
+  static constexpr int PER = (VladNorm<T>::CH + NT - 1) / NT;
+  T v[PER];
+  __device__ __forceinline__ void load(const T* __restrict__ e, int m, int t) {
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int i = t + NT * j;
+      v[j] = i < m ? e[i] : T(0);
+    }
+  }
+  __device__ __forceinline__ void store(T* __restrict__ sq, int t) const {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int i = t + NT * j;
+      if (i < VladNorm<T>::CH) sq[i] = v[j] * v[j];
+    }
+  }
+};
 
 template <typename T>
 __global__ __launch_bounds__(256) void vlad_normalize_kernel(T* __restrict__ enc, size_t len) {
@@ -314,18 +367,31 @@ __global__ __launch_bounds__(256) void vlad_normalize_kernel(T* __restrict__ enc
   vlad_norm_stage<T, 256>(e, (int)min((size_t)CH, len), sq[0], tid);
   __syncthreads();
   T n = T(0);
-  for (size_t c = 0; c < nch; ++c) {
-    if (tid < 64) {
+  if constexpr (sizeof(T) == 4) {
+    // all four waves sum each chunk; the next chunk's loads are in flight meanwhile
+    __shared__ VnExchange xch;
+    int step = 0;
+    VnPrefetch<T, 256> pf;
+    for (size_t c = 0; c < nch; ++c) {
       const int m = (int)min((size_t)CH, len - c * CH);
-      if constexpr (sizeof(T) == 4)
-        n = vlad_exact_chain(n, reinterpret_cast<const float*>(sq[c & 1]), m, c == 0 ? min(m, 64) : 0, tid);
-      else if (tid == 0)
-        n = vlad_norm_chain<T>(n, sq[c & 1], m);
-    } else if (c + 1 < nch) {
-      const size_t c0 = (c + 1) * CH;
-      vlad_norm_stage<T, 192>(e + c0, (int)min((size_t)CH, len - c0), sq[(c + 1) & 1], tid - 64);
+      if (c + 1 < nch) pf.load(e + (c + 1) * CH, (int)min((size_t)CH, len - (c + 1) * CH), tid);
+      n = vlad_exact_chain4(n, reinterpret_cast<const float*>(sq[c & 1]), m, c == 0 ? min(m, 64) : 0, tid, xch,
+                            step);
+      // sq[(c + 1) & 1] was last read while summing chunk c - 1, before the
+      // barriers of chunk c's steps (chunk c >= 1 has at least one step)
+      if (c + 1 < nch) pf.store(sq[(c + 1) & 1], tid);
+      __syncthreads();
     }
-    __syncthreads();
+  } else {
+    for (size_t c = 0; c < nch; ++c) {
+      if (tid < 64) {
+        if (tid == 0) n = vlad_norm_chain<T>(n, sq[c & 1], (int)min((size_t)CH, len - c * CH));
+      } else if (c + 1 < nch) {
+        const size_t c0 = (c + 1) * CH;
+        vlad_norm_stage<T, 192>(e + c0, (int)min((size_t)CH, len - c0), sq[(c + 1) & 1], tid - 64);
+      }
+      __syncthreads();
+    }
   }
   if (tid == 0) {
     n = dev_sqrt<T>(n);
