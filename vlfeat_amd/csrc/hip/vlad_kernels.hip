@@ -375,7 +375,7 @@ __global__ __launch_bounds__(256) void vlad_normalize_kernel(T* __restrict__ enc
     for (size_t c = 0; c < nch; ++c) {
       const int m = (int)min((size_t)CH, len - c * CH);
       if (c + 1 < nch) pf.load(e + (c + 1) * CH, (int)min((size_t)CH, len - (c + 1) * CH), tid);
-      n = vlad_exact_chain4(n, reinterpret_cast<const float*>(sq[c & 1]), m, c == 0 ? min(m, 64) : 0, tid, xch,
+      n = vlad_exact_chain4(n, reinterpret_cast<const float*>(sq[c & 1]), m, c == 0 ? min(m, 1024) : 0, tid, xch,
                             step);
       // sq[(c + 1) & 1] was last read while summing chunk c - 1, before the
       // barriers of chunk c's steps (chunk c >= 1 has at least one step)
