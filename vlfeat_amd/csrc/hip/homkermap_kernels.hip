@@ -84,6 +84,7 @@ void vlhip_homkermap_evaluate(int dtype, void* out, void const* in, size_t n, do
   std::lock_guard<std::recursive_mutex> lk(ctx().mu);
   VLHIP_REQUIRE(order >= 0 && numSubdivisions >= 1 && minExponent < maxExponent, "invalid homogeneous kernel map");
   if (n == 0) return;
+  VLHIP_REQUIRE((n + 255) / 256 < (size_t(1) << 31), "too many values for one homogeneous kernel map launch");
   const double sub = 1.0 / (double)numSubdivisions;  // the map's subdivision, as the reference computes it
   if (dtype == 1)
     homkermap_launch<float>((const float*)in, n, table, order, numSubdivisions, sub, minExponent, maxExponent,

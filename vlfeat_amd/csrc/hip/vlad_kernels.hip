@@ -686,6 +686,7 @@ static void launch_vlad_hard_cb(T* enc, const T* means, int D, int K, const T* x
                                 int numImages, const uint32_t* hard, int flags) {
   const int nkb = (K + CB - 1) / CB;
   const int padImages = numImages >= 8 ? ((numImages + 7) / 8) * 8 : numImages;
+  VLHIP_REQUIRE((size_t)padImages * nkb < (size_t(1) << 31), "GPU VLAD encoder: too many (image, cluster block) pairs");
   const size_t sh = (((size_t)CB * D * sizeof(T) + 15) & ~size_t(15)) + CB * sizeof(double) + CB * sizeof(T);
   auto kern = vlad_hard_kernel<T, CB>;
   VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
@@ -717,6 +718,8 @@ static void launch_vlad(T* enc, const T* means, int D, int K, const T* x, const 
   } else {
     const int numKB = (K + VS_COMPS - 1) / VS_COMPS, numDS = (D + VS_DIMS - 1) / VS_DIMS;
     const int padImages = numImages >= 8 ? ((numImages + 7) / 8) * 8 : numImages;
+    VLHIP_REQUIRE((size_t)padImages * numKB * numDS < (size_t(1) << 31) && (size_t)K * numImages < (size_t(1) << 31),
+                  "GPU VLAD encoder: too many (image, cluster block, dimension slice) workgroups");
     const unsigned blocks = (unsigned)((size_t)padImages * numKB * numDS);
     if (hard)
       VLHIP_LAUNCH("vlad_slice", (vlad_slice_kernel<T, true>), dim3(blocks), dim3(256), 0, x, offsets, numImages, D,
@@ -740,6 +743,7 @@ void vlad_t(T* enc, const T* means, size_t D, size_t K, const T* x, const uint64
   VLHIP_REQUIRE(K >= 1 && D >= 1, "numClusters and dimension must be positive");
   VLHIP_REQUIRE((assign != nullptr) != (hard != nullptr), "exactly one of assignments / hardAssignments");
   VLHIP_REQUIRE(D <= 512, "GPU VLAD encoder supports dimension <= 512");
+  VLHIP_REQUIRE(K < (size_t(1) << 31) && numImages < (size_t(1) << 31), "GPU VLAD encoder: K or image count too large");
   if (numImages == 0) return;
   launch_vlad<T>(enc, means, (int)D, (int)K, x, offsets, (int)numImages, assign, hard, flags);
 }
