@@ -54,9 +54,14 @@ KERNEL_FLOP_PER_DESC = {
     "fisher_estep": 4 * DIM * K_COMP,  # FP32 E-step
 }
 # Matrix-core flop actually issued per descriptor and component by the
-# compensated kernels (three products for the logits, two for the moments,
-# one for the posterior mass), relative to the algorithmic count.
-EXECUTED_PER_ALGORITHMIC = {"mfma_fisher": (3 * 2 * DIM + 2 * 2 * DIM + 32) / (8 * DIM), "mfma_lse": 3.0}
+# compensated kernels, relative to the algorithmic count:
+#   mfma_fisher: logits, three products over the 2D features (3 x 2 x 2D flop),
+#                moments, two products (hi, lo) for each of x' and x'^2
+#                (2 x 2 x 2D), posterior mass, one 16x16x32 product with ones
+#                per 16 components (32 flop); algorithmic 8D
+#   mfma_lse:    logits only, 3 x 2 x 2D against the algorithmic 4D
+EXECUTED_PER_ALGORITHMIC = {"mfma_fisher": (3 * 2 * 2 * DIM + 2 * 2 * 2 * DIM + 32) / (8 * DIM),
+                            "mfma_lse": (3 * 2 * 2 * DIM) / (4 * DIM)}
 
 
 def log(msg: str) -> None:
@@ -786,6 +791,23 @@ def main() -> int:
         ex = achieved * EXECUTED_PER_ALGORITHMIC[kname]
         roofline["executed_matrix_tflops"] = ex
         roofline["executed_frac"] = ex / peak
+    # every matrix-core kernel of the step, and the step as a whole (8DK
+    # algorithmic flop per descriptor over the whole step's wall time)
+    per_kernel = {}
+    for k, (kt_ms, kt_n) in timings.items():
+        if k not in KERNEL_FLOP_PER_DESC or not kt_n:
+            continue
+        k_avg = kt_ms / kt_n
+        k_desc = N / (kt_n / args.steps)
+        k_tf = KERNEL_FLOP_PER_DESC[k] * k_desc / (k_avg / 1000) / 1e12
+        e = {"avg_ms": k_avg, "launches_per_step": kt_n / args.steps, "achieved_tflops": k_tf, "frac": k_tf / peak}
+        if k in EXECUTED_PER_ALGORITHMIC:
+            e["executed_frac"] = k_tf * EXECUTED_PER_ALGORITHMIC[k] / peak
+        per_kernel[k] = e
+    roofline["per_kernel"] = per_kernel
+    step_tf = 8 * DIM * K_COMP * N / (ms_per_step / 1000) / 1e12
+    roofline["step_achieved_tflops"] = step_tf
+    roofline["step_frac"] = step_tf / peak
 
     t_host = terms.cpu().numpy()
     result = {

@@ -304,6 +304,27 @@ def test_vlad_hard_wide_descriptors(dt):
         assert rel(enc, oracle().vlad(x, mu, dense, flags)) < (5e-5 if dt == np.float32 else 1e-12)
 
 
+@needs_ref
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_vlad_dimension_1024_dense_and_hard(dt):
+    # D above the per-cluster norm's LDS stage (512 values): the norm is summed
+    # and divided from global memory in the same order
+    rng = np.random.default_rng(1024)
+    n, D, K = 700, 1024, 9
+    x = rng.standard_normal((n, D)).astype(dt)
+    mu = rng.standard_normal((K, D)).astype(dt)
+    a = rng.integers(0, K, n).astype(np.uint32)
+    dense = np.zeros((n, K), dt)
+    dense[np.arange(n), a] = 1
+    soft = rng.uniform(0, 1, (n, K)).astype(dt)
+    soft[soft < 0.6] = 0
+    R = reference()
+    for flags in (1, 15):
+        assert np.array_equal(va.vl_vlad_hard(x, mu, a, flags), R.vlad(x, mu, dense, flags)), flags
+        assert np.array_equal(va.vl_vlad(x, mu, dense, flags), R.vlad(x, mu, dense, flags)), flags
+        assert np.array_equal(va.vl_vlad(x, mu, soft, flags), R.vlad(x, mu, soft, flags)), flags
+
+
 def _vlad_batch_case(seed, n_img, N_max, D, K):
     rng = np.random.default_rng(seed)
     counts = rng.integers(1, N_max, n_img)

@@ -81,3 +81,36 @@ def test_plan_replays_match_eager(K, flags, vflags):
     for i in (0, 1, 2):
         eo, _ = oracle().fisher(x[off[i]:off[i + 1]], m, c, p, flags)
         assert rel(enc[i], eo) < 2e-4
+
+
+def test_plan_replay_survives_library_table_growth():
+    # the plan's quantizer table is its own: a library call that grows the
+    # library's quantizer table between replays (more centres, wider
+    # descriptors) leaves the replayed encodings unchanged bit for bit
+    L = va.lib()
+    K, D = 96, 80
+    counts = np.array([700, 1, 2500, 64, 1300], np.int64)
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint64)
+    n = len(counts)
+    x = synth.sift_like(int(off[-1]), D, seed=21)
+    m, c, p = gmm_params(x, K, seed=22)
+    dx, dm, dc, dp = (DeviceArray(a) for a in (x, m, c, p))
+    denc = DeviceArray(nbytes=4 * n * 2 * K * D)
+    dvlad = DeviceArray(nbytes=4 * n * K * D)
+    dterms = DeviceArray(nbytes=8 * n)
+    plan = L.vlhip_encode_plan_new(D, K, off.ctypes.data, n, 3, 1)
+    try:
+        L.vlhip_encode_plan_run(plan, dx.ptr, dm.ptr, dc.ptr, dp.ptr, denc.ptr, dvlad.ptr, dterms.ptr)
+        e0 = denc.to_host(np.float32, (n, 2 * K * D))
+        v0 = dvlad.to_host(np.float32, (n, K * D))
+        for Kb, Db in ((1000, 120), (2048, 128)):
+            xb = synth.sift_like(5000, Db, seed=Kb, project=Db != 128)
+            va.vl_kmeans_quantize(xb, xb[:Kb])
+            L.vlhip_encode_plan_run(plan, dx.ptr, dm.ptr, dc.ptr, dp.ptr, denc.ptr, dvlad.ptr, dterms.ptr)
+            assert np.array_equal(denc.to_host(np.float32, (n, 2 * K * D)), e0), Kb
+            assert np.array_equal(dvlad.to_host(np.float32, (n, K * D)), v0), Kb
+    finally:
+        L.vlhip_encode_plan_delete(plan)
+    assign, _ = va.vl_kmeans_quantize(x, m)
+    for i in range(n):
+        assert np.array_equal(v0[i], va.vl_vlad_hard(x[off[i]:off[i + 1]], m, assign[off[i]:off[i + 1]], 1)), i

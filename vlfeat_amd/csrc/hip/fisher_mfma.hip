@@ -1373,6 +1373,7 @@ struct EncodePlan {
   unsigned char* dTables = nullptr;
   double* dNorm = nullptr;
   uint32_t* dAssign = nullptr;
+  unsigned char* dQtab = nullptr;  // centre table of the screened quantizer (hard VLAD), plan-owned
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1385,7 +1386,8 @@ struct EncodePlan {
   float* dProjX = nullptr;  // totalRows x D, the projected descriptors
 };
 
-// autotuner cache: (D, K, flags, rows bucket) -> variant
+// autotuner cache: (D, K, Fisher flags, VLAD flags, load-stage input dimension,
+// renormalisation, rows bucket) -> variant
 static std::vector<std::pair<std::vector<long long>, int>>& tuning_cache() {
   static std::vector<std::pair<std::vector<long long>, int>> c;
   return c;
@@ -1408,7 +1410,8 @@ static void plan_enqueue(EncodePlan& P, const float* x, const float* means, cons
                                         P.dNorm, numTerms, numKB, nullptr, nullptr, 0, shmem);
   if (P.flags & 2) launch_fisher_normalize(enc, P.dNorm, numKB, P.numImages, (size_t)2 * P.K * P.D);
   if (P.vladFlags >= 0 && vladEnc) {
-    quantize_t<float>(P.dAssign, nullptr, x, means, P.totalRows, P.D, P.K, 8);
+    // the plan's own centre table: no library buffer is captured into the graph
+    quantize_t<float>(P.dAssign, nullptr, x, means, P.totalRows, P.D, P.K, 8, P.dQtab);
     vlad_t<float>(vladEnc, means, P.D, P.K, x, P.dOff, P.numImages, nullptr, P.dAssign, P.vladFlags);
   }
 }
@@ -1726,7 +1729,11 @@ void* vlhip_encode_plan_new(size_t D, size_t K, uint64_t const* offsets_host, si
                                                                               : (int)((K + 255) / 256) * 256;
   VLHIP_CHECK(hipMalloc(&P->dTables, mfma_tables_bytes<_Float16>(P->NS * 16, Kp)));
   VLHIP_CHECK(hipMalloc(&P->dNorm, numImages * (size_t)(Kp / 128) * sizeof(double)));
-  if (vladFlags >= 0) VLHIP_CHECK(hipMalloc(&P->dAssign, (P->totalRows ? P->totalRows : 1) * sizeof(uint32_t)));
+  if (vladFlags >= 0) {
+    VLHIP_CHECK(hipMalloc(&P->dAssign, (P->totalRows ? P->totalRows : 1) * sizeof(uint32_t)));
+    const size_t qb = quantize_screened_bytes(D, K);
+    if (qb) VLHIP_CHECK(hipMalloc(&P->dQtab, qb));
+  }
   VLHIP_CHECK(hipMemcpy(P->dOff, offsets_host, (numImages + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
   VLHIP_CHECK(hipMemcpy(P->dGstart, gs.data(), (numImages + 1) * sizeof(long long), hipMemcpyHostToDevice));
   VLHIP_CHECK(hipMemcpy(P->dGtab, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice));
@@ -1742,7 +1749,7 @@ void vlhip_encode_plan_delete(void* plan) {
   if (P->stream) VLHIP_CHECK(hipStreamSynchronize(P->stream));
   if (P->exec) VLHIP_CHECK(hipGraphExecDestroy(P->exec));
   if (P->graph) VLHIP_CHECK(hipGraphDestroy(P->graph));
-  void* bufs[] = {P->dOff, P->dGstart, P->dGtab, P->dRec, P->dTables, P->dNorm, P->dAssign,
+  void* bufs[] = {P->dOff, P->dGstart, P->dGtab, P->dRec, P->dTables, P->dNorm, P->dAssign, P->dQtab,
                   P->dProjP, P->dProjC, P->dProjX};
   for (void* b : bufs)
     if (b) VLHIP_CHECK(hipFree(b));
@@ -1766,6 +1773,7 @@ void vlhip_encode_plan_run(void* plan, void const* data, void const* means, void
   if (P->variant < 0) {
     // autotuner: reuse a cached choice for this shape, else time both tilings
     std::vector<long long> shape = {(long long)P->D, (long long)P->K, (long long)P->flags,
+                                    (long long)P->vladFlags, (long long)P->Din, (long long)P->renorm,
                                     (long long)(64 - __builtin_clzll((unsigned long long)P->totalRows | 1))};
     for (auto& e : tuning_cache())
       if (e.first == shape) P->variant = e.second;
@@ -1802,8 +1810,10 @@ void vlhip_encode_plan_run(void* plan, void const* data, void const* means, void
     P->graph = nullptr;
     ctx().timing = false;  // no event records inside the capture
     VLHIP_CHECK(hipStreamBeginCapture(P->stream, hipStreamCaptureModeRelaxed));
+    ctx().capturing = true;  // scratch() refuses to run: nothing library-owned enters the graph
     plan_enqueue_any(*P, P->variant, x, (const float*)means, (const float*)covs, (const float*)priors,
                      (float*)fisherEnc, (float*)vladEnc, numTerms);
+    ctx().capturing = false;
     VLHIP_CHECK(hipStreamEndCapture(P->stream, &P->graph));
     VLHIP_CHECK(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
     for (int i = 0; i < 7; ++i) P->key[i] = key[i];

@@ -247,12 +247,13 @@ __global__ __launch_bounds__(256) void ann_merge_kernel(const T* __restrict__ x,
 }
 
 template <typename T>
-void quantize_t(uint32_t* assign, T* dist, const T* x, const T* centers, size_t N, size_t D, size_t K, int lanes) {
+void quantize_t(uint32_t* assign, T* dist, const T* x, const T* centers, size_t N, size_t D, size_t K, int lanes,
+                void* ws) {
   VLHIP_REQUIRE(K >= 1 && D >= 1, "numCenters and dimension must be positive");
   if (N == 0) return;
   if constexpr (std::is_same<T, float>::value) {
     if ((lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8) &&
-        quantize_screened(assign, dist, x, centers, N, D, K, lanes))
+        quantize_screened(assign, dist, x, centers, N, D, K, lanes, ws))
       return;
   }
   const unsigned blocks = (unsigned)((N + QZ_ROWS - 1) / QZ_ROWS);
@@ -286,8 +287,9 @@ void l2_to_point_t(T* out, const T* x, const T* point, size_t N, size_t D, int l
   }
 }
 
-template void quantize_t<float>(uint32_t*, float*, const float*, const float*, size_t, size_t, size_t, int);
-template void quantize_t<double>(uint32_t*, double*, const double*, const double*, size_t, size_t, size_t, int);
+template void quantize_t<float>(uint32_t*, float*, const float*, const float*, size_t, size_t, size_t, int, void*);
+template void quantize_t<double>(uint32_t*, double*, const double*, const double*, size_t, size_t, size_t, int,
+                                 void*);
 
 template <typename T>
 void ann_merge_t(uint32_t* assign, T* dist, const uint32_t* prev, const T* x, const T* centers, size_t N,

@@ -325,15 +325,28 @@ __global__ __launch_bounds__(QM_WAVES * 64) void qm_quantize_kernel(
   }
 }
 
+// Bytes of the centre table [chi | clo | cc | max bits] of a screened shape
+// (0 when the shape is not screened).
+size_t quantize_screened_bytes(size_t D, size_t K) {
+  if (D > 128 || K < 64) return 0;
+  const size_t Dq = (D + 31) / 32 * 32;
+  const size_t Kp = (K + QM_BLOCK - 1) / QM_BLOCK * QM_BLOCK;
+  const size_t bW = (Kp * Dq * sizeof(__bf16) + 255) & ~size_t(255);
+  const size_t bC = (Kp * sizeof(float) + 255) & ~size_t(255);
+  return 2 * bW + bC + 256;
+}
+
 bool quantize_screened(uint32_t* assign, float* dist, const float* x, const float* centers, size_t N, size_t D,
-                       size_t K, int lanes) {
+                       size_t K, int lanes, void* ws) {
   if (D > 128 || K < 64 || N == 0) return false;
   const int NQ = (int)((D + 31) / 32);
   const int Dq = NQ * 32;
   const int Kp = (int)((K + QM_BLOCK - 1) / QM_BLOCK) * QM_BLOCK;
   const size_t bW = ((size_t)Kp * Dq * sizeof(__bf16) + 255) & ~size_t(255);
   const size_t bC = ((size_t)Kp * sizeof(float) + 255) & ~size_t(255);
-  unsigned char* base = (unsigned char*)scratch(SLOT_QTAB, 2 * bW + bC + 256);
+  // the caller's workspace (encode plans, the no-allocation entry point) or the
+  // library's grow-only table
+  unsigned char* base = ws ? (unsigned char*)ws : (unsigned char*)scratch(SLOT_QTAB, 2 * bW + bC + 256);
   __bf16* chi = (__bf16*)base;
   __bf16* clo = (__bf16*)(base + bW);
   float* cc = (float*)(base + 2 * bW);

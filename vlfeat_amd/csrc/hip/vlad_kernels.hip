@@ -120,24 +120,36 @@ __global__ __launch_bounds__(256) void vlad_slice_kernel(
 
 // vlad.c:215-227: each cluster's D values divided by max(||.||, 1e-12), the
 // squared norm summed over d in order by one lane.  Workgroup per (image,
-// cluster), in encoding order.
+// cluster), in encoding order.  Clusters of up to VN_STAGE values are staged
+// in LDS; longer ones are summed and divided straight from global memory
+// (same additions in the same order).
+constexpr int VN_STAGE = 512;
 template <typename T>
 __global__ __launch_bounds__(64) void vlad_component_norm_kernel(T* __restrict__ enc, int D, int K) {
 #pragma clang fp contract(off)
-  __shared__ T z[512];
+  __shared__ T z[VN_STAGE];
   __shared__ T nsh;
   T* e = enc + (size_t)blockIdx.x * D;
-  for (int i = threadIdx.x; i < D; i += 64) z[i] = e[i];
+  const bool staged = D <= VN_STAGE;
+  if (staged)
+    for (int i = threadIdx.x; i < D; i += 64) z[i] = e[i];
   __syncthreads();
   if (threadIdx.x == 0) {
     T n = T(0);
-    for (int i = 0; i < D; ++i) n = n + z[i] * z[i];
+    if (staged) {
+      for (int i = 0; i < D; ++i) n = n + z[i] * z[i];
+    } else {
+      for (int i = 0; i < D; ++i) {
+        const T v = e[i];
+        n = n + v * v;
+      }
+    }
     n = dev_sqrt<T>(n);
     nsh = (T)(((double)n > 1e-12) ? (double)n : 1e-12);
   }
   __syncthreads();
   const T n = nsh;
-  for (int i = threadIdx.x; i < D; i += 64) e[i] = z[i] / n;
+  for (int i = threadIdx.x; i < D; i += 64) e[i] = (staged ? z[i] : e[i]) / n;
 }
 
 // vlad.c:230-241: enc /= max(||enc||, 1e-12), the squared norm summed in
@@ -742,7 +754,7 @@ void vlad_t(T* enc, const T* means, size_t D, size_t K, const T* x, const uint64
             const T* assign, const uint32_t* hard, int flags) {
   VLHIP_REQUIRE(K >= 1 && D >= 1, "numClusters and dimension must be positive");
   VLHIP_REQUIRE((assign != nullptr) != (hard != nullptr), "exactly one of assignments / hardAssignments");
-  VLHIP_REQUIRE(D <= 512, "GPU VLAD encoder supports dimension <= 512");
+  VLHIP_REQUIRE(D < (size_t(1) << 24), "GPU VLAD encoder: dimension too large");
   VLHIP_REQUIRE(K < (size_t(1) << 31) && numImages < (size_t(1) << 31), "GPU VLAD encoder: K or image count too large");
   if (numImages == 0) return;
   launch_vlad<T>(enc, means, (int)D, (int)K, x, offsets, (int)numImages, assign, hard, flags);

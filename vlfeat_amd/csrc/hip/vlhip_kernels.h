@@ -34,9 +34,12 @@ double run_estep(const T* x, size_t N, size_t D, size_t K, const T* means, const
                  T* q, T* row_m, T* row_s, int* row_arg, const char* name, bool want_ll);
 
 // Exact float quantizer screened on the matrix cores (kmeans_mfma.hip);
-// returns false (nothing launched) when the shape is not supported.
+// returns false (nothing launched) when the shape is not supported.  ws: a
+// device workspace of quantize_screened_bytes(D, K) bytes, or nullptr for the
+// library's own grow-only table.
+size_t quantize_screened_bytes(size_t D, size_t K);
 bool quantize_screened(uint32_t* assign, float* dist, const float* x, const float* centers, size_t N, size_t D,
-                       size_t K, int lanes);
+                       size_t K, int lanes, void* ws);
 
 // Divides every image's encoding by max(||enc||, 1e-12) (fisher.c:474-485).
 void launch_fisher_normalize(float* enc, const double* normpart, int numKB, size_t numImages, size_t len);
@@ -60,9 +63,11 @@ template <typename T>
 void vlad_t(T* enc, const T* means, size_t D, size_t K, const T* x, const uint64_t* offsets, size_t numImages,
             const T* assign, const uint32_t* hard, int flags);
 
-// Exact nearest-centre quantizer (kmeans_kernels.hip); dist may be nullptr.
+// Exact nearest-centre quantizer (kmeans_kernels.hip); dist may be nullptr;
+// ws as for quantize_screened.
 template <typename T>
-void quantize_t(uint32_t* assign, T* dist, const T* x, const T* centers, size_t N, size_t D, size_t K, int lanes);
+void quantize_t(uint32_t* assign, T* dist, const T* x, const T* centers, size_t N, size_t D, size_t K, int lanes,
+                void* ws = nullptr);
 
 // Pre-encoding projection (project_kernels.hip): out = P (x - c), each row
 // optionally renormalised to unit L2 norm; float, outDimension <= 128.

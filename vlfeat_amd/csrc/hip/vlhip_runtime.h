@@ -78,6 +78,7 @@ struct Context {
   void* buf[SLOT_COUNT] = {};
   size_t cap[SLOT_COUNT] = {};
   bool timing = false;
+  bool capturing = false;  // an encode plan's graph is being captured on its stream
   std::vector<KernelTiming> timings;
   std::vector<PendingEvent> pending;
   std::vector<hipEvent_t> event_pool;

@@ -44,6 +44,10 @@ void ensure_init() {
 
 void* scratch(Slot s, size_t bytes) {
   Context& c = g_ctx;
+  // a captured graph keeps the pointers it was recorded with, and a library
+  // buffer may be freed and reallocated by any later call
+  if (c.capturing)
+    fatal("scratch", __FILE__, __LINE__, "a library work buffer was requested while an encode plan is captured");
   if (bytes == 0) bytes = 256;
   if (c.cap[s] < bytes) {
     if (c.buf[s]) {
