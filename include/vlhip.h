@@ -213,6 +213,19 @@ VLHIP_API void vlhip_kmeans_quantize (int dtype, uint32_t * assignments, void * 
                                       void const * data, void const * centers,
                                       size_t numData, size_t dimension, size_t numCenters, int lanes) ;
 
+/* vlhip_kmeans_quantize without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * hipGraph capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as vlhip_kmeans_quantize; the matrix-core screen's
+ * centre table lives in a static device array (2.0 MiB per GPU, held until the
+ * process ends), so float data with dimension <= 128 takes at most 4096
+ * centers (more aborts with a message); other shapes use the exhaustive
+ * kernel, which needs no work buffer.  Every call on a GPU shares that GPU's
+ * table: calls that overlap on one GPU race. */
+VLHIP_API void vlhip_kmeans_quantize_noalloc (int dtype, uint32_t * assignments, void * distances,
+                                              void const * data, void const * centers,
+                                              size_t numData, size_t dimension, size_t numCenters, int lanes) ;
+
 /* Squared L2 distance of every datum to one point, same lane order
  * (used by k-means++ seeding, kmeans.c:610-616). */
 VLHIP_API void vlhip_l2_to_point (int dtype, void * distances, void const * data, void const * point,

@@ -148,6 +148,8 @@ _EXT_SIGS = {
     "vlhip_vlad_encode_batch": (None, [C.c_int, _p, _p, C.c_size_t, C.c_size_t, _p, _p, C.c_size_t,
                                        _p, _p, C.c_int]),
     "vlhip_kmeans_quantize": (None, [C.c_int, _p, _p, _p, _p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_int]),
+    "vlhip_kmeans_quantize_noalloc": (None, [C.c_int, _p, _p, _p, _p, C.c_size_t, C.c_size_t, C.c_size_t,
+                                             C.c_int]),
     "vlhip_encode_plan_new": (_p, [_sz, _sz, _p, _sz, C.c_int, C.c_int]),
     "vlhip_encode_plan_run": (None, [_p, _p, _p, _p, _p, _p, _p, _p]),
     "vlhip_encode_plan_info": (None, [_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),

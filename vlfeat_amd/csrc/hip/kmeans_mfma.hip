@@ -327,13 +327,13 @@ __global__ __launch_bounds__(QM_WAVES * 64) void qm_quantize_kernel(
 
 // Bytes of the centre table [chi | clo | cc | max bits] of a screened shape
 // (0 when the shape is not screened).
+static constexpr size_t qm_table_bytes(size_t D, size_t K) {
+  return 2 * (((K + QM_BLOCK - 1) / QM_BLOCK * QM_BLOCK * ((D + 31) / 32 * 32) * 2 + 255) & ~size_t(255)) +
+         (((K + QM_BLOCK - 1) / QM_BLOCK * QM_BLOCK * sizeof(float) + 255) & ~size_t(255)) + 256;
+}
 size_t quantize_screened_bytes(size_t D, size_t K) {
   if (D > 128 || K < 64) return 0;
-  const size_t Dq = (D + 31) / 32 * 32;
-  const size_t Kp = (K + QM_BLOCK - 1) / QM_BLOCK * QM_BLOCK;
-  const size_t bW = (Kp * Dq * sizeof(__bf16) + 255) & ~size_t(255);
-  const size_t bC = (Kp * sizeof(float) + 255) & ~size_t(255);
-  return 2 * bW + bC + 256;
+  return qm_table_bytes(D, K);
 }
 
 bool quantize_screened(uint32_t* assign, float* dist, const float* x, const float* centers, size_t N, size_t D,
@@ -376,4 +376,67 @@ bool quantize_screened(uint32_t* assign, float* dist, const float* x, const floa
   return true;
 }
 
+// ---------------------------------------------------------------------------
+// No-allocation form of the quantizer, for callers that cannot allocate device
+// memory (code under graph capture, serving loops).  The screened kernel's
+// centre table lives in a static device array, one copy per GPU, held until
+// the process ends; its device address is looked up once per device.  Every
+// call on a GPU shares that GPU's table, so calls that overlap on one GPU race.
+constexpr size_t kNoAllocMaxCenters = 4096;
+constexpr size_t kNoAllocTableBytes = qm_table_bytes(128, kNoAllocMaxCenters);  // 2.0 MiB
+
 }  // namespace vlhip
+
+static __device__ __align__(256) unsigned char noalloc_scratch[vlhip::kNoAllocTableBytes];
+
+namespace vlhip {
+
+constexpr int kNoAllocMaxDevices = 64;
+static void* g_noalloc_table[kNoAllocMaxDevices] = {};
+
+// Device address of this GPU's table (looked up on the first call on the
+// device, with the device current).
+static void* noalloc_table() {
+  int dev = 0;
+  VLHIP_CHECK(hipGetDevice(&dev));
+  VLHIP_REQUIRE(dev >= 0 && dev < kNoAllocMaxDevices, "device index out of range");
+  if (!g_noalloc_table[dev]) {
+    void* p = nullptr;
+    VLHIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)));
+    g_noalloc_table[dev] = p;
+  }
+  return g_noalloc_table[dev];
+}
+
+}  // namespace vlhip
+
+using namespace vlhip;
+
+extern "C" void vlhip_kmeans_quantize_noalloc(int dtype, uint32_t* assign, void* dist, void const* data,
+                                              void const* centers, size_t N, size_t D, size_t K, int lanes) {
+  // argument checks come first: they need no device
+  VLHIP_REQUIRE(dtype == 1 || dtype == 2, "unknown data type");
+  VLHIP_REQUIRE(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8, "lanes must be 1, 2, 4 or 8");
+  VLHIP_REQUIRE(K >= 1 && D >= 1, "numCenters and dimension must be positive");
+  const bool screened = dtype == 1 && quantize_screened_bytes(D, K) != 0;
+  VLHIP_REQUIRE(!screened || K <= kNoAllocMaxCenters,
+                "vlhip_kmeans_quantize_noalloc: float data with dimension <= 128 takes at most 4096 centers "
+                "(the size of its static table)");
+  ensure_init();
+  std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+  // no timing events: the launches below create nothing
+  const bool savedTiming = ctx().timing;
+  ctx().timing = false;
+  if (dtype == 1) {
+    if (screened && N > 0) {
+      quantize_screened(assign, (float*)dist, (const float*)data, (const float*)centers, N, D, K, lanes,
+                        noalloc_table());
+    } else {
+      // the exhaustive kernel, which uses no work buffer
+      quantize_t<float>(assign, (float*)dist, (const float*)data, (const float*)centers, N, D, K, lanes);
+    }
+  } else {
+    quantize_t<double>(assign, (double*)dist, (const double*)data, (const double*)centers, N, D, K, lanes);
+  }
+  ctx().timing = savedTiming;
+}
