@@ -1,17 +1,22 @@
 """Gated Fisher encoder (MODE_GATED) against the dense F16X3 encoder and the
 FP64 oracle.
 
+The exact first pass judges the gate: a group is served sparsely only when,
+in every row, the gate's 16 candidates carry an exact posterior mass (their
+logits normalised by the first pass's log-sum-exp over all components) of at
+least `need`; the network's own predicted mass only pre-screens rows.
+
 * With a gate that is never confident every group takes the dense path, and
   the encodings equal the F16X3 encoder's bit for bit.
-* On a mixture with peaked posteriors a trained gate serves most groups with
-  exact posteriors over its 16 candidates (normalised over them).  The error
-  this truncation adds depends on the mass the gate must predict before a
-  group skips the dense E-step (need); it is printed for 0.99, 0.999 and
-  0.9999.  At 0.999 the Fisher vectors stay within 1e-3 relative L2 of the
-  FP64 oracle and the term counts within 1%, with at least a fifth of the
-  groups served by the gate (at 0.99 more groups are served but the dropped
-  tail mass, amplified by the 1/sqrt(prior) factors, exceeds the bound).
+* On a mixture with peaked posteriors a trained gate serves groups at need
+  0.999 with every image (the maximum over the images, not a median) within
+  1e-3 relative L2 of the FP64 oracle; the served fraction and the worst
+  image are printed for 0.99, 0.999 and 0.9999.
+* Over 100 images of 3,000 descriptors (more than the gate's probe) the worst
+  image stays within 1e-3 of the FP64 oracle at need 0.999.
 """
+import concurrent.futures as cf
+
 import numpy as np
 import pytest
 
@@ -94,14 +99,16 @@ def test_gated_peaked_mixture_vs_fp64(peaked, flags):
     err, dt, frac = worst[0.999]
     assert err <= 1e-3
     assert dt <= 0.01
-    assert frac >= 0.2
+    assert frac >= 0.1
+    # a stricter bar never serves more groups
+    assert worst[0.9999][2] <= worst[0.999][2] <= worst[0.99][2]
 
 
-def test_gated_probe_large_batch(peaked):
+def test_gated_100_images_worst_image(peaked):
     # a batch larger than the gate's probe: a gate that serves nothing on the
     # probe is not run on the rest (dense path, bit for bit F16X3); a useful
-    # gate keeps serving the whole batch.  Over 100 images the truncation
-    # error is printed per operating point (median and worst image).
+    # gate keeps serving the whole batch, and the worst of 100 images stays
+    # within 1e-3 of the FP64 oracle
     _, m, c, p, g = peaked
     x = synth.mixture(300000, 32, 96, seed=11)
     off = (np.arange(101) * 3000).astype(np.uint64)
@@ -114,8 +121,12 @@ def test_gated_probe_large_batch(peaked):
     finally:
         va.set_fisher_gate(None)
     assert np.array_equal(eg, e16) and np.array_equal(tg, t16)
+    O = oracle()
+    with cf.ThreadPoolExecutor(8) as ex:
+        ref = list(ex.map(lambda i: O.fisher(x[off[i]:off[i + 1]], m, c, p, 3)[0], range(100)))
+    dense = np.array([rel(e16[i], ref[i]) for i in range(100)])
     stats = {}
-    for need in (0.999, 0.9999, 0.99999):
+    for need in (0.999, 0.9999):
         va.set_fisher_gate(g, 16, need)
         va.fisher_gate_stats(reset=True)
         try:
@@ -124,10 +135,11 @@ def test_gated_probe_large_batch(peaked):
         finally:
             va.set_fisher_gate(None)
         groups, served = va.fisher_gate_stats(reset=True)
-        errs = np.array([rel(eg[i], e16[i]) for i in range(100)])
-        stats[need] = (served / groups, float(np.median(errs)), float(errs.max()))
-        print(f"large batch need={need}: served {served}/{groups} groups, rel diff to F16X3 median "
-              f"{np.median(errs):.2e}, 90th pct {np.percentile(errs, 90):.2e}, max {errs.max():.2e}")
-    assert stats[0.999][0] >= 0.2          # the gate stays on for the whole batch
-    assert stats[0.999][1] <= 1e-3         # typical image within the bound
-    assert stats[0.99999][2] <= stats[0.999][2]
+        errs = np.array([rel(eg[i], ref[i]) for i in range(100)])
+        stats[need] = (served / groups, float(errs.max()))
+        print(f"100 images need={need}: served {served}/{groups} groups, rel err to FP64 median "
+              f"{np.median(errs):.2e}, 90th pct {np.percentile(errs, 90):.2e}, max {errs.max():.2e} "
+              f"(dense F16X3 max {dense.max():.2e})")
+    assert stats[0.999][0] >= 0.1          # the gate stays on for the whole batch
+    assert stats[0.999][1] <= 1e-3         # every image within the bound
+    assert stats[0.9999][1] <= 1e-3

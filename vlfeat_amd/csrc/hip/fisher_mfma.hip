@@ -694,21 +694,27 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
     }
 }
 
-// Gated groups.  One wave per group: the group is served sparsely when the
-// gate is confident for every one of its rows; then, for each row, the exact
-// logits of its M <= 16 candidate components (the same FP32 tables as the
-// dense path: bias + log2 e * sum_d sigma^-2 (mu' x' - x'^2 / 2)) are
-// normalised over the candidates, thresholded at 1e-6 (fisher.c:436-441; FAST:
-// one-hot on the first largest), scaled by sq and listed as
-// {component, q * sq} in list[group][32][M] (padding rows: {-1, 0}).  The
-// terms are counted into numTerms of the group's image.
+// Gated groups.  One wave per group, after the first pass has written every
+// group's exact row statistics (-max logit, sq / sum over all K components).
+// For each row the exact logits of its M <= 16 candidate components (the same
+// FP32 tables as the dense path: bias + log2 e * sum_d sigma^-2 (mu' x' - x'^2 / 2))
+// give exact posteriors q = 2^(logit - max) / sum -- normalised over all
+// components by the first pass, not over the candidates -- and their mass.
+// The group is served sparsely only if every row passed the network's
+// pre-screen (conf) and the candidates' exact mass of every row is >= need;
+// with FAST encodings the largest candidate must also exceed 1 - mass, so that
+// it is the row's true first maximum.  A served group lists {component, q * sq}
+// in list[group][32][M] (q < 1e-6 dropped, fisher.c:436-441; padding rows:
+// {-1, 0}) and counts its terms into numTerms of its image; the posterior mass
+// outside the candidates, at most 1 - need per row, is dropped.
 template <int NS>
 __global__ __launch_bounds__(256) void mfma_sparse_kernel(
     const float* __restrict__ x, int D, long long numGroups, const int4* __restrict__ gtab,
     const unsigned char* __restrict__ conf, const int* __restrict__ cand, int M, const float* __restrict__ bias,
     const float* __restrict__ mup, const float* __restrict__ sinv, const float* __restrict__ g,
-    const MfmaScales* __restrict__ scp, int flags, unsigned char* __restrict__ gflag, int2* __restrict__ list,
-    unsigned long long* __restrict__ numTerms) {
+    const MfmaScales* __restrict__ scp, const unsigned char* __restrict__ rec, float need, int flags,
+    unsigned char* __restrict__ gflag, int2* __restrict__ list, unsigned long long* __restrict__ numTerms) {
+  typedef GroupRec<NS> R;
   constexpr int Dp = NS * 16;
   const int lane = threadIdx.x & 63;
   const long long grp = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -716,21 +722,28 @@ __global__ __launch_bounds__(256) void mfma_sparse_kernel(
   const int4 gr = gtab[grp];
   const long long row0 = gr.x;
   const int nrow = min(GR, gr.y - gr.x);
+  // the network's pre-screen: a row it does not expect to be served is not tried
   const bool rowOk = lane >= nrow || conf[row0 + lane] != 0;
-  const bool sparse = __all(rowOk) != 0;
-  if (lane == 0) gflag[grp] = sparse ? 1 : 0;
-  if (!sparse) return;
+  if (__all(rowOk) == 0) {
+    if (lane == 0) gflag[grp] = 0;
+    return;
+  }
+  const float* Sf = reinterpret_cast<const float*>(rec + (size_t)grp * R::kBytes + R::kA + R::kB);
   const bool fast = (flags & 4) != 0;
   const float sq = scp->sq;
-  const float qmin = __uint_as_float(0x358637beu);  // next float above float(1e-6)
+  const float invsq = 1.0f / sq;
+  const float qminS = __uint_as_float(0x358637beu) * sq;  // next float above float(1e-6), scaled
   const float kL2e = 1.4426950408889634f;
   const int j = lane & 15, sub = lane >> 4;
-  unsigned terms = 0;
+  float qv[GR / 4];  // scaled exact posterior q * sq of (row 4 it + sub, candidate j)
+  int cv[GR / 4];
+  bool ok = true;
+#pragma unroll
   for (int it = 0; it < GR / 4; ++it) {
     const int r = 4 * it + sub;
     const bool valid = r < nrow && j < M;
     const int comp = valid ? cand[(row0 + r) * M + j] : -1;
-    float l = -INFINITY;
+    float v = 0.f;
     if (valid && comp >= 0) {
       const float* xr = x + (row0 + r) * D;
       const float* mu = mup + (size_t)comp * Dp;
@@ -740,29 +753,45 @@ __global__ __launch_bounds__(256) void mfma_sparse_kernel(
         const float xc = xr[d] - g[d];
         acc += si[d] * (mu[d] * xc - 0.5f * xc * xc);
       }
-      l = bias[comp] + kL2e * acc;
+      // 2^(logit - max) * sq / sum: the row statistics of the first pass
+      v = __builtin_amdgcn_exp2f(bias[comp] + kL2e * acc + Sf[r]) * Sf[GR + r];
     }
-    // normalise over the candidates of the row (lanes 16 sub .. 16 sub + 15)
-    float m = l;
+    qv[it] = v;
+    cv[it] = valid ? comp : -1;
+    // certified mass and largest posterior of the row (lanes 16 sub .. 16 sub + 15)
+    float ms = v, mx = v;
 #pragma unroll
-    for (int o = 1; o < 16; o <<= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-    const float e = (l == -INFINITY) ? 0.f : __builtin_amdgcn_exp2f(l - m);
-    float s = e;
+    for (int o = 1; o < 16; o <<= 1) {
+      ms += __shfl_xor(ms, o, 64);
+      mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    }
+    const float mass = ms * invsq;
+    const bool rowServed = r >= nrow || (mass >= need && (!fast || mx * invsq > 1.0f - mass));
+    ok = ok && rowServed;
+  }
+  const bool served = __all(ok) != 0;
+  if (lane == 0) gflag[grp] = served ? 1 : 0;
+  if (!served) return;
+  unsigned terms = 0;
 #pragma unroll
-    for (int o = 1; o < 16; o <<= 1) s += __shfl_xor(s, o, 64);
-    float q = s > 0.f ? e / s : 0.f;
+  for (int it = 0; it < GR / 4; ++it) {
+    const int r = 4 * it + sub;
+    const int comp = cv[it];
+    float v = qv[it];
     if (fast) {
-      int best = (l == m && l != -INFINITY) ? comp : 0x7fffffff;
+      // one-hot on the first largest candidate (value, then lowest index)
+      float mx = v;
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+      int best = (comp >= 0 && v == mx && v > 0.f) ? comp : 0x7fffffff;
 #pragma unroll
       for (int o = 1; o < 16; o <<= 1) best = min(best, __shfl_xor(best, o, 64));
-      q = (comp == best) ? 1.f : 0.f;
+      v = (comp >= 0 && comp == best) ? sq : 0.f;
+    } else if (!(v >= qminS)) {
+      v = 0.f;
     }
-    float v = 0.f;
-    if (valid && comp >= 0 && q >= qmin) {
-      v = q * sq;
-      terms += 1;
-    }
-    if (j < M) list[((size_t)grp * GR + r) * M + j] = make_int2(valid ? comp : -1, __float_as_int(v));
+    if (comp >= 0 && v > 0.f) terms += 1;
+    if (j < M) list[((size_t)grp * GR + r) * M + j] = make_int2(comp, __float_as_int(v));
   }
   if (numTerms) {
     unsigned t = terms;
@@ -1168,29 +1197,42 @@ static void mfma_tables_launch(const MfmaTables<H>& t, const float* means, const
                Kp, t.g, t.scp, t.Whi, t.Wlo, t.bias, t.mup, t.isg, t.sinv);
 }
 
-// Both passes over one batch of images [b0, b0 + n) whose descriptors start at
-// xb (nG groups, group table gtab with batch-relative rows).
-template <typename H, int NS, int CT, int WAVES>
-static void mfma_batch_launch(const MfmaTables<H>& t, const float* xb, int D, int K, int Kp, int flags,
-                              unsigned char* rec, size_t nG, const int4* gtab, const uint64_t* offsets,
-                              const long long* gstart, int b0, int n, const float* priors, float* enc,
-                              double* normpart, unsigned long long* numTerms, int numKB,
-                              const unsigned char* gflag, const int2* list, int M, size_t shmem) {
-  if (nG > 0) {
-    constexpr int LG = LseGroups<NS>::value;
-    const unsigned lb = (unsigned)((nG + LG - 1) / LG);
-    if (flags & 4) {
-      VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, true>), dim3(lb), dim3(LG * 64), 0, xb, D, Kp, t.Whi,
-                   t.Wlo, t.bias, t.scp, t.g, rec, (long long)nG, gtab, gflag);
-    } else {
-      VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, false>), dim3(lb), dim3(LG * 64), 0, xb, D, Kp, t.Whi,
-                   t.Wlo, t.bias, t.scp, t.g, rec, (long long)nG, gtab, gflag);
-    }
+// The two passes over one batch of images [b0, b0 + n) whose descriptors start
+// at xb (nG groups, group table gtab with batch-relative rows).
+template <typename H, int NS>
+static void mfma_first_pass(const MfmaTables<H>& t, const float* xb, int D, int Kp, int flags, unsigned char* rec,
+                            size_t nG, const int4* gtab) {
+  if (nG == 0) return;
+  constexpr int LG = LseGroups<NS>::value;
+  const unsigned lb = (unsigned)((nG + LG - 1) / LG);
+  if (flags & 4) {
+    VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, true>), dim3(lb), dim3(LG * 64), 0, xb, D, Kp, t.Whi, t.Wlo,
+                 t.bias, t.scp, t.g, rec, (long long)nG, gtab, (const unsigned char*)nullptr);
+  } else {
+    VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, false>), dim3(lb), dim3(LG * 64), 0, xb, D, Kp, t.Whi, t.Wlo,
+                 t.bias, t.scp, t.g, rec, (long long)nG, gtab, (const unsigned char*)nullptr);
   }
+}
+
+template <typename H, int NS, int CT, int WAVES>
+static void mfma_second_pass(const MfmaTables<H>& t, int D, int K, int flags, unsigned char* rec,
+                             const uint64_t* offsets, const long long* gstart, int b0, int n, const float* priors,
+                             float* enc, double* normpart, unsigned long long* numTerms, int numKB,
+                             const unsigned char* gflag, const int2* list, int M, size_t shmem) {
   const int padImages = n >= 8 ? ((n + 7) / 8) * 8 : n;
   VLHIP_LAUNCH("mfma_fisher", (mfma_fisher_kernel<float, H, NS, CT, WAVES>), dim3((unsigned)((size_t)padImages * numKB)),
                dim3(WAVES * 64), shmem, rec, offsets, gstart, b0, n, D, K, t.Whi, t.Wlo, t.bias, t.scp, t.mup,
                t.isg, t.sinv, priors, flags, enc, normpart, numTerms, numKB, gflag, list, M, 0.f, nullptr);
+}
+
+template <typename H, int NS, int CT, int WAVES>
+static void mfma_batch_launch(const MfmaTables<H>& t, const float* xb, int D, int K, int Kp, int flags,
+                              unsigned char* rec, size_t nG, const int4* gtab, const uint64_t* offsets,
+                              const long long* gstart, int b0, int n, const float* priors, float* enc,
+                              double* normpart, unsigned long long* numTerms, int numKB, size_t shmem) {
+  mfma_first_pass<H, NS>(t, xb, D, Kp, flags, rec, nG, gtab);
+  mfma_second_pass<H, NS, CT, WAVES>(t, D, K, flags, rec, offsets, gstart, b0, n, priors, enc, normpart, numTerms,
+                                     numKB, nullptr, nullptr, 0, shmem);
 }
 
 template <typename H, int NS, int CT, int WAVES>
@@ -1241,51 +1283,62 @@ static void run_mfma(float* enc, const float* means, const float* covs, const fl
       gtab = (int4*)scratch(SLOT_GROUPTAB, nG * sizeof(int4));
       VLHIP_LAUNCH("mfma_group_table", mfma_group_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                    offsets, gstart, (int)b0, n, gtab);
-      if (gu && gateOff) ctx().gate_groups += nG;
-      if (gu && !gateOff) {
-        // gate: candidates and confidence per row, then the groups it serves
+    }
+    // first pass over every group: the exact row statistics, which the second
+    // pass normalises with and which judge the gate's candidates
+    mfma_first_pass<H, NS>(t, x + rowBase * D, (int)D, Kp, flags, rec, nG, gtab);
+    if (nG > 0 && gu) {
+      ctx().gate_groups += nG;
+      if (!gateOff) {
+        // gate: candidates and the network's pre-screen per row, then the
+        // groups whose candidates carry an exact mass >= need in every row
         int* cand = (int*)scratch(SLOT_GATE, (size_t)rows * gu->M * sizeof(int));
         unsigned char* gb = (unsigned char*)scratch(SLOT_GATE2, al256(rows * sizeof(float)) + al256(rows) + al256(nG));
         float* mass = (float*)gb;
         unsigned char* conf = gb + al256(rows * sizeof(float));
         gflag = conf + al256(rows);
+        list = (int2*)scratch(SLOT_GATELIST, nG * GR * gu->M * sizeof(int2));
+        auto sparse = [&](unsigned long long* terms) {
+          VLHIP_LAUNCH("mfma_sparse", mfma_sparse_kernel<NS>, dim3((unsigned)((nG + 3) / 4)), dim3(256), 0,
+                       x + rowBase * D, (int)D, (long long)nG, gtab, conf, cand, gu->M, t.bias, t.mup, t.sinv, t.g,
+                       t.scp, rec, gu->need, flags, gflag, list, terms);
+        };
         // The gate first runs on a probe of the batch's first rows; when it
-        // serves fewer than 1% of the probe's groups, the rest of the batch
-        // takes the dense path without running the network (every group it
-        // does not serve is dense anyway, so only the gate's cost changes).
+        // serves fewer than 1% of the groups that lie inside the probe, the
+        // rest of the batch (and of the call) takes the dense path without
+        // running the network: only the gate's cost changes.
         const size_t probe = std::min<size_t>(rows, kGateProbeRows);
         vlhip_gate_select(gu->gate, x + rowBase * D, probe, gu->M, gu->need, cand, mass, conf);
+        bool on = true;
         if (probe < rows) {
-          std::vector<unsigned char> hc(probe);
-          download(hc.data(), conf, probe);
-          size_t inProbe = 0, confident = 0;
+          VLHIP_CHECK(hipMemsetAsync(conf + probe, 0, rows - probe, ctx().stream));
+          sparse(nullptr);
+          std::vector<unsigned char> hf(nG);
+          download(hf.data(), gflag, nG);
+          size_t inProbe = 0, servedProbe = 0, gi = 0;
           for (size_t i = b0; i < b1; ++i) {
             const uint64_t a = off[i] - rowBase, e = off[i + 1] - rowBase;
-            for (uint64_t r = a; r < e && r + GR <= probe; r += GR) {
-              const uint64_t re = std::min<uint64_t>(e, r + GR);
-              bool all = true;
-              for (uint64_t q = r; q < re && all; ++q) all = hc[q] != 0;
+            for (uint64_t r = a; r < e; r += GR, ++gi) {
+              if (std::min<uint64_t>(e, r + GR) > probe) continue;
               ++inProbe;
-              confident += all ? 1 : 0;
+              servedProbe += hf[gi];
             }
           }
-          if (inProbe >= 64 && confident * 100 < inProbe) {
+          if (inProbe >= 64 && servedProbe * 100 < inProbe) {
+            on = false;
             gateOff = true;
-            VLHIP_CHECK(hipMemsetAsync(conf + probe, 0, rows - probe, ctx().stream));
           } else {
             vlhip_gate_select(gu->gate, x + (rowBase + probe) * D, rows - probe, gu->M, gu->need,
                               cand + probe * gu->M, mass + probe, conf + probe);
           }
         }
-        list = (int2*)scratch(SLOT_GATELIST, nG * GR * gu->M * sizeof(int2));
-        VLHIP_LAUNCH("mfma_sparse", mfma_sparse_kernel<NS>, dim3((unsigned)((nG + 3) / 4)), dim3(256), 0,
-                     x + rowBase * D, (int)D, (long long)nG, gtab, conf, cand, gu->M, t.bias, t.mup, t.sinv, t.g,
-                     t.scp, flags, gflag, list, numTerms);
-        std::vector<unsigned char> hf(nG);
-        download(hf.data(), gflag, nG);
         size_t served = 0;
-        for (size_t i = 0; i < nG; ++i) served += hf[i];
-        ctx().gate_groups += nG;
+        if (on) {
+          sparse(numTerms);
+          std::vector<unsigned char> hf(nG);
+          download(hf.data(), gflag, nG);
+          for (size_t i = 0; i < nG; ++i) served += hf[i];
+        }
         ctx().gate_served += served;
         if (served == 0) {
           // every group of the batch is dense: the plain dense kernels, which give
@@ -1295,9 +1348,8 @@ static void run_mfma(float* enc, const float* means, const float* covs, const fl
         }
       }
     }
-    mfma_batch_launch<H, NS, CT, WAVES>(t, x + rowBase * D, (int)D, (int)K, Kp, flags, rec, nG, gtab, offsets, gstart,
-                                        (int)b0, n, priors, enc, normpart, numTerms, numKB, gflag, list,
-                                        gu ? gu->M : 0, shmem);
+    mfma_second_pass<H, NS, CT, WAVES>(t, (int)D, (int)K, flags, rec, offsets, gstart, (int)b0, n, priors, enc,
+                                       normpart, numTerms, numKB, gflag, list, gu ? gu->M : 0, shmem);
     b0 = b1;
   }
   if (flags & 2) {
@@ -1407,7 +1459,7 @@ static void plan_enqueue(EncodePlan& P, const float* x, const float* means, cons
   for (const PlanBatch& b : P.batches)
     mfma_batch_launch<H, NS, CT, WAVES>(t, x + b.rowBase * P.D, (int)P.D, (int)P.K, Kp, P.flags, P.dRec, b.nG,
                                         P.dGtab + b.gOff, P.dOff, P.dGstart, (int)b.b0, (int)b.n, priors, enc,
-                                        P.dNorm, numTerms, numKB, nullptr, nullptr, 0, shmem);
+                                        P.dNorm, numTerms, numKB, shmem);
   if (P.flags & 2) launch_fisher_normalize(enc, P.dNorm, numKB, P.numImages, (size_t)2 * P.K * P.D);
   if (P.vladFlags >= 0 && vladEnc) {
     // the plan's own centre table: no library buffer is captured into the graph
