@@ -128,6 +128,20 @@ VLHIP_API void vlhip_fisher_encode_batch (int dtype, void * enc,
                                           void const * data, uint64_t const * offsets, size_t numImages,
                                           int flags, int mode, unsigned long long * numTerms) ;
 
+/* Choice between the two matrix-core Fisher encoders (F16X3 / BF16X3 modes):
+ * the two-pass encoder (first pass: row log-sum-exp over all components;
+ * second pass: posteriors and moments per component block) and the fused
+ * single-pass encoder (the component blocks of an image exchange per-row
+ * partial log-sum-exps, so every logit is computed once; one cooperative
+ * launch, float data, dimension <= 80, 129 to 2048 components).  mode 0:
+ * always two-pass; 1 (default): fused when the batch has at least one image
+ * per stream of the launch and the mixture fits one component block (K <=
+ * 256), where it is the faster of the two; 2: fused whenever the shape
+ * allows.  Both are as accurate against an FP64 evaluation; their encodings
+ * differ by rounding.  An extension of this library, no reference
+ * counterpart. */
+VLHIP_API void vlhip_fisher_set_fused (int mode) ;
+
 /* Gate used by VLHIP_MODE_GATED (a vlhip_gate_new handle, NULL to clear):
  * per descriptor its `candidates` (<= 16) most probable components; groups of
  * 32 consecutive descriptors of an image whose predicted mass on those

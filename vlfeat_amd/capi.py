@@ -125,6 +125,7 @@ _EXT_SIGS = {
     "vl_fisher_get_precision": (C.c_int, []),
     "vl_fisher_set_gate": (None, [_p, C.c_int, C.c_float]),
     "vlhip_fisher_set_gate": (None, [_p, C.c_int, C.c_float]),
+    "vlhip_fisher_set_fused": (None, [C.c_int]),
     "vlhip_fisher_gate_stats": (None, [C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong), C.c_int]),
     "vl_vlad_encode_hard": (None, [_p, C.c_uint32, _p, _sz, _sz, _p, _sz, _p, C.c_int]),
     "vlhip_available": (C.c_int, []),

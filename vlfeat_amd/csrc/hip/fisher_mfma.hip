@@ -58,6 +58,7 @@ struct MfmaScales {
   float s1, s2;     // x' and x'^2 multipliers
   float sq;         // posterior multiplier
   unsigned bound;   // bits of the (non-negative) range bound on |x'|, max-reduced atomically
+  float shift;      // logit origin of the fused encoder (mfma_shift_kernel)
 };
 
 // ---------------------------------------------------------------------------
@@ -836,6 +837,71 @@ __device__ __forceinline__ void rec_store(const u32x4 (&pre)[PER], unsigned char
   }
 }
 
+// Fisher-vector slice of one (image, component block) from the moment
+// accumulators in registers (fisher.c:419-485 in FP64): lane l holds features
+// 16ft + (l & 15) of components kw + 16c + 4(l >> 4) + r.  Writes u and v,
+// the slice's squared norm into normpart[img * numKB + kb] and the
+// workgroup's kept terms (per-wave lane-mask counts) into numTerms[img].
+template <typename T, int NS, int CT, int WAVES>
+__device__ __forceinline__ void fisher_slice_finalize(
+    const f32x4 (&macc0)[CT], const f32x4 (&macc)[CT][2][NS], const MfmaScales& sc, int kw, int lr, int lg, int tid,
+    int lane, int img, int kb, int numKB, long long nImg, int D, int K, const float* __restrict__ mup,
+    const float* __restrict__ isg, const float* __restrict__ sinv, const T* __restrict__ priors, int flags,
+    T* __restrict__ enc, double* __restrict__ normpart, unsigned long long* __restrict__ numTerms, unsigned terms,
+    double* red) {
+  constexpr int Dp = NS * 16;
+  const double invq = 1.0 / (double)sc.sq;
+  const double inv1 = invq / (double)sc.s1, inv2 = invq / (double)sc.s2;
+  const bool sqrtFlag = (flags & 1) != 0;
+  T* encU = enc + (size_t)img * 2 * K * D;
+  T* encV = encU + (size_t)K * D;
+  double nrm = 0.0;
+#pragma unroll
+  for (int c = 0; c < CT; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double S0 = (double)macc0[c][r] * invq;
+      const int k = kw + 16 * c + 4 * lg + r;
+      const double prior = k < K ? (double)priors[k] : 0.0;
+      const bool use = k < K && prior >= 1e-6;
+      double up = 1.0, vp = 1.0;
+      if (nImg > 0 && use) {
+        up = 1.0 / ((double)nImg * sqrt(prior));
+        vp = 1.0 / ((double)nImg * sqrt(2.0 * prior));
+      }
+#pragma unroll
+      for (int ft = 0; ft < NS; ++ft) {
+        const int d = 16 * ft + lr;
+        if (k >= K || d >= D) continue;
+        T u = T(0), v = T(0);
+        if (use) {
+          const double S1 = (double)macc[c][0][ft][r] * inv1;
+          const double S2 = (double)macc[c][1][ft][r] * inv2;
+          const double mu = (double)mup[(size_t)k * Dp + d];
+          const double uu = (double)isg[(size_t)k * Dp + d] * (S1 - mu * S0);
+          const double vv = (double)sinv[(size_t)k * Dp + d] * (S2 - 2.0 * mu * S1 + mu * mu * S0) - S0;
+          u = (T)(uu * up);
+          v = (T)(vv * vp);
+        }
+        if (sqrtFlag) { u = signed_sqrt(u); v = signed_sqrt(v); }
+        encU[(size_t)k * D + d] = u;
+        encV[(size_t)k * D + d] = v;
+        nrm += (double)u * (double)u + (double)v * (double)v;
+      }
+      // one (component tile, row) at a time: its loads are not hoisted above
+      // the previous one's, which keeps the FP64 temporaries few (the fused
+      // kernel calls this with its weights and accumulators live)
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  const double bn = block_sum_waves<WAVES>(nrm, red);
+  if (tid == 0) normpart[(size_t)img * numKB + kb] = bn;
+  if (numTerms) {
+    // terms is the wave's count (lane masks), entered once per wave
+    const double bt = block_sum_waves<WAVES>(lane == 0 ? (double)terms : 0.0, red);
+    if (tid == 0) atomicAdd(&numTerms[img], (unsigned long long)bt);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Second pass.  Workgroup = (image, WAVES x 16CT components); wave w owns the
 // CT component tiles 16CT w .. with their split weights in registers and
@@ -1089,49 +1155,654 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
     }
     return;
   }
-  const bool sqrtFlag = (flags & 1) != 0;
-  T* encU = enc + (size_t)img * 2 * K * D;
-  T* encV = encU + (size_t)K * D;
-  double nrm = 0.0;
+  fisher_slice_finalize<T, NS, CT, WAVES>(macc0, macc, sc, kw, lr, lg, tid, lane, img, kb, numKB, nImg, D, K, mup,
+                                          isg, sinv, priors, flags, enc, normpart, numTerms, terms, red);
+}
+
+// ---------------------------------------------------------------------------
+// Fused single-pass encoder.  The two-pass form computes every logit twice:
+// once for the row log-sum-exp (first pass) and once more for the posteriors
+// (second pass), because a workgroup owns only a block of components and the
+// normaliser of a row needs all of them.  Here the numKB workgroups that own
+// the component blocks of an image ("partners") walk the image's groups in
+// step and exchange, per group, each row's partial statistics over their own
+// block (max logit, sum of 2^(logit - max) and, for FAST, the first argmax):
+//
+//   iteration n:  build the split fragments of group n in LDS, logits(n),
+//                 publish the block's partial statistics of group n;
+//                 moments of group n - 1 (its posteriors held in registers),
+//                 which hides the partners' latency;
+//                 wait for the partners' partials of group n, combine them
+//                 in block order (so the result does not depend on arrival
+//                 order), posteriors of group n -> registers.
+//
+// Partials are tagged 64-bit words {tag = n + 1 | value} written and read with
+// device-scope relaxed atomics, so a reader that sees the tag sees the value:
+// no fence is needed, and a ring of FX_RING slots per (stream, block) is
+// enough because a workgroup can publish group n + 2 only after every partner
+// has consumed its group n.  All workgroups are co-resident (cooperative
+// launch), so every wait ends; each wait is still bounded (FX_SPIN_MAX polls,
+// or a partner's abort), after which the workgroup raises the abort flag and
+// returns, and the host reports the failure.
+//
+// Blocks are dealt to the XCDs round robin, and the partners of a stream sit
+// on one XCD (they read the same descriptors).  Each stream encodes its list
+// of images (balanced by size on the host) one after the other; the
+// arithmetic of a row, a group and an image is that of the second pass, with
+// the normaliser combined from the block partials, so an image's encoding
+// does not depend on the batch or on the stream it lands on.
+// Logit origin of the fused encoder: max_k b_k - (D / 2) log2 e, about where
+// a row's largest logit falls (a component whose squared Mahalanobis distance
+// is D).  The fused encoder accumulates logit - shift, so that the logits that
+// matter are formed near zero and rounded there, as the second pass forms
+// logit - row maximum; the shift cancels in the posteriors.
+__global__ __launch_bounds__(256) void mfma_shift_kernel(const float* __restrict__ bias, int Kp, int D,
+                                                         MfmaScales* __restrict__ scp) {
+  __shared__ float red[256];
+  float m = -INFINITY;
+  for (int k = threadIdx.x; k < Kp; k += 256) m = fmaxf(m, bias[k]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float mb = red[0];
+    scp->shift = (mb == -INFINITY) ? 0.f : (float)((double)mb - 0.5 * (double)D * kLog2eD);
+  }
+}
+
+// Reductions over the 16 lanes of a DPP row (lanes 16i .. 16i + 15), every
+// lane receiving the result: swaps within pairs and quads, then the row's
+// half mirror and mirror (four VALU operations, no LDS round trip).  IEEE
+// max and addition are commutative, so all 16 lanes end with the same bits.
+__device__ __forceinline__ float dpp_f(float v, int ctrl) {
+  switch (ctrl) {  // (the control must be a constant)
+    case 0: return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+    case 1: return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+    case 2: return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));
+    default: return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false));
+  }
+}
+__device__ __forceinline__ int dpp_i(int v, int ctrl) {
+  switch (ctrl) {
+    case 0: return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+    case 1: return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
+    case 2: return __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);
+    default: return __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);
+  }
+}
+__device__ __forceinline__ float row16_max(float v) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v = fmaxf(v, dpp_f(v, c));
+  return v;
+}
+__device__ __forceinline__ float row16_sum(float v) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v = v + dpp_f(v, c);
+  return v;
+}
+// first maximum of (value, index) pairs: larger value, then smaller index
+__device__ __forceinline__ void row16_argmax(float& v, int& i) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const float ov = dpp_f(v, c);
+    const int oi = dpp_i(i, c);
+    if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+  }
+}
+
+constexpr int FX_RING = 4;
+constexpr int FX_SPIN_MAX = 1 << 20;
+
+__device__ __forceinline__ unsigned long long fx_word(unsigned tag, unsigned bits) {
+  return ((unsigned long long)tag << 32) | (unsigned long long)bits;
+}
+__device__ __forceinline__ void fx_store(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long fx_load(const unsigned long long* p) {
+  return __hip_atomic_load(const_cast<unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The fused kernel's output for one (image, component block): the raw FP32
+// moment accumulators, still scaled by the operand scales, S1 and S2 in the
+// places of the image's u and v entries and S0 in s0raw[image][K]; lane l
+// holds features 16ft + (l & 15) of components kw + 16c + 4(l >> 4) + r.
+// mfma_fused_finalize_kernel turns them into the Fisher vector.
+template <int NS, int CT>
+__device__ __forceinline__ void fused_store_raw(const f32x4 (&macc0)[CT], const f32x4 (&macc)[CT][2][NS], int kw,
+                                                int lr, int lg, int img, int D, int K, float* __restrict__ enc,
+                                                float* __restrict__ s0raw) {
+  // the lane's coordinates pass through an empty asm, so that the store
+  // offsets are formed here and not hoisted out of the caller's loops (where
+  // eighty of them would stay live across the whole sweep)
+  int k0 = kw + 4 * lg, l0 = lr;
+  asm volatile("" : "+v"(k0), "+v"(l0));
+  const size_t KD = (size_t)K * D;
+  float* e = enc + (size_t)img * 2 * KD;
 #pragma unroll
   for (int c = 0; c < CT; ++c)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const double S0 = (double)macc0[c][r] * invq;
-      const int k = kw + 16 * c + 4 * lg + r;
-      const double prior = k < K ? (double)priors[k] : 0.0;
-      const bool use = k < K && prior >= 1e-6;
-      double up = 1.0, vp = 1.0;
-      if (nImg > 0 && use) {
-        up = 1.0 / ((double)nImg * sqrt(prior));
-        vp = 1.0 / ((double)nImg * sqrt(2.0 * prior));
-      }
+      const int k = k0 + 16 * c + r;
+      if (k >= K) continue;
+      if (l0 == 0) s0raw[(size_t)img * K + k] = macc0[c][r];
+      float* eu = e + (size_t)k * D + l0;
 #pragma unroll
       for (int ft = 0; ft < NS; ++ft) {
-        const int d = 16 * ft + lr;
-        if (k >= K || d >= D) continue;
-        T u = T(0), v = T(0);
-        if (use) {
-          const double S1 = (double)macc[c][0][ft][r] * inv1;
-          const double S2 = (double)macc[c][1][ft][r] * inv2;
-          const double mu = (double)mup[(size_t)k * Dp + d];
-          const double uu = (double)isg[(size_t)k * Dp + d] * (S1 - mu * S0);
-          const double vv = (double)sinv[(size_t)k * Dp + d] * (S2 - 2.0 * mu * S1 + mu * mu * S0) - S0;
-          u = (T)(uu * up);
-          v = (T)(vv * vp);
-        }
-        if (sqrtFlag) { u = signed_sqrt(u); v = signed_sqrt(v); }
-        encU[(size_t)k * D + d] = u;
-        encV[(size_t)k * D + d] = v;
-        nrm += (double)u * (double)u + (double)v * (double)v;
+        if (16 * ft + l0 >= D) continue;
+        eu[16 * ft] = macc[c][0][ft][r];
+        eu[KD + 16 * ft] = macc[c][1][ft][r];
       }
     }
-  const double bn = block_sum_waves<WAVES>(nrm, red);
-  if (tid == 0) normpart[(size_t)img * numKB + kb] = bn;
-  if (numTerms) {
-    // terms is the wave's count (lane masks), entered once per wave
-    const double bt = block_sum_waves<WAVES>(lane == 0 ? (double)terms : 0.0, red);
-    if (tid == 0) atomicAdd(&numTerms[img], (unsigned long long)bt);
+}
+
+// Staging of a group's rows: the 32 rows are contiguous in memory, so with D
+// a multiple of 4 and 16-byte aligned data they are copied straight into LDS
+// (global_load_lds_dwordx4: lane l of a wave writes bytes 16l .. 16l + 15 of
+// the wave's 1 KiB piece), issued a whole iteration ahead and holding no
+// registers; otherwise they are loaded through registers when needed.  Rows
+// past the image are masked when the fragments are built; addresses past the
+// end of the data are clamped to its last 16 bytes.
+constexpr int FX_PIECE = 1024;  // bytes per wave-wide copy
+
+template <typename H, int NS, int CT, int WAVES, bool FAST, bool DMA>
+__global__ __launch_bounds__(WAVES * 64, 1) void mfma_fused_kernel(
+    const float* __restrict__ x, unsigned long long xBytes, const uint64_t* __restrict__ offsets,
+    const long long* __restrict__ streamOff, const int* __restrict__ streamImg, int numKB, int D, int K,
+    const H* __restrict__ Whi, const H* __restrict__ Wlo, const float* __restrict__ bias,
+    const MfmaScales* __restrict__ scp, const float* __restrict__ g, float* __restrict__ enc,
+    float* __restrict__ s0raw, unsigned long long* __restrict__ numTerms, unsigned long long* __restrict__ xch,
+    unsigned* __restrict__ abortFlag) {
+  typedef typename VecOf<H>::type V8;
+  typedef GroupRec<NS> R;
+  constexpr int Dp = NS * 16;
+  constexpr int NT = WAVES * 64;
+  constexpr int BC = WAVES * 16 * CT;
+  constexpr size_t FRAG = R::kA + R::kB;         // A and B fragments of one group
+  constexpr int XR = (GR * Dp * 4 + FX_PIECE - 1) / FX_PIECE * (FX_PIECE / 4) + 8;  // staged floats (+ slack)
+  constexpr int NA = 2 * NS * 64, NB = NS * 64;  // fragment items (A: hi + lo, B: 4 planes)
+  constexpr int NI = (NA + NB + NT - 1) / NT;
+  extern __shared__ __align__(16) unsigned char smem_raw[];  // 2 x FRAG
+  __shared__ __align__(16) float xr[XR];         // the group's rows, [row][D], uncentred
+  __shared__ __align__(16) float gsh[Dp];
+  __shared__ float pm[WAVES][GR], ps[WAVES][GR];
+  __shared__ int pa[WAVES][GR];
+  __shared__ float fm[GR], frs[GR];
+  __shared__ int fa[GR];
+  __shared__ int fabort;
+
+  // block -> (stream, component block); the partners of a stream share an XCD
+  const int b = blockIdx.x, xcd = b & 7, slot = b >> 3;
+  const int kb = slot % numKB, stream = (slot / numKB) * 8 + xcd;
+  const long long i0 = streamOff[stream], i1 = streamOff[stream + 1];
+  if (i0 >= i1) return;  // the partners have the same (empty) list
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int lr = lane & 15, lg = lane >> 4;
+  const int kw = kb * BC + 16 * CT * w;  // first component of this wave
+  const MfmaScales sc = *scp;
+  const float qminS = __uint_as_float(0x358637beu) * sc.sq;  // next float above float(1e-6), scaled
+
+  V8 whi[CT][NS], wlo[CT][NS];
+  float bc[CT];
+#pragma unroll
+  for (int c = 0; c < CT; ++c) {
+    const int comp = kw + 16 * c + lr;
+    bc[c] = bias[comp];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      whi[c][j] = *reinterpret_cast<const V8*>(Whi + (size_t)comp * 2 * Dp + 32 * j + 8 * lg);
+      wlo[c][j] = *reinterpret_cast<const V8*>(Wlo + (size_t)comp * 2 * Dp + 32 * j + 8 * lg);
+    }
+  }
+  float bcs[CT];  // biases relative to the logit origin
+#pragma unroll
+  for (int c = 0; c < CT; ++c) bcs[c] = bc[c] - sc.shift;
+  for (int d = tid; d < Dp; d += NT) gsh[d] = g[d];
+  V8 ones;
+#pragma unroll
+  for (int v = 0; v < 8; ++v) ones[v] = H(1.f);
+  f32x4 macc[CT][2][NS], macc0[CT];
+#pragma unroll
+  for (int c = 0; c < CT; ++c) {
+    macc0[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s2 = 0; s2 < 2; ++s2)
+#pragma unroll
+      for (int ft = 0; ft < NS; ++ft) macc[c][s2][ft] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+  unsigned terms = 0;
+  V8 aqp[CT];  // posteriors (scaled, 16-bit) of the pending group, A fragments of the moments
+  unsigned long long* own = xch + (size_t)(stream * numKB + kb) * FX_RING * 3 * GR;
+
+  // rows [rowA, rowA + 32) into xr (see above)
+  auto stage = [&](long long rowA) __attribute__((always_inline)) {
+    if constexpr (DMA) {
+      constexpr int Q = (GR * Dp * 4 + FX_PIECE - 1) / FX_PIECE;
+      const unsigned long long base = (unsigned long long)rowA * (unsigned long long)D * 4ull;
+      const int qn = (GR * D * 4 + FX_PIECE - 1) / FX_PIECE;
+#pragma unroll
+      for (int q0 = 0; q0 < Q; q0 += WAVES) {
+        const int q = q0 + w;
+        if (q < qn) {
+          unsigned long long off = base + (unsigned long long)q * FX_PIECE + 16ull * lane;
+          if (off + 16 > xBytes) off = xBytes - 16;
+          __builtin_amdgcn_global_load_lds(
+              (const __attribute__((address_space(1))) void*)(reinterpret_cast<const char*>(x) + off),
+              (__attribute__((address_space(3))) void*)(xr + q * (FX_PIECE / 4)), 16, 0, 0);
+        }
+      }
+    } else {
+      const long long rowLim = (long long)(xBytes / 4ull / (unsigned long long)D);
+      for (int i = tid; i < GR * D; i += NT) {
+        const long long row = rowA + i / D;
+        xr[i] = row < rowLim ? x[rowA * D + i] : 0.f;
+      }
+    }
+  };
+
+  int pendImg = -1;            // image of the pending group (moments not yet added)
+  bool pendLast = false;       // the pending group is its image's last
+  int pendBuf = 0;
+  long long n = 0;             // groups of this stream so far (the exchange sequence)
+
+  // moments of the pending group; when it was its image's last, the image's raw slice
+  auto flush = [&]() __attribute__((always_inline)) {
+    const V8* Bp = reinterpret_cast<const V8*>(smem_raw + (size_t)pendBuf * FRAG + R::kA);
+#pragma unroll
+    for (int c = 0; c < CT; ++c) macc0[c] = mfma16(aqp[c], ones, macc0[c]);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+#pragma unroll
+      for (int ft = 0; ft < NS; ++ft) {
+        const V8 bx = Bp[(p * NS + ft) * 64 + lane];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) macc[c][p >> 1][ft] = mfma16(aqp[c], bx, macc[c][p >> 1][ft]);
+      }
+    }
+    if (pendLast) {
+      fused_store_raw<NS, CT>(macc0, macc, kw, lr, lg, pendImg, D, K, enc, s0raw);
+      // terms: the wave's count (lane masks, the same in every lane)
+      if (numTerms && lane == 0 && terms) atomicAdd(&numTerms[pendImg], (unsigned long long)terms);
+      terms = 0;
+#pragma unroll
+      for (int c = 0; c < CT; ++c) {
+        macc0[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2)
+#pragma unroll
+          for (int ft = 0; ft < NS; ++ft) macc[c][s2][ft] = f32x4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+    pendImg = -1;
+  };
+
+  // the stream's groups as one sequence (images without rows are skipped:
+  // the finalisation writes their zero encodings); position (li, gi)
+  auto skip_empty = [&](long long l) __attribute__((always_inline)) {
+    while (l < i1) {
+      const int im = streamImg[l];
+      if (offsets[im + 1] > offsets[im]) break;
+      ++l;
+    }
+    return l;
+  };
+  long long li = skip_empty(i0), gi = 0;
+  if (li < i1) stage((long long)offsets[streamImg[li]]);
+  if constexpr (DMA) __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed
+  __syncthreads();                                  // xr holds the first group's rows
+  while (true) {
+    const bool have = li < i1;
+    int img = 0;
+    int nrow = 0, buf = 0;
+    bool lastOfImg = false;
+    long long nli = li, ngi = gi;
+    f32x4 acc[2][CT];
+    const unsigned tag = (unsigned)(n + 1);  // exchange sequence of this group
+    const int rs = (int)(n % FX_RING);
+    const int xrow = lane & (GR - 1);
+    float bm = -INFINITY, bs = 0.f;          // the block's partial of row xrow (wave 0)
+    int ba = 0;
+    if (have) {
+      img = streamImg[li];
+      const long long rB = (long long)offsets[img], rE = (long long)offsets[img + 1];
+      const long long row0 = rB + GR * gi;
+      nrow = (int)min((long long)GR, rE - row0);
+      buf = (int)(n & 1);
+      unsigned char* fr = smem_raw + (size_t)buf * FRAG;
+      // (xr holds the group's rows: copied before the previous iteration's last barrier)
+      // ---- split fragments of the group (the first pass's record layout),
+      // centred x' = x - g, zero past D and past the image
+      {
+        V8* Ap = reinterpret_cast<V8*>(fr);
+        V8* Bp = reinterpret_cast<V8*>(fr + R::kA);
+        // the thread index passes through an empty asm: the item addresses
+        // (runtime row stride D) are formed here on every group instead of
+        // being hoisted out of the loops and held in registers throughout
+        int tv = tid;
+        asm volatile("" : "+v"(tv));
+#pragma unroll
+        for (int q = 0; q < NI; ++q) {
+          const int it = tv + NT * q;
+          if (it < NA) {
+            const int t = it / (NS * 64), rem = it - t * (NS * 64), j = rem >> 6, l = rem & 63;
+            const int f0 = 32 * j + 8 * (l >> 4);
+            const bool sqf = f0 < Dp;
+            const int d0 = sqf ? f0 : f0 - Dp;
+            const int row = 16 * t + (l & 15);
+            float xc[8];
+            if (DMA && row < nrow && d0 + 8 <= D) {
+              // (D is a multiple of 4 here: 16-byte aligned reads)
+              const f32x4 a0 = *reinterpret_cast<const f32x4*>(&xr[row * D + d0]) -
+                               *reinterpret_cast<const f32x4*>(&gsh[d0]);
+              const f32x4 a1 = *reinterpret_cast<const f32x4*>(&xr[row * D + d0 + 4]) -
+                               *reinterpret_cast<const f32x4*>(&gsh[d0 + 4]);
+#pragma unroll
+              for (int v = 0; v < 4; ++v) { xc[v] = a0[v]; xc[4 + v] = a1[v]; }
+            } else {
+#pragma unroll
+              for (int v = 0; v < 8; ++v) {
+                const int d = d0 + v;
+                xc[v] = (row < nrow && d < D) ? xr[row * D + d] - gsh[d] : 0.f;
+              }
+            }
+            V8 hi, lo;
+            make_frag<H>(xc, sqf, sc.s1, sc.s2, hi, lo);
+            Ap[((t * NS + j) * 2 + 0) * 64 + l] = hi;
+            Ap[((t * NS + j) * 2 + 1) * 64 + l] = lo;
+          } else if (it < NA + NB) {
+            const int ib = it - NA, ft = ib >> 6, l = ib & 63;
+            const int d = 16 * ft + (l & 15), lg2 = l >> 4;
+            const float gd = d < D ? gsh[d] : 0.f;
+            V8 b1h, b1l, b2h, b2l;
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+              const int row = 16 * (v >> 2) + 4 * lg2 + (v & 3);
+              const float xc = (row < nrow && d < D) ? xr[row * D + d] - gd : 0.f;
+              H h, lo;
+              split16<H>(xc * sc.s1, h, lo);
+              b1h[v] = h;
+              b1l[v] = lo;
+              split16<H>(xc * xc * sc.s2, h, lo);
+              b2h[v] = h;
+              b2l[v] = lo;
+            }
+            Bp[(0 * NS + ft) * 64 + l] = b1h;
+            Bp[(1 * NS + ft) * 64 + l] = b1l;
+            Bp[(2 * NS + ft) * 64 + l] = b2h;
+            Bp[(3 * NS + ft) * 64 + l] = b2l;
+          }
+        }
+      }
+      __syncthreads();  // fragments complete; xr is free
+      // the next group's rows (possibly of the next image), copied while this
+      // one is computed
+      lastOfImg = row0 + GR >= rE;
+      if (lastOfImg) {
+        nli = skip_empty(li + 1);
+        ngi = 0;
+      } else {
+        ngi = gi + 1;
+      }
+      if (nli < i1) stage((long long)offsets[streamImg[nli]] + GR * ngi);
+      // ---- logits of the group minus the logit origin (log2 units)
+      {
+        const V8* Ap = reinterpret_cast<const V8*>(fr);
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+          for (int c = 0; c < CT; ++c) acc[t][c] = f32x4{bcs[c], bcs[c], bcs[c], bcs[c]};
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+          V8 ah[2], al[2];
+#pragma unroll
+          for (int t = 0; t < 2; ++t) {
+            ah[t] = Ap[((t * NS + j) * 2 + 0) * 64 + lane];
+            al[t] = Ap[((t * NS + j) * 2 + 1) * 64 + lane];
+          }
+#pragma unroll
+          for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int c = 0; c < CT; ++c) acc[t][c] = mfma16(ah[t], whi[c][j], acc[t][c]);
+#pragma unroll
+          for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int c = 0; c < CT; ++c) acc[t][c] = mfma16(al[t], whi[c][j], acc[t][c]);
+#pragma unroll
+          for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int c = 0; c < CT; ++c) acc[t][c] = mfma16(ah[t], wlo[c][j], acc[t][c]);
+        }
+      }
+      // ---- the wave's partial statistics of each row (16 lanes share a row)
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          float m = acc[t][0][r];
+          float bv = acc[t][0][r];
+          int bi = kw + lr;
+#pragma unroll
+          for (int c = 1; c < CT; ++c) {
+            m = fmaxf(m, acc[t][c][r]);
+            if (FAST && acc[t][c][r] > bv) { bv = acc[t][c][r]; bi = kw + 16 * c + lr; }
+          }
+          m = row16_max(m);
+          const float ms = (m == -INFINITY) ? 0.f : m;
+          float s = 0.f;
+#pragma unroll
+          for (int c = 0; c < CT; ++c) s += __builtin_amdgcn_exp2f(acc[t][c][r] - ms);
+          s = row16_sum(s);
+          if constexpr (FAST) row16_argmax(bv, bi);
+          if (lr == 0) {
+            const int row = 16 * t + 4 * lg + r;
+            pm[w][row] = m;
+            ps[w][row] = s;
+            pa[w][row] = bi;
+          }
+        }
+      __syncthreads();
+      // ---- the block's partial of each row, published for the partners
+      // (wave 0; lanes l and l + 32 both hold row l & 31, so that the wave's
+      // control flow stays uniform)
+      if (w == 0) {
+#pragma unroll
+        for (int v = 0; v < WAVES; ++v) bm = fmaxf(bm, pm[v][xrow]);
+        const float bms = (bm == -INFINITY) ? 0.f : bm;
+        bool found = false;
+#pragma unroll
+        for (int v = 0; v < WAVES; ++v) {
+          const float mv = pm[v][xrow];
+          const float sv = ps[v][xrow];
+          if (sv != 0.f) bs += sv * __builtin_amdgcn_exp2f(((mv == -INFINITY) ? 0.f : mv) - bms);
+          if (!found && mv == bm) { ba = pa[v][xrow]; found = true; }
+        }
+        if (lane < GR) {
+          unsigned long long* sl = own + (size_t)rs * 3 * GR;
+          fx_store(sl + xrow, fx_word(tag, __float_as_uint(bm)));
+          fx_store(sl + GR + xrow, fx_word(tag, __float_as_uint(bs)));
+          fx_store(sl + 2 * GR + xrow, fx_word(tag, (unsigned)ba));
+        }
+      }
+    }
+    // ---- moments of the pending group, while the partners publish (the
+    // only place the accumulators are updated)
+    if (pendImg >= 0) flush();
+    if (!have) break;
+    {
+      // ---- the partners' partials, combined in block order: a running
+      // maximum M and sum S = sum_p s_p 2^(m_p - M), and the first block whose
+      // maximum is M gives the first argmax (blocks ascend in component index)
+      if (w == 0) {
+        bool ok = true;
+        float M = -INFINITY, S = 0.f;
+        int A = 0;
+        for (int p = 0; p < numKB && ok; ++p) {
+          float mp = bm, sp = bs;
+          int ap = ba;
+          if (p != kb) {
+            const unsigned long long* sl = xch + ((size_t)(stream * numKB + p) * FX_RING + rs) * 3 * GR;
+            int spins = 0;
+            while (true) {  // uniform over the wave: it leaves when every lane's row has arrived
+              const unsigned long long a0 = fx_load(sl + xrow), a1 = fx_load(sl + GR + xrow),
+                                       a2 = fx_load(sl + 2 * GR + xrow);
+              const bool ready =
+                  (unsigned)(a0 >> 32) == tag && (unsigned)(a1 >> 32) == tag && (unsigned)(a2 >> 32) == tag;
+              if (__all(ready)) {
+                mp = __uint_as_float((unsigned)a0);
+                sp = __uint_as_float((unsigned)a1);
+                ap = (int)(unsigned)a2;
+                break;
+              }
+              if (++spins > FX_SPIN_MAX ||
+                  __hip_atomic_load(abortFlag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                ok = false;
+                break;
+              }
+              __builtin_amdgcn_s_sleep(1);
+            }
+            if (!ok) break;
+          }
+          const float mps = (mp == -INFINITY) ? 0.f : mp;
+          if (mp > M) {
+            const float Mso = (M == -INFINITY) ? 0.f : M;
+            S = (S != 0.f ? S * __builtin_amdgcn_exp2f(Mso - mps) : 0.f) + sp;
+            M = mp;
+            A = ap;
+          } else if (sp != 0.f) {
+            S += sp * __builtin_amdgcn_exp2f(mps - ((M == -INFINITY) ? 0.f : M));
+          }
+        }
+        if (ok && lane < GR) {
+          const bool real = lane < nrow;
+          fm[lane] = (M == -INFINITY) ? 0.f : M;
+          frs[lane] = real ? sc.sq / S : 0.f;
+          fa[lane] = real ? A : -1;
+        }
+        if (lane == 0) {
+          fabort = ok ? 0 : 1;
+          if (!ok) __hip_atomic_store(abortFlag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      // the next group's rows have landed (they are read after this barrier)
+      if constexpr (DMA) __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (fabort) return;  // uniform: every wave read the same flag after the barrier
+      // ---- posteriors of this group -> the pending A fragments
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = 16 * t + 4 * lg + r;
+          const float Ms = fm[row], rsv = frs[row];
+          const int A = fa[row];
+#pragma unroll
+          for (int c = 0; c < CT; ++c) {
+            if constexpr (FAST) {
+              const bool hit = kw + 16 * c + lr == A && bc[c] != -INFINITY;
+              terms += (unsigned)__popcll(__ballot(hit));
+              aqp[c][4 * t + r] = hit ? H(sc.sq) : H(0.f);
+            } else {
+              const float v = __builtin_amdgcn_exp2f(acc[t][c][r] - Ms) * rsv;
+              const bool keep = v >= qminS;
+              terms += (unsigned)__popcll(__ballot(keep));
+              aqp[c][4 * t + r] = H(keep ? v : 0.f);
+            }
+          }
+        }
+      pendImg = img;
+      pendLast = lastOfImg;
+      pendBuf = buf;
+    }
+    li = nli;
+    gi = ngi;
+    ++n;
+  }
+}
+
+// Fisher vectors from the fused kernel's raw accumulators (fisher.c:419-485 in
+// FP64, the same arithmetic as fisher_slice_finalize): workgroup per image;
+// the per-component factors S0, 1/(n sqrt(pi)), 1/(n sqrt(2 pi)) in LDS; with
+// the L2 normalisation the entries are computed twice, once for the squared
+// norm (FP64, fixed order) and once to be divided by it, so that each raw
+// value is read twice and each result written once.
+constexpr int FF_THREADS = 512;
+__global__ __launch_bounds__(FF_THREADS) void mfma_fused_finalize_kernel(
+    float* __restrict__ enc, const float* __restrict__ s0raw, const MfmaScales* __restrict__ scp,
+    const float* __restrict__ mup, const float* __restrict__ isg, const float* __restrict__ sinv,
+    const float* __restrict__ priors, const uint64_t* __restrict__ offsets, int D, int Dp, int K, int flags) {
+  extern __shared__ double kf[];  // [S0 | u factor | v factor] x K; u factor < 0: component skipped
+  __shared__ double red[FF_THREADS / 64];
+  __shared__ float nsh;
+  const int img = blockIdx.x, tid = threadIdx.x;
+  const long long nImg = (long long)offsets[img + 1] - (long long)offsets[img];
+  if (nImg == 0) {
+    // the fused kernel skips images without descriptors: all-zero encodings
+    float* e = enc + (size_t)img * 2 * (size_t)K * D;
+    for (size_t i = tid; i < 2 * (size_t)K * D; i += FF_THREADS) e[i] = 0.f;
+    return;
+  }
+  const MfmaScales sc = *scp;
+  const double invq = 1.0 / (double)sc.sq;
+  const double inv1 = invq / (double)sc.s1, inv2 = invq / (double)sc.s2;
+  for (int k = tid; k < K; k += FF_THREADS) {
+    const double prior = (double)priors[k];
+    const bool use = prior >= 1e-6;
+    double up = 1.0, vp = 1.0;
+    if (nImg > 0 && use) {
+      up = 1.0 / ((double)nImg * sqrt(prior));
+      vp = 1.0 / ((double)nImg * sqrt(2.0 * prior));
+    }
+    kf[k] = (double)s0raw[(size_t)img * K + k] * invq;
+    kf[K + k] = use ? up : -1.0;
+    kf[2 * K + k] = vp;
+  }
+  __syncthreads();
+  const size_t KD = (size_t)K * D;
+  float* e = enc + (size_t)img * 2 * KD;
+  const bool sqrtFlag = (flags & 1) != 0;
+  auto entry = [&](size_t i, float& u, float& v) __attribute__((always_inline)) {
+    const int k = (int)(i / (size_t)D), d = (int)(i - (size_t)k * D);
+    const double up = kf[K + k];
+    u = 0.f;
+    v = 0.f;
+    if (up >= 0.0) {
+      const double S0 = kf[k];
+      const double S1 = (double)e[i] * inv1;
+      const double S2 = (double)e[KD + i] * inv2;
+      const double mu = (double)mup[(size_t)k * Dp + d];
+      const double uu = (double)isg[(size_t)k * Dp + d] * (S1 - mu * S0);
+      const double vv = (double)sinv[(size_t)k * Dp + d] * (S2 - 2.0 * mu * S1 + mu * mu * S0) - S0;
+      u = (float)(uu * up);
+      v = (float)(vv * kf[2 * K + k]);
+    }
+    if (sqrtFlag) { u = signed_sqrt(u); v = signed_sqrt(v); }
+  };
+  float n = 1.f;
+  const bool l2 = (flags & 2) != 0;
+  if (l2) {
+    double acc = 0.0;
+    for (size_t i = tid; i < KD; i += FF_THREADS) {
+      float u, v;
+      entry(i, u, v);
+      acc += (double)u * (double)u + (double)v * (double)v;
+    }
+    const double tot = block_sum_waves<FF_THREADS / 64>(acc, red);
+    if (tid == 0) {
+      const float nn = (float)sqrt(tot);
+      nsh = (float)(((double)nn > 1e-12) ? (double)nn : 1e-12);
+    }
+    __syncthreads();
+    n = nsh;
+  }
+  for (size_t i = tid; i < KD; i += FF_THREADS) {
+    float u, v;
+    entry(i, u, v);
+    e[i] = l2 ? u / n : u;
+    e[KD + i] = l2 ? v / n : v;
   }
 }
 
@@ -1357,6 +2028,111 @@ static void run_mfma(float* enc, const float* means, const float* covs, const fl
   }
 }
 
+// Fused single-pass encoder (mfma_fused_kernel): one cooperative launch over
+// all images.  Returns false (nothing launched) when the shape or the device
+// does not allow it or, in the automatic mode, when there are fewer images
+// than streams (the two-pass encoder spreads a few images over more of the
+// device) or more than one component block per image: with partners, the
+// per-group exchange and the fragments every partner builds cost more than
+// the first pass saves (K = 1024, D = 80, 2048 images of 3000 descriptors:
+// 21.2 ms fused against 15.7 ms in two passes; K = 256: 4.7 against 5.5 ms).
+static int g_fused_mode = 1;  // 0: never, 1: automatic, 2: whenever the shape allows
+
+template <typename H, int NS, int CT, int WAVES>
+static bool run_fused(float* enc, const float* means, const float* covs, const float* priors, size_t D, size_t K,
+                      const float* x, const uint64_t* offsets, size_t numImages, int flags,
+                      unsigned long long* numTerms, const std::vector<uint64_t>& off) {
+  constexpr int Dp = NS * 16;
+  constexpr int BC = WAVES * 16 * CT;
+  constexpr int NT = WAVES * 64;
+  typedef GroupRec<NS> R;
+  if (g_fused_mode == 0) return false;
+  const int Kp = (int)((K + BC - 1) / BC) * BC;
+  const int numKB = Kp / BC;
+  if (K > 2048) return false;  // the finalisation keeps 3 K doubles in LDS
+  const int spx = (ctx().num_cus / 8) / numKB;  // streams per XCD
+  if (spx < 1) return false;
+  const int S = 8 * spx, G = S * numKB;
+  if (g_fused_mode == 1 && (numImages < (size_t)S || numKB > 1)) return false;
+  if (numImages >= (size_t(1) << 31)) return false;
+  const unsigned long long xBytes = (unsigned long long)off[numImages] * D * sizeof(float);
+  const bool dma = (D % 4) == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0 && xBytes >= 16;
+  const bool fast = (flags & 4) != 0;
+  const void* kern = fast ? (dma ? (const void*)mfma_fused_kernel<H, NS, CT, WAVES, true, true>
+                                 : (const void*)mfma_fused_kernel<H, NS, CT, WAVES, true, false>)
+                          : (dma ? (const void*)mfma_fused_kernel<H, NS, CT, WAVES, false, true>
+                                 : (const void*)mfma_fused_kernel<H, NS, CT, WAVES, false, false>);
+  const size_t shmem = 2 * (R::kA + R::kB);
+  VLHIP_CHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+  int occ = 0, coop = 0;
+  VLHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, NT, shmem));
+  VLHIP_CHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, ctx().device));
+  if (!coop || occ < 1 || G > occ * ctx().num_cus) return false;
+
+  unsigned char* base = (unsigned char*)scratch(SLOT_PREP, mfma_tables_bytes<H>(Dp, Kp));
+  const MfmaTables<H> t = mfma_tables_carve<H>(base, Dp, Kp);
+  mfma_tables_launch<H>(t, means, covs, priors, (int)D, Dp, (int)K, Kp);
+  VLHIP_LAUNCH("mfma_shift", mfma_shift_kernel, dim3(1), dim3(256), 0, t.bias, Kp, (int)D, t.scp);
+  float* s0raw = (float*)scratch(SLOT_PARTIAL, numImages * K * sizeof(float));
+  if (numTerms) zero_words(numTerms, 2 * numImages);
+
+  // streams: images by size, largest first, each to the least-loaded stream
+  std::vector<int> order(numImages);
+  for (size_t i = 0; i < numImages; ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return off[a + 1] - off[a] > off[b + 1] - off[b]; });
+  std::vector<std::vector<int>> lists(S);
+  std::vector<unsigned long long> load(S, 0);
+  for (int img : order) {
+    int best = 0;
+    for (int s = 1; s < S; ++s)
+      if (load[s] < load[best]) best = s;
+    lists[best].push_back(img);
+    load[best] += (off[img + 1] - off[img] + GR - 1) / GR + 1;  // groups, plus the image's finalisation
+  }
+  std::vector<long long> hOff(S + 1);
+  std::vector<int> hImg;
+  hImg.reserve(numImages);
+  hOff[0] = 0;
+  for (int s = 0; s < S; ++s) {
+    hImg.insert(hImg.end(), lists[s].begin(), lists[s].end());
+    hOff[s + 1] = (long long)hImg.size();
+  }
+  unsigned char* sb = (unsigned char*)scratch(SLOT_STREAMS, al256((S + 1) * sizeof(long long)) +
+                                                                al256(numImages * sizeof(int)));
+  long long* dOff = (long long*)sb;
+  int* dImg = (int*)(sb + al256((S + 1) * sizeof(long long)));
+  upload(dOff, hOff.data(), (S + 1) * sizeof(long long));
+  upload(dImg, hImg.data(), numImages * sizeof(int));
+  // exchange slots (tags start at 1, so zero means unwritten) and the abort flag
+  const size_t xw = (size_t)S * numKB * FX_RING * 3 * GR;
+  unsigned char* xb = (unsigned char*)scratch(SLOT_FUSED, xw * 8 + 256);
+  zero_words(xb, (xw * 8 + 256) / 4);
+  unsigned long long* xch = (unsigned long long*)xb;
+  unsigned* abortFlag = (unsigned*)(xb + xw * 8);
+  sync();  // the host lists are local
+
+  int Di = (int)D, Ki = (int)K;
+  unsigned long long xb64 = xBytes;
+  void* args[] = {(void*)&x,    (void*)&xb64,  (void*)&offsets, (void*)&dOff,  (void*)&dImg,
+                  (void*)&numKB, (void*)&Di,   (void*)&Ki,      (void*)&t.Whi, (void*)&t.Wlo,
+                  (void*)&t.bias, (void*)&t.scp, (void*)&t.g,   (void*)&enc,   (void*)&s0raw,
+                  (void*)&numTerms, (void*)&xch, (void*)&abortFlag};
+  hipEvent_t ts = nullptr;
+  const int ti = timing_begin("mfma_fused", &ts);
+  VLHIP_CHECK(hipLaunchCooperativeKernel(kern, dim3((unsigned)G), dim3(NT), args, (unsigned)shmem, ctx().stream));
+  if (ti >= 0) timing_end(ti, ts);
+  const size_t fsh = 3 * (size_t)K * sizeof(double);
+  VLHIP_CHECK(hipFuncSetAttribute((const void*)mfma_fused_finalize_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)fsh));
+  VLHIP_LAUNCH("mfma_fused_finalize", mfma_fused_finalize_kernel, dim3((unsigned)numImages), dim3(FF_THREADS), fsh,
+               enc, s0raw, t.scp, t.mup, t.isg, t.sinv, priors, offsets, (int)D, Dp, (int)K, flags);
+  unsigned ab = 0;
+  download(&ab, abortFlag, sizeof(unsigned));
+  VLHIP_REQUIRE(ab == 0, "the fused matrix-core encoder's exchange between component blocks timed out");
+  return true;
+}
+
 // Component tiling: 8 waves x 32 components for large mixtures, 4 waves x 32
 // components when K is small (less padding) or D > 80 (registers).
 template <typename H, int NS>
@@ -1365,6 +2141,8 @@ static void run_mfma_k(float* enc, const float* means, const float* covs, const 
                        unsigned long long* numTerms, const std::vector<uint64_t>& off, const GateUse* gu) {
   if constexpr (NS <= 5) {
     if (K > 128) {
+      if (!gu && run_fused<H, NS, 2, 8>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off))
+        return;
       run_mfma<H, NS, 2, 8>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off, gu);
       return;
     }
@@ -1728,6 +2506,13 @@ double vlhip_gmm_em_step(int mode, double* out_host, void const* priors, void co
                                  (const float*)data, N);
   return em_dispatch<__bf16>(out_host, (const float*)means, (const float*)covs, (const float*)priors, D, K,
                              (const float*)data, N);
+}
+
+void vlhip_fisher_set_fused(int mode) {
+  ensure_init();
+  std::lock_guard<std::recursive_mutex> lk(ctx().mu);
+  VLHIP_REQUIRE(mode >= 0 && mode <= 2, "the fused-encoder mode is 0 (never), 1 (automatic) or 2 (whenever possible)");
+  g_fused_mode = mode;
 }
 
 void* vlhip_encode_plan_new(size_t D, size_t K, uint64_t const* offsets_host, size_t numImages, int fisherFlags,

@@ -56,6 +56,8 @@ enum Slot {
   SLOT_GROUPS,        // per-image first group index of the matrix-core Fisher encoder
   SLOT_GROUPTAB,      // per-group first row and image end of the matrix-core Fisher encoder
   SLOT_GATELIST,      // posteriors of gate-served groups (component, q) of the matrix-core Fisher encoder
+  SLOT_FUSED,         // partner exchange slots and abort flag of the fused matrix-core encoder
+  SLOT_STREAMS,       // per-stream image lists of the fused matrix-core encoder
   SLOT_COUNT
 };
 
