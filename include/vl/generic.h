@@ -4,7 +4,7 @@
  * VLAD and k-means interfaces and their callers use: type codes
  * (generic.h:35-44), type sizes (84-99), allocator and printf hooks (138-156),
  * thread count (114-116), SIMD switch (104-108), the per-thread random
- * generator (110) and timers (204-206).
+ * generator (110), the per-thread error state (120-132) and timers (204-206).
  */
 #ifndef VL_GENERIC_H
 #define VL_GENERIC_H
@@ -52,6 +52,20 @@ VL_EXPORT vl_bool vl_cpu_has_sse3 (void) ;
 VL_EXPORT vl_bool vl_cpu_has_sse2 (void) ;
 VL_EXPORT vl_size vl_get_num_cpus (void) ;
 VL_EXPORT VlRand * vl_get_rand (void) ;
+
+/* Error state of the calling thread (generic.h:120-132, generic.c:1238-1275):
+ * the code and message of the last error set with vl_set_last_error. */
+#define VL_ERR_MSG_LEN 1024
+#define VL_ERR_OK       0
+#define VL_ERR_OVERFLOW 1
+#define VL_ERR_ALLOC    2
+#define VL_ERR_BAD_ARG  3
+#define VL_ERR_IO       4
+#define VL_ERR_EOF      5
+#define VL_ERR_NO_MORE  5
+VL_EXPORT int vl_get_last_error (void) ;
+VL_EXPORT char const * vl_get_last_error_message (void) ;
+VL_EXPORT int vl_set_last_error (int error, char const * errorMessage, ...) ;
 
 VL_EXPORT vl_size vl_get_max_threads (void) ;
 VL_EXPORT void vl_set_num_threads (vl_size n) ;

@@ -1,8 +1,9 @@
 /* kdtree.h -- randomised kd-forest used by the approximate k-means
  * refinement (VLFeat 0.9.21 vl/kdtree.h).
  *
- * Replaces vl_kdforest_new / build / query / delete (kdtree.c:332-380,
- * 530-580, 774-847) for float and double data and the L2 distance.  The build
+ * Replaces vl_kdforest_new / build / query / query_with_array / delete and
+ * the parameter getters (kdtree.c:332-380, 530-580, 774-1075; kdtree.h:
+ * 136-180) for float and double data and the L2 distance.  The build
  * draws from the calling thread's generator exactly as the reference does
  * (variance sampling and split-dimension choice, kdtree.c:206 and 260), so
  * k-means runs that interleave tree builds with other random draws stay in
@@ -46,5 +47,13 @@ VL_EXPORT vl_size vl_kdforestsearcher_query (VlKDForestSearcher * self, VlKDFore
                                              vl_size numNeighbors, void const * query) ;
 VL_EXPORT vl_size vl_kdforest_query (VlKDForest * self, VlKDForestNeighbor * neighbors,
                                      vl_size numNeighbors, void const * query) ;
+VL_EXPORT vl_size vl_kdforest_query_with_array (VlKDForest * self, vl_uint32 * index, vl_size numNeighbors,
+                                                vl_size numQueries, void * distance, void const * queries) ;
+VL_EXPORT vl_size vl_kdforest_get_num_trees (VlKDForest const * self) ;
+VL_EXPORT vl_size vl_kdforest_get_data_dimension (VlKDForest const * self) ;
+VL_EXPORT vl_type vl_kdforest_get_data_type (VlKDForest const * self) ;
+VL_EXPORT VlKDTreeThresholdingMethod vl_kdforest_get_thresholding_method (VlKDForest const * self) ;
+VL_EXPORT VlKDForest * vl_kdforest_searcher_get_forest (VlKDForestSearcher const * self) ;
+VL_EXPORT VlKDForestSearcher * vl_kdforest_get_searcher (VlKDForest const * self, vl_uindex pos) ;
 
 #endif /* VL_KDTREE_H */

@@ -95,3 +95,83 @@ def test_random_selection_seeding_matches_reference(dtype):
         res.append(lib.kmeans_centers(km, dtype))
         lib.vl_kmeans_delete(km)
     assert np.array_equal(res[0], res[1])
+
+
+@needs_ref
+@pytest.mark.parametrize("dtype,trees,comparisons", [(np.float32, 2, 30), (np.float64, 3, 0)])
+def test_kdforest_query_with_array_and_getters_match_reference(dtype, trees, comparisons):
+    # the array query of kdtree.c:870 over several host threads, as
+    # toolbox/misc/vl_kdtreequery.c and apps/recognition/encodeImage.m use it
+    rng = np.random.default_rng(31)
+    pts = rng.standard_normal((1500, 12)).astype(dtype)
+    queries = rng.standard_normal((400, 12)).astype(dtype)
+    nn = 4
+    out = []
+    for lib in (vlfeat_amd.lib(), reference()):
+        lib.seed(5)
+        kd, f = _kdforest(lib, pts, trees, comparisons)
+        kd.vl_kdforest_query_with_array.restype = C.c_uint64
+        kd.vl_kdforest_query_with_array.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p,
+                                                     C.c_void_p]
+        for name, res in (("vl_kdforest_get_num_trees", C.c_uint64), ("vl_kdforest_get_data_dimension", C.c_uint64),
+                          ("vl_kdforest_get_data_type", C.c_uint32),
+                          ("vl_kdforest_get_thresholding_method", C.c_int)):
+            getattr(kd, name).restype = res
+            getattr(kd, name).argtypes = [C.c_void_p]
+        lib.set_num_threads(4)
+        idx = np.empty((400, nn), np.uint32)
+        dist = np.empty((400, nn), dtype)
+        ncomp = kd.vl_kdforest_query_with_array(f, idx.ctypes.data, nn, 400, dist.ctypes.data, queries.ctypes.data)
+        lib.set_num_threads(0)
+        getters = (kd.vl_kdforest_get_num_trees(f), kd.vl_kdforest_get_data_dimension(f),
+                   kd.vl_kdforest_get_data_type(f), kd.vl_kdforest_get_thresholding_method(f))
+        single = []
+        nb = (Neighbor * nn)()
+        for qv in queries[:20]:
+            kd.vl_kdforest_query(f, nb, nn, qv.ctypes.data)
+            single.append([nb[i].index for i in range(nn)])
+        kd.vl_kdforest_delete(f)
+        out.append((idx, dist, ncomp, getters, single))
+    (i0, d0, c0, g0, s0), (i1, d1, c1, g1, s1) = out
+    assert np.array_equal(i0, i1) and np.array_equal(d0, d1)
+    assert c0 == c1 and g0 == g1 and s0 == s1
+    assert g0 == (trees, 12, 1 if dtype == np.float32 else 2, 0)
+    assert [list(r) for r in i0[:20]] == s0  # the array form equals one query at a time
+
+
+def test_error_state_is_per_thread():
+    import threading
+    L = vlfeat_amd.lib().lib
+    L.vl_set_last_error.restype = C.c_int
+    L.vl_set_last_error.argtypes = [C.c_int, C.c_char_p, C.c_int]
+    L.vl_get_last_error.restype = C.c_int
+    L.vl_get_last_error_message.restype = C.c_char_p
+    assert L.vl_set_last_error(3, b"bad argument %d", 42) == 3
+    assert L.vl_get_last_error() == 3 and L.vl_get_last_error_message() == b"bad argument 42"
+    seen = []
+
+    def other():
+        seen.append((L.vl_get_last_error(), L.vl_get_last_error_message()))
+        L.vl_set_last_error(4, None, 0)
+        seen.append((L.vl_get_last_error(), L.vl_get_last_error_message()))
+    t = threading.Thread(target=other)
+    t.start()
+    t.join()
+    assert seen == [(0, b""), (4, b"")]
+    assert L.vl_get_last_error() == 3 and L.vl_get_last_error_message() == b"bad argument 42"
+    L.vl_set_last_error(0, None, 0)
+
+
+def test_kmeans_rejects_l1_distance_with_a_message():
+    # only the L2 distance is provided: an L1 k-means aborts with a message
+    # before any device work (no GPU needed)
+    import subprocess
+    import sys
+    from pathlib import Path
+    code = ("import ctypes as C, numpy as np, vlfeat_amd as va; L = va.lib(); "
+            "km = L.vl_kmeans_new(1, 0); x = np.zeros((100, 4), np.float32); "
+            "L.vl_kmeans_cluster(km, x.ctypes.data, 4, 100, 3)")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120,
+                       cwd=str(Path(__file__).resolve().parent.parent))
+    assert r.returncode != 0
+    assert "only the L2 distance" in r.stdout + r.stderr

@@ -17,6 +17,7 @@ Context& ctx() { return g_ctx; }
   vl_printf_func_t pf = vl_get_printf_func();
   if (pf) {
     pf("vlhip: fatal: %s failed at %s:%d: %s\n", what, file, line, detail ? detail : "");
+    std::fflush(stdout);  // the hook's text must not stay in a buffer at abort()
   }
   std::fprintf(stderr, "vlhip: fatal: %s failed at %s:%d: %s\n", what, file, line, detail ? detail : "");
   std::fflush(stderr);

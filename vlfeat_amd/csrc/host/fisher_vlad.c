@@ -8,6 +8,7 @@
  * and copy the encoding back into the caller's buffer.  An unknown data type
  * aborts, as in the reference.
  */
+#include "host_fatal.h"
 #include "vl/fisher.h"
 #include "vl/vlad.h"
 #include "vlhip.h"
@@ -24,8 +25,7 @@ vl_fisher_set_precision (int precision)
 {
   if (precision != VL_FISHER_PRECISION_FP32 && precision != VL_FISHER_PRECISION_BF16X3 &&
       precision != VL_FISHER_PRECISION_F16X3 && precision != VL_FISHER_PRECISION_GATED) {
-    VL_PRINTF("vl_fisher_set_precision: unknown precision %d\n", precision) ;
-    abort() ;
+    vl_host_fatal("vl_fisher_set_precision: unknown precision %d\n", precision) ;
   }
   g_precision = precision ;
 }
@@ -42,8 +42,7 @@ static void
 check_type (vl_type t, char const * who)
 {
   if (t != VL_TYPE_FLOAT && t != VL_TYPE_DOUBLE) {
-    VL_PRINTF("%s: unsupported data type %u\n", who, (unsigned)t) ;
-    abort() ;
+    vl_host_fatal("%s: unsupported data type %u\n", who, (unsigned)t) ;
   }
 }
 

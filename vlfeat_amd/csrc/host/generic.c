@@ -4,6 +4,7 @@
  * Semantics follow vl/generic.c (VLFeat 0.9.21): the allocator hooks default
  * to the C library (generic.c:1289-1362), the printf hook to printf
  * (generic.c:1385-1400), SIMD is enabled by default (generic.c:1602-1632), and
+ * the error code and message are per calling thread (generic.c:1238-1275),
  * every calling thread owns its own MT19937 generator seeded on first use by
  * vl_rand_init (generic.c:1460-1464, state created in generic.c:1512-1540:
  * vl_rand_init leaves the generator unseeded, so its first draw seeds it with
@@ -11,6 +12,7 @@
  */
 #include "vl/generic.h"
 #include "vl/random.h"
+#include "host_fatal.h"
 
 #include <pthread.h>
 #include <stdarg.h>
@@ -102,6 +104,46 @@ vl_get_rand (void)
   }
   return r ;
 }
+
+/* ---- fatal errors ---------------------------------------------------------- */
+
+void
+vl_host_fatal (char const * format, ...)
+{
+  char msg [1024] ;
+  va_list args ;
+  va_start(args, format) ;
+  vsnprintf(msg, sizeof(msg), format, args) ;
+  va_end(args) ;
+  if (g_printf != quiet_printf) g_printf("%s", msg) ;
+  fflush(stdout) ;
+  fprintf(stderr, "%s", msg) ;
+  fflush(stderr) ;
+  abort() ;
+}
+
+/* ---- error state of the calling thread ------------------------------------ */
+
+static __thread int g_last_error = VL_ERR_OK ;
+static __thread char g_last_error_message [VL_ERR_MSG_LEN] ;
+
+VL_EXPORT int
+vl_set_last_error (int error, char const * errorMessage, ...)
+{
+  va_list args ;
+  va_start(args, errorMessage) ;
+  if (errorMessage) {
+    vsnprintf(g_last_error_message, sizeof(g_last_error_message), errorMessage, args) ;
+  } else {
+    g_last_error_message[0] = 0 ;
+  }
+  va_end(args) ;
+  g_last_error = error ;
+  return error ;
+}
+
+VL_EXPORT int vl_get_last_error (void) { return g_last_error ; }
+VL_EXPORT char const * vl_get_last_error_message (void) { return g_last_error_message ; }
 
 /* ---- timers --------------------------------------------------------------- */
 

@@ -16,6 +16,7 @@
  * through vl/kmeans.h and the host random generator exactly as gmm.c:1315-1443
  * does, so identical seeds give identical starting points.
  */
+#include "host_fatal.h"
 #include "vl/gmm.h"
 #include "vlhip.h"
 
@@ -216,8 +217,7 @@ vl_gmm_set_precision (VlGMM * self, int precision)
 {
   if (precision != VL_GMM_PRECISION_FP32 && precision != VL_GMM_PRECISION_BF16X3 &&
       precision != VL_GMM_PRECISION_F16X3) {
-    VL_PRINTF("vl_gmm_set_precision: unknown precision %d\n", precision) ;
-    abort() ;
+    vl_host_fatal("vl_gmm_set_precision: unknown precision %d\n", precision) ;
   }
   self->precision = precision ;
 }

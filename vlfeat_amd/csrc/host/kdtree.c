@@ -20,6 +20,7 @@
 #include "vl/kdtree.h"
 
 #include <assert.h>
+#include <stdlib.h>
 #include <string.h>
 
 typedef struct {
@@ -329,8 +330,13 @@ vl_kdforest_new_searcher (VlKDForest * f)
   s->forest = f ;
   s->heap = (KdState*) vl_malloc(sizeof(KdState) * (f->maxNumNodes ? f->maxNumNodes : 1)) ;
   s->idBook = (vl_uindex*) vl_calloc(f->numData ? f->numData : 1, sizeof(vl_uindex)) ;
-  s->next = f->searchers ;
-  f->searchers = s ;
+  /* appended: searchers are numbered in creation order (kdtree.c:387-407) */
+  {
+    VlKDForestSearcher ** p = &f->searchers ;
+    while (*p) p = &(*p)->next ;
+    s->next = NULL ;
+    *p = s ;
+  }
   return s ;
 }
 
@@ -466,4 +472,65 @@ vl_kdforest_query (VlKDForest * self, VlKDForestNeighbor * nb, vl_size numNb, vo
 {
   VlKDForestSearcher * s = self->searchers ? self->searchers : vl_kdforest_new_searcher(self) ;
   return vl_kdforestsearcher_query(s, nb, numNb, query) ;
+}
+
+/* kdtree.c:870-946: numQueries queries at once, the queries split over
+ * vl_get_max_threads() host threads, each with its own searcher; indexes and
+ * distances are numNeighbors x numQueries (distances in the data type, may be
+ * NULL).  Returns the total number of comparisons. */
+VL_EXPORT vl_size
+vl_kdforest_query_with_array (VlKDForest * self, vl_uint32 * indexes, vl_size numNeighbors,
+                              vl_size numQueries, void * distances, void const * queries)
+{
+  vl_size total = 0 ;
+  vl_size const dim = self->dimension ;
+  vl_type const type = self->dataType ;
+  if (type != VL_TYPE_FLOAT && type != VL_TYPE_DOUBLE) abort() ;
+#pragma omp parallel num_threads((int)vl_get_max_threads()) reduction(+:total)
+  {
+    VlKDForestSearcher * s ;
+    VlKDForestNeighbor * nb ;
+    vl_index q ;
+#pragma omp critical (vl_kdforest_searchers)
+    {
+      s = vl_kdforest_new_searcher(self) ;
+      nb = (VlKDForestNeighbor*) vl_calloc(numNeighbors ? numNeighbors : 1, sizeof(VlKDForestNeighbor)) ;
+    }
+#pragma omp for
+    for (q = 0 ; q < (vl_index)numQueries ; ++q) {
+      vl_uindex n ;
+      void const * query = (type == VL_TYPE_FLOAT) ? (void const*)((float const*)queries + q * dim)
+                                                    : (void const*)((double const*)queries + q * dim) ;
+      total += vl_kdforestsearcher_query(s, nb, numNeighbors, query) ;
+      for (n = 0 ; n < numNeighbors ; ++n) {
+        indexes[q * numNeighbors + n] = (vl_uint32) nb[n].index ;
+        if (distances) {
+          if (type == VL_TYPE_FLOAT) ((float*)distances)[q * numNeighbors + n] = (float) nb[n].distance ;
+          else ((double*)distances)[q * numNeighbors + n] = nb[n].distance ;
+        }
+      }
+    }
+#pragma omp critical (vl_kdforest_searchers)
+    {
+      vl_kdforestsearcher_delete(s) ;
+      vl_free(nb) ;
+    }
+  }
+  return total ;
+}
+
+VL_EXPORT vl_size vl_kdforest_get_num_trees (VlKDForest const * self) { return self->numTrees ; }
+VL_EXPORT vl_size vl_kdforest_get_data_dimension (VlKDForest const * self) { return self->dimension ; }
+VL_EXPORT vl_type vl_kdforest_get_data_type (VlKDForest const * self) { return self->dataType ; }
+VL_EXPORT VlKDTreeThresholdingMethod
+vl_kdforest_get_thresholding_method (VlKDForest const * self) { return self->thresholdingMethod ; }
+VL_EXPORT VlKDForest * vl_kdforest_searcher_get_forest (VlKDForestSearcher const * self) { return self->forest ; }
+
+/* the pos-th searcher in creation order, NULL past the last (kdtree.c:443-452) */
+VL_EXPORT VlKDForestSearcher *
+vl_kdforest_get_searcher (VlKDForest const * self, vl_uindex pos)
+{
+  VlKDForestSearcher * s = self->searchers ;
+  while (s && pos) { s = s->next ; --pos ; }
+  return s ;
 }

@@ -17,6 +17,7 @@
  * Lloyd solution (both are exact Lloyd iterations; Elkan only skips distance
  * evaluations on the CPU).
  */
+#include "host_fatal.h"
 #include "vl/kmeans.h"
 #include "vl/kdtree.h"
 #include "vlhip.h"
@@ -38,8 +39,7 @@ static void
 require_l2 (VlKMeans const * self, char const * who)
 {
   if (self->distance != VlDistanceL2) {
-    VL_PRINTF("%s: only the L2 distance is provided by this library\n", who) ;
-    abort() ;
+    vl_host_fatal("%s: only the L2 distance is provided by this library\n", who) ;
   }
 }
 
