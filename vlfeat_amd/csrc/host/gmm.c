@@ -418,7 +418,7 @@ vl_gmm_cluster (VlGMM * self, void const * data, vl_size numData)
     double LL ;
     double timeRef ;
     if (self->verbosity) {
-      VL_PRINTF("gmm: clustering: starting repetition %d of %d\n", (int)repetition + 1, (int)self->numRepetitions) ;
+      VL_PRINTF("gmm: fit %d of %d\n", (int)repetition + 1, (int)self->numRepetitions) ;
     }
     timeRef = vl_get_cpu_time() ;
     switch (self->initialization) {
@@ -428,12 +428,12 @@ vl_gmm_cluster (VlGMM * self, void const * data, vl_size numData)
       default : abort() ;
     }
     if (self->verbosity) {
-      VL_PRINTF("gmm: model initialized in %.2f s\n", vl_get_cpu_time() - timeRef) ;
+      VL_PRINTF("gmm: initial model ready after %.2f s\n", vl_get_cpu_time() - timeRef) ;
     }
     timeRef = vl_get_cpu_time() ;
     LL = vl_gmm_em(self, data, numData) ;
     if (self->verbosity) {
-      VL_PRINTF("gmm: optimization terminated in %.2f s with loglikelihood %f\n",
+      VL_PRINTF("gmm: EM done after %.2f s, log-likelihood %f\n",
                 vl_get_cpu_time() - timeRef, LL) ;
     }
     if (LL > bestLL || repetition == 0) {
@@ -462,7 +462,7 @@ vl_gmm_cluster (VlGMM * self, void const * data, vl_size numData)
   if (self->posteriors) { vl_free(self->posteriors) ; self->posteriors = NULL ; }
   self->LL = bestLL ;
   if (self->verbosity) {
-    VL_PRINTF("gmm: all repetitions terminated with final loglikelihood %f\n", self->LL) ;
+    VL_PRINTF("gmm: best log-likelihood over the fits %f\n", self->LL) ;
   }
   return bestLL ;
 }

@@ -80,65 +80,68 @@ FN(gmm_finalize) (VlGMM * self, double const * out)
   for (k = 0 ; k < K ; ++k) priors[k] /= sum ;
 }
 
-/* Empty-mode restarts, gmm.c:815-999.  mass = total posterior mass per
- * component (no threshold).  Returns the number of splits performed. */
+/* Empty-component restarts, the rule of gmm.c:815-999.  A component whose
+ * total posterior mass (massIn, no threshold) is below
+ * GMM_MIN_POSTERIOR * max(1, N / K) is refilled by splitting the component
+ * with the largest term in the mixture's entropy,
+ *     pi_k (D/2 (1 + log 2 pi) + 1/2 sum_d log sigma_kd^2 - log pi_k)
+ * (summed in that order, so that the same donor wins as in the reference),
+ * along the donor's dimension of largest variance at its mean: the GPU moves
+ * the donor's posteriors on one side to the empty component, and the M-step
+ * is redone after every split.  Returns the number of splits. */
 static vl_size
 FN(gmm_restart) (VlGMM * self, vl_size numData, double const * massIn)
 {
-  vl_size K = self->numClusters, D = self->dimension ;
-  TYPE * priors = (TYPE*) self->priors ;
-  TYPE * means = (TYPE*) self->means ;
-  TYPE * covs = (TYPE*) self->covariances ;
+  vl_size const K = self->numClusters, D = self->dimension ;
+  TYPE const * priors = (TYPE const*) self->priors ;
+  TYPE const * means = (TYPE const*) self->means ;
+  TYPE const * covs = (TYPE const*) self->covariances ;
+  double const threshold = GMM_MIN_POSTERIOR * VL_MAX(1.0, (double)numData / K) ;
   double * mass ;
-  double * out ;
-  vl_index i_cl, j_cl, d ;
+  double * moments ;
+  vl_index target, k, d ;
   vl_size splits = 0 ;
   if (K <= 1) return 0 ;
   mass = (double*) vl_malloc(sizeof(double) * K) ;
   memcpy(mass, massIn, sizeof(double) * K) ;
-  out = (double*) vl_malloc(sizeof(double) * (2 * K + 2 * K * D)) ;
+  moments = (double*) vl_malloc(sizeof(double) * (2 * K + 2 * K * D)) ;
 
-  for (i_cl = 0 ; i_cl < (signed)K ; ++i_cl) {
-    double size = -VL_INFINITY_D ;
-    vl_index best = -1 ;
-    if (mass[i_cl] >= GMM_MIN_POSTERIOR * VL_MAX(1.0, (double)numData / K)) continue ;
+  for (target = 0 ; target < (signed)K ; ++target) {
+    double donorTerm = -VL_INFINITY_D, widest = -VL_INFINITY_D ;
+    vl_index donor = -1, axis = -1 ;
+    if (mass[target] >= threshold) continue ;
     if (self->verbosity) {
-      VL_PRINTF("gmm: mode %d is nearly empty (mass %f)\n", (int)i_cl, mass[i_cl]) ;
+      VL_PRINTF("gmm: component %d carries almost no posterior mass (%f)\n", (int)target, mass[target]) ;
     }
-    /* component with the largest entropy contribution, gmm.c:908-935 */
-    for (j_cl = 0 ; j_cl < (signed)K ; ++j_cl) {
-      double size_ ;
-      if (priors[j_cl] < GMM_MIN_PRIOR) continue ;
-      size_ = 0.5 * D * (1.0 + log(2 * VL_PI)) ;
-      for (d = 0 ; d < (signed)D ; ++d) size_ += 0.5 * log((double)covs[j_cl * D + d]) ;
-      size_ = priors[j_cl] * (size_ - log((double)priors[j_cl])) ;
-      if (size_ > size) { size = size_ ; best = j_cl ; }
+    for (k = 0 ; k < (signed)K ; ++k) {
+      double term ;
+      if (priors[k] < GMM_MIN_PRIOR) continue ;
+      term = 0.5 * D * (1.0 + log(2 * VL_PI)) ;
+      for (d = 0 ; d < (signed)D ; ++d) term += 0.5 * log((double)covs[k * D + d]) ;
+      term = priors[k] * (term - log((double)priors[k])) ;
+      if (term > donorTerm) { donorTerm = term ; donor = k ; }
     }
-    j_cl = best ;
-    if (j_cl == i_cl || j_cl < 0) {
+    if (donor == target || donor < 0) {
       if (self->verbosity) {
-        VL_PRINTF("gmm: mode %d is empty, but no other mode to split could be found\n", (int)i_cl) ;
+        VL_PRINTF("gmm: no other component can be split to refill component %d\n", (int)target) ;
       }
       continue ;
     }
     if (self->verbosity) {
-      VL_PRINTF("gmm: reinitializing empty mode %d with mode %d (prior %f, mass %f, score %f)\n",
-                (int)i_cl, (int)j_cl, (double)priors[j_cl], mass[j_cl], size) ;
+      VL_PRINTF("gmm: refilling component %d by splitting component %d (prior %f, mass %f, entropy term %f)\n",
+                (int)target, (int)donor, (double)priors[donor], mass[donor], donorTerm) ;
     }
-    /* dimension of maximum variance, gmm.c:958-966 */
-    size = -VL_INFINITY_D ;
-    best = -1 ;
     for (d = 0 ; d < (signed)D ; ++d) {
-      double s2 = covs[j_cl * D + d] ;
-      if (s2 > size) { size = s2 ; best = d ; }
+      double const v = covs[donor * D + d] ;
+      if (v > widest) { widest = v ; axis = d ; }
     }
-    vlhip_gmm_split(TYPE_CODE, self->dPost, self->dData, numData, D, K, (size_t)i_cl, (size_t)j_cl,
-                    (size_t)best, (double)means[best + j_cl * D]) ;
-    FN(gmm_moments)(self, out, numData) ;
-    FN(gmm_finalize)(self, out) ;
+    vlhip_gmm_split(TYPE_CODE, self->dPost, self->dData, numData, D, K, (size_t)target, (size_t)donor,
+                    (size_t)axis, (double)means[axis + donor * D]) ;
+    FN(gmm_moments)(self, moments, numData) ;
+    FN(gmm_finalize)(self, moments) ;
     splits ++ ;
   }
-  vl_free(out) ;
+  vl_free(moments) ;
   vl_free(mass) ;
   return splits ;
 }
@@ -199,15 +202,15 @@ FN(gmm_em) (VlGMM * self, TYPE const * data, vl_size numData)
                                 p + 2 * sizeof(TYPE) * KD, p, D, p + sizeof(TYPE) * KD, self->dData) ;
     }
     if (self->verbosity > 1) {
-      VL_PRINTF("gmm: em: expectation step completed in %.2f s\n", vl_get_cpu_time() - time) ;
+      VL_PRINTF("gmm: EM: posteriors of all data in %.2f s\n", vl_get_cpu_time() - time) ;
     }
     if (self->verbosity) {
-      VL_PRINTF("gmm: em: iteration %d: loglikelihood = %f (variation = %f)\n",
+      VL_PRINTF("gmm: EM step %d: log-likelihood %f, change %f\n",
                 (int)iteration, LL, LL - previousLL) ;
     }
     if (iteration >= self->maxNumIterations) {
       if (self->verbosity) {
-        VL_PRINTF("gmm: em: terminating because the maximum number of iterations (%d) has been reached.\n",
+        VL_PRINTF("gmm: EM stops after the iteration limit (%d)\n",
                   (int)self->maxNumIterations) ;
       }
       break ;
@@ -215,7 +218,7 @@ FN(gmm_em) (VlGMM * self, TYPE const * data, vl_size numData)
     eps = fabs((LL - previousLL) / LL) ;
     if ((iteration > 0) && (eps < 0.00001)) {
       if (self->verbosity) {
-        VL_PRINTF("gmm: em: terminating because the algorithm fully converged (log-likelihood variation = %f).\n", eps) ;
+        VL_PRINTF("gmm: EM stops: relative log-likelihood change %f below the tolerance\n", eps) ;
       }
       break ;
     }

@@ -252,7 +252,7 @@ vl_kmeans_cluster (VlKMeans * self, void const * data, vl_size dimension,
     double energy ;
     double timeRef ;
     if (self->verbosity) {
-      VL_PRINTF("kmeans: repetition %d of %d\n", (int)repetition + 1, (int)self->numRepetitions) ;
+      VL_PRINTF("kmeans: run %d of %d\n", (int)repetition + 1, (int)self->numRepetitions) ;
     }
     timeRef = vl_get_cpu_time() ;
     switch (self->initialization) {
@@ -266,12 +266,12 @@ vl_kmeans_cluster (VlKMeans * self, void const * data, vl_size dimension,
         abort() ;
     }
     if (self->verbosity) {
-      VL_PRINTF("kmeans: K-means initialized in %.2f s\n", vl_get_cpu_time() - timeRef) ;
+      VL_PRINTF("kmeans: seeds chosen after %.2f s\n", vl_get_cpu_time() - timeRef) ;
     }
     timeRef = vl_get_cpu_time() ;
     energy = vl_kmeans_refine_centers(self, data, numData) ;
     if (self->verbosity) {
-      VL_PRINTF("kmeans: K-means terminated in %.2f s with energy %g\n", vl_get_cpu_time() - timeRef, energy) ;
+      VL_PRINTF("kmeans: refinement done after %.2f s, energy %g\n", vl_get_cpu_time() - timeRef, energy) ;
     }
     /* repetition 0 is kept even when its energy is NaN (kmeans.c:2070-2073) */
     if (energy < bestEnergy || repetition == 0) {

@@ -258,7 +258,7 @@ FN(km_refine) (VlKMeans * self, TYPE const * data, vl_size numData)
     energy = 0 ;
     for (x = 0 ; x < numData ; ++x) energy += distances[x] ;
     if (self->verbosity) {
-      VL_PRINTF("kmeans: %s iter %d: energy = %g\n", ann ? "ANN" : "Lloyd", (int)iteration, energy) ;
+      VL_PRINTF("kmeans: %s step %d, energy %g\n", ann ? "ANN" : "Lloyd", (int)iteration, energy) ;
     }
     if (iteration >= self->maxNumIterations) break ;
     if (energy == previousEnergy) break ;
@@ -289,11 +289,11 @@ FN(km_refine) (VlKMeans * self, TYPE const * data, vl_size numData)
     }
     vlhip_memcpy_h2d(dUpd, upd, updBytes) ;
     vlhip_kmeans_update_centers(TYPE_CODE, dCenters, dData, (vl_uint32 const*) dUpd,
-                                (vl_uint64 const*) (dUpd + memBytes), (vl_uint64 const*) (dUpd + memBytes) + K + 1,
+                                (uint64_t const*) (dUpd + memBytes), (uint64_t const*) (dUpd + memBytes) + K + 1,
                                 K, D) ;
     vlhip_memcpy_d2h(self->centers, dCenters, es * D * K) ;
     if (self->verbosity && numRestarted) {
-      VL_PRINTF("kmeans: iter %d: restarted %d centers\n", (int)iteration, (int)numRestarted) ;
+      VL_PRINTF("kmeans: step %d moved %d empty centers to random data\n", (int)iteration, (int)numRestarted) ;
     }
   }
   vlhip_unlock() ;
