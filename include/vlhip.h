@@ -240,6 +240,15 @@ VLHIP_API void vlhip_kmeans_quantize_noalloc (int dtype, uint32_t * assignments,
                                               void const * data, void const * centers,
                                               size_t numData, size_t dimension, size_t numCenters, int lanes) ;
 
+/* Statistics of the quantizer's matrix-core screen, counted while kernel
+ * timing is on (vlhip_timing_enable): counts[0] rows screened, counts[1]
+ * exact distances evaluated for listed candidates (a center, or every center
+ * one lane of the screen saw when two of them were near-ties), counts[2] rows
+ * scanned exactly over all centers (no finite screen value, or more list
+ * entries than the screen keeps).  A
+ * diagnostic with no counterpart in the reference.  reset != 0 clears them. */
+VLHIP_API void vlhip_kmeans_screen_stats (uint64_t * counts, int reset) ;
+
 /* Squared L2 distance of every datum to one point, same lane order
  * (used by k-means++ seeding, kmeans.c:610-616). */
 VLHIP_API void vlhip_l2_to_point (int dtype, void * distances, void const * data, void const * point,

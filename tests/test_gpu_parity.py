@@ -481,6 +481,52 @@ def test_quantize_screened_ties_and_scales(D, K, scale):
 
 
 @needs_ref
+@pytest.mark.parametrize("K", [512, 1030])
+def test_quantize_screened_crowded_windows(K):
+    # the screen keeps at most two centres per lane within 2 eps of the lane's
+    # running minimum and sends the row to the full scan beyond that; here
+    # rows sit at the centre of 2D centres x0 +- 0.5 e_d, all at exactly the
+    # same distance 0.25 (every lane overflows; the first index must win), or
+    # just off it (near ties inside the window), or sweep past a decreasing
+    # run of centres (the window restarts when the minimum drops by more than
+    # 2 eps)
+    D = 80
+    rng = np.random.default_rng(K)
+    cen = (rng.standard_normal((K, D)) * 4).astype(np.float32)
+    x0 = np.round(rng.standard_normal(D)).astype(np.float32)
+    ring = np.concatenate([x0 + 0.5 * np.eye(D, dtype=np.float32), x0 - 0.5 * np.eye(D, dtype=np.float32)])
+    cen[100:100 + 2 * D] = ring[rng.permutation(2 * D)]
+    # a run of centres approaching x1, one per lane position, farther first
+    x1 = (rng.standard_normal(D) * 2).astype(np.float32)
+    u = rng.standard_normal(D).astype(np.float32)
+    u /= np.linalg.norm(u)
+    for j, k in enumerate(range(300, 300 + 16 * 12, 16)):
+        cen[k] = x1 + np.float32(3.0 * 0.5 ** j) * u
+    x = (rng.standard_normal((4000, D)) * 4).astype(np.float32)
+    x[:40] = x0
+    x[40:80] = x0 + (rng.standard_normal((40, D)) * 1e-4).astype(np.float32)
+    x[80:120] = x1
+    x[120:160] = x1 + (rng.standard_normal((40, D)) * 1e-3).astype(np.float32)
+    # rows without a finite distance take the full scan: distance +inf, the
+    # first centre (the reference leaves their assignment undefined)
+    x[160:163] = np.nan
+    x[163, 7] = np.nan
+    a, d = va.vl_kmeans_quantize(x, cen)
+    assert np.all(a[160:164] == 0) and np.all(d[160:164] == np.inf)
+    R = reference()
+    km = R.kmeans_new(np.float32)
+    R.vl_kmeans_set_centers(km, cen.ctypes.data, D, K)
+    ar, dr = R.kmeans_quantize(km, x)
+    R.vl_kmeans_delete(km)
+    # all 2D ring centres tie at exactly 0.25: the first of them, 100, wins
+    assert np.all(dr[:40] == np.float32(0.25)) and np.all(ar[:40] == 100)
+    fin = np.ones(len(x), bool)
+    fin[160:164] = False
+    assert np.array_equal(a[fin], ar[fin])
+    assert np.array_equal(d, dr)
+
+
+@needs_ref
 @pytest.mark.parametrize("algo,comparisons", [(va.VlKMeansLloyd, 100), (va.VlKMeansANN, 20),
                                               (va.VlKMeansANN, 5000)])
 def test_kmeans_cluster_matches_reference(algo, comparisons):

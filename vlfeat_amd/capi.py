@@ -149,6 +149,7 @@ _EXT_SIGS = {
     "vlhip_vlad_encode_batch": (None, [C.c_int, _p, _p, C.c_size_t, C.c_size_t, _p, _p, C.c_size_t,
                                        _p, _p, C.c_int]),
     "vlhip_kmeans_quantize": (None, [C.c_int, _p, _p, _p, _p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_int]),
+    "vlhip_kmeans_screen_stats": (None, [_p, C.c_int]),
     "vlhip_kmeans_quantize_noalloc": (None, [C.c_int, _p, _p, _p, _p, C.c_size_t, C.c_size_t, C.c_size_t,
                                              C.c_int]),
     "vlhip_encode_plan_new": (_p, [_sz, _sz, _p, _sz, C.c_int, C.c_int]),
