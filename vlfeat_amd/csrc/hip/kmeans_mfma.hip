@@ -344,10 +344,7 @@ __global__ __launch_bounds__(QM_WAVES * 64, 2) void qm_quantize_kernel(
   for (int t = 0; t < 2; ++t)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      float M = m1[t][r];
-#pragma unroll
-      for (int o = 1; o < 16; o <<= 1) M = fminf(M, __shfl_xor(M, o, 64));
-      const float lim = M + w2[t][r];
+      const float lim = row16_min(m1[t][r]) + w2[t][r];
       const int rw = 16 * t + 4 * lg + r;
       if (k1[t][r] >= 0 && m1[t][r] <= lim) {
         const int slot = atomicAdd(&cnt[w][rw], 1);

@@ -51,4 +51,50 @@ __device__ __forceinline__ double block_sum_256(double v, double* red /* >= 256 
   return r;
 }
 
+// Reductions over the 16 lanes of a DPP row (lanes 16i .. 16i + 15), every
+// lane receiving the result: swaps within pairs and quads, then the row's
+// half mirror and mirror (four VALU operations, no LDS round trip).  IEEE
+// min, max and addition are commutative, so all 16 lanes end with the same
+// bits.
+__device__ __forceinline__ float dpp_f(float v, int ctrl) {
+  switch (ctrl) {  // (the control must be a constant)
+    case 0: return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+    case 1: return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+    case 2: return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));
+    default: return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false));
+  }
+}
+__device__ __forceinline__ int dpp_i(int v, int ctrl) {
+  switch (ctrl) {
+    case 0: return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+    case 1: return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
+    case 2: return __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);
+    default: return __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);
+  }
+}
+__device__ __forceinline__ float row16_max(float v) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v = fmaxf(v, dpp_f(v, c));
+  return v;
+}
+__device__ __forceinline__ float row16_min(float v) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v = fminf(v, dpp_f(v, c));
+  return v;
+}
+__device__ __forceinline__ float row16_sum(float v) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v = v + dpp_f(v, c);
+  return v;
+}
+// first maximum of (value, index) pairs: larger value, then smaller index
+__device__ __forceinline__ void row16_argmax(float& v, int& i) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const float ov = dpp_f(v, c);
+    const int oi = dpp_i(i, c);
+    if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+  }
+}
+
 }  // namespace vlhip
