@@ -52,6 +52,8 @@ KERNEL_FLOP_PER_DESC = {
     "mfma_lse": 4 * DIM * K_COMP,      # logits for the row statistics
     "fisher_accum": 4 * DIM * K_COMP,  # FP32 accumulation
     "fisher_estep": 4 * DIM * K_COMP,  # FP32 E-step
+    "f32_fisher": 8 * DIM * K_COMP,    # FP32 matrix cores: logits + moments of its components
+    "f32_lse": 4 * DIM * K_COMP,       # FP32 matrix cores: logits for the row statistics
 }
 # Matrix-core flop actually issued per descriptor and component by the
 # compensated kernels, relative to the algorithmic count:
@@ -60,8 +62,13 @@ KERNEL_FLOP_PER_DESC = {
 #                (2 x 2 x 2D), posterior mass, one 16x16x32 product with ones
 #                per 16 components (32 flop); algorithmic 8D
 #   mfma_lse:    logits only, 3 x 2 x 2D against the algorithmic 4D
+#   f32_fisher:  plain FP32 logits (2 x 2D) and moments (2 x 2D), plus the
+#                posterior mass, one 16x16x4 product with ones per 16
+#                components and 4 rows (32 flop per descriptor and component)
+#   f32_lse:     logits only, exactly the algorithmic count
 EXECUTED_PER_ALGORITHMIC = {"mfma_fisher": (3 * 2 * 2 * DIM + 2 * 2 * 2 * DIM + 32) / (8 * DIM),
-                            "mfma_lse": (3 * 2 * 2 * DIM) / (4 * DIM)}
+                            "mfma_lse": (3 * 2 * 2 * DIM) / (4 * DIM),
+                            "f32_fisher": (8 * DIM + 32) / (8 * DIM), "f32_lse": 1.0}
 
 
 def log(msg: str) -> None:
@@ -664,7 +671,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", default="gated", choices=["fp32", "bf16x3", "f16x3", "gated"])
+    ap.add_argument("--mode", default="gated", choices=["fp32", "fp32mfma", "bf16x3", "f16x3", "gated"])
     ap.add_argument("--images", type=int, default=NUM_IMAGES)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--oracle-images", type=int, default=3)
@@ -719,7 +726,8 @@ def main() -> int:
     pd = torch.as_tensor(p, device=dev)
     offsets = np.arange(n_img + 1, dtype=np.uint64) * PER_IMAGE
     od = torch.as_tensor(offsets.astype(np.int64), device=dev)
-    modes = {"fp32": va.MODE_FP32, "bf16x3": va.MODE_BF16X3, "f16x3": va.MODE_F16X3, "gated": va.MODE_GATED}
+    modes = {"fp32": va.MODE_FP32, "fp32mfma": va.MODE_FP32_MFMA, "bf16x3": va.MODE_BF16X3, "f16x3": va.MODE_F16X3,
+             "gated": va.MODE_GATED}
     torch.cuda.synchronize()
 
     gate_info = None
@@ -780,7 +788,7 @@ def main() -> int:
     desc_per_launch = N / launches_per_step
     flop_per_launch = KERNEL_FLOP_PER_DESC.get(kname, 8 * DIM * K_COMP) * desc_per_launch
     achieved = flop_per_launch / (avg_ms / 1000) / 1e12
-    peak = PEAK_FP32_TFLOPS if args.mode == "fp32" else PEAK_F16_TFLOPS
+    peak = PEAK_FP32_TFLOPS if args.mode in ("fp32", "fp32mfma") else PEAK_F16_TFLOPS
     bpd = traffic_per_descriptor(kname)
     roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                 "traffic": bpd * desc_per_launch if bpd else None, "kernel": kname, "kernel_avg_ms": avg_ms,
@@ -816,7 +824,7 @@ def main() -> int:
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         # gated: the gate network runs in bf16, the exact posteriors and moments of
         # every group it does not serve in compensated fp16 (all groups of this workload)
-        "dtype": {"fp32": "fp32", "bf16x3": "bf16", "f16x3": "fp16", "gated": "fp16"}[args.mode],
+        "dtype": {"fp32": "fp32", "fp32mfma": "fp32", "bf16x3": "bf16", "f16x3": "fp16", "gated": "fp16"}[args.mode],
         "data": "synthetic (SIFT-like rectified/clipped Gaussian projected to 80-d, seed 1; GMM fitted by GPU EM)",
         "config": {"workload": "improved Fisher vectors, 4096 images x 3000 descriptors, K=1024, D=80",
                    "K": K_COMP, "D": DIM, "images": n_img, "descriptors_per_image": PER_IMAGE,
@@ -828,15 +836,22 @@ def main() -> int:
     n_check = max(args.oracle_images, 64 if not args.no_cpu else 0)
     enc_sel = enc[: min(n_img, n_check)].cpu().numpy()
     del enc
+    # The plain FP32 GPU comparator: FP32 operands on the FP32 matrix cores,
+    # posteriors kept in registers (fp32mfma); the reference-parity FP32 path
+    # (VALU arithmetic of the reference, posterior matrix written) beside it.
     enc32 = None
-    if args.mode != "fp32" and not args.no_fp32:
-        ms32, t32, e32, _ = run("fp32", 1, 1)
-        result["fp32_gpu"] = {"value": N / (ms32 / 1000), "ms_per_step": ms32,
-                              "kernels_ms_per_step": {k: v[0] for k, v in t32.items() if v[1]}}
+    if args.mode not in ("fp32", "fp32mfma") and not args.no_fp32:
+        ms32, t32, e32, _ = run("fp32mfma", 2, 1)
+        result["fp32_gpu"] = {"value": N / (ms32 / 1000), "ms_per_step": ms32, "mode": "fp32mfma",
+                              "kernels_ms_per_step": {k: v[0] / 2 for k, v in t32.items() if v[1]},
+                              "fp32_matrix_ceiling_descriptors_per_s": PEAK_FP32_TFLOPS * 1e12 / (8 * DIM * K_COMP)}
         result["speedup_vs_fp32_gpu"] = ms32 / ms_per_step
         enc32 = e32[: min(n_img, n_check)].cpu().numpy()
         del e32
-    elif args.mode == "fp32":
+        msp, tp, _, _ = run("fp32", 1, 1)
+        result["fp32_parity_gpu"] = {"value": N / (msp / 1000), "ms_per_step": msp, "mode": "fp32",
+                                     "kernels_ms_per_step": {k: v[0] for k, v in tp.items() if v[1]}}
+    elif args.mode in ("fp32", "fp32mfma"):
         enc32 = enc_sel
     if args.mode == "gated":
         # the dense compensated-fp16 encoder the gate falls back to, on the same batch

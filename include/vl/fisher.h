@@ -49,6 +49,9 @@ VL_EXPORT void vl_fisher_encode_batch
 #define VL_FISHER_PRECISION_BF16X3   1
 #define VL_FISHER_PRECISION_GATED    2
 #define VL_FISHER_PRECISION_F16X3    3
+/* Plain FP32 operands on the FP32 matrix cores (exact FP32 products, FP32
+ * accumulation, the posteriors never leave registers), D <= 128. */
+#define VL_FISHER_PRECISION_FP32_MFMA 4
 VL_EXPORT void vl_fisher_set_precision (int precision) ;
 VL_EXPORT int vl_fisher_get_precision (void) ;
 

@@ -32,6 +32,7 @@
 #define VLHIP_MODE_BF16X3   1  /* compensated split-BF16 MFMA contractions, FP32 accumulation */
 #define VLHIP_MODE_GATED    2  /* learned gate + exact top-M posteriors, dense fallback */
 #define VLHIP_MODE_F16X3    3  /* compensated split-FP16 MFMA contractions (power-of-two scaled) */
+#define VLHIP_MODE_FP32_MFMA 4 /* plain FP32 operands on the FP32 matrix cores (float data, D <= 128) */
 
 /* ---- runtime ------------------------------------------------------------ */
 VLHIP_API int    vlhip_available (void) ;                 /* 1 when a GPU can be used (never aborts) */

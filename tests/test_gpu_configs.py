@@ -277,6 +277,31 @@ def test_config5_fp32_vs_reference(cfg5):
         assert max(er) < 1e-4
 
 
+def test_config5_fp32_matrix_cores_vs_reference(cfg5):
+    """The FP32 matrix-core encoder (the bench's FP32 comparator) at the
+    metric's shape: every image within 1e-4 of the FP64 oracle and of the
+    reference's float path."""
+    _, xe, off, m, c, p, ref = cfg5
+    with va.fisher_precision(va.MODE_FP32_MFMA):
+        e, t = va.vl_fisher_batch(xe, off, m, c, p, 3)
+    errs = [rel(e[i], ref[i][0]) for i in range(8)]
+    print(f"config 5 FP32 matrix cores vs FP64 oracle: per image " + ", ".join(f"{v:.2e}" for v in errs))
+    assert max(errs) < 1e-4
+    for i in range(8):
+        assert abs(int(t[i]) - int(ref[i][1])) <= max(2, int(ref[i][1]) // 10000)
+    R = reference()
+    if R is not None:
+        R.set_num_threads(0)
+        rf = [R.fisher(xe[int(off[i]):int(off[i + 1])], m, c, p, 3)[0] for i in range(8)]
+        er = [rel(e[i], rf[i]) for i in range(8)]
+        eref = [rel(rf[i], ref[i][0]) for i in range(8)]
+        print(f"config 5 FP32 matrix cores vs reference float: max {max(er):.2e} "
+              f"(reference vs FP64: max {max(eref):.2e})")
+        # the reference's float L2 norm (a sequential float sum) is itself off
+        # in scale by up to ~1e-4 at this size
+        assert all(a < 1e-4 + b for a, b in zip(er, eref))
+
+
 @pytest.mark.parametrize("mode", [va.MODE_F16X3, va.MODE_BF16X3], ids=["f16x3", "bf16x3"])
 def test_config5_split16_vs_fp64(cfg5, mode):
     _, xe, off, m, c, p, ref = cfg5

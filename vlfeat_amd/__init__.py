@@ -17,7 +17,7 @@ import numpy as np
 from .capi import (  # noqa: F401
     CLib, VL_TYPE_DOUBLE, VL_TYPE_FLOAT, FISHER_FAST, FISHER_IMPROVED, FISHER_NORMALIZED,
     FISHER_SQUARE_ROOT, VLAD_NORMALIZE_COMPONENTS, VLAD_NORMALIZE_MASS, VLAD_SQUARE_ROOT,
-    VLAD_UNNORMALIZED, MODE_BF16X3, MODE_F16X3, MODE_FP32, MODE_GATED, VlKMeansANN, VlKMeansElkan,
+    VLAD_UNNORMALIZED, MODE_BF16X3, MODE_F16X3, MODE_FP32, MODE_FP32_MFMA, MODE_GATED, VlKMeansANN, VlKMeansElkan,
     VlKMeansLloyd, VlKMeansPlusPlus, VlKMeansRandomSelection, dtype_code, ptr,
 )
 
@@ -55,7 +55,7 @@ def vl_fisher(data, means, covariances, priors, flags=0):
 
 class fisher_precision:
     """Context manager selecting the float Fisher arithmetic (MODE_FP32,
-    MODE_BF16X3, MODE_F16X3 or MODE_GATED) for the calling thread."""
+    MODE_FP32_MFMA, MODE_BF16X3, MODE_F16X3 or MODE_GATED) for the calling thread."""
 
     def __init__(self, mode):
         self.mode = mode

@@ -32,7 +32,7 @@ VLAD_SQUARE_ROOT = 2
 VLAD_UNNORMALIZED = 4
 VLAD_NORMALIZE_MASS = 8
 
-MODE_FP32, MODE_BF16X3, MODE_GATED, MODE_F16X3 = 0, 1, 2, 3
+MODE_FP32, MODE_BF16X3, MODE_GATED, MODE_F16X3, MODE_FP32_MFMA = 0, 1, 2, 3, 4
 
 _p = C.c_void_p
 _sz = C.c_uint64

@@ -531,6 +531,9 @@ extern "C" void vlhip_fisher_encode_batch(int dtype, void* enc, void const* mean
   } else if (mode == VLHIP_MODE_BF16X3 || mode == VLHIP_MODE_F16X3) {
     fisher_split16((float*)enc, (const float*)means, (const float*)covs, (const float*)priors, D, K,
                    (const float*)data, offsets, numImages, flags, numTerms, mode == VLHIP_MODE_F16X3, nullptr);
+  } else if (mode == VLHIP_MODE_FP32_MFMA) {
+    fisher_f32mfma((float*)enc, (const float*)means, (const float*)covs, (const float*)priors, D, K,
+                   (const float*)data, offsets, numImages, flags, numTerms);
   } else if (mode == VLHIP_MODE_GATED) {
     VLHIP_REQUIRE(ctx().gate != nullptr, "VLHIP_MODE_GATED needs a gate (vlhip_fisher_set_gate)");
     const GateUse gu{ctx().gate, ctx().gate_m, ctx().gate_need};

@@ -2279,6 +2279,516 @@ void fisher_split16(float* enc, const float* means, const float* covs, const flo
   else mfma_dispatch<__bf16>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, gu);
 }
 
+// ---------------------------------------------------------------------------
+// FP32 matrix-core encoder (VLHIP_MODE_FP32_MFMA).  The two contractions of
+// this file with plain FP32 operands on v_mfma_f32_16x16x4_f32 (exact FP32
+// products, FP32 accumulation; no split, no scaling: the tables are those of
+// the compensated encoder built with float in place of the 16-bit type, whose
+// low parts are zero).  Two passes as in the compensated encoder, but without
+// operand records: the first pass writes each group's row statistics (-max
+// logit, 1 / sum, argmax: 384 bytes per group of 32 descriptors) and the
+// second rebuilds its operands from the descriptors, staged in LDS as
+// x' = x - g.  Nothing of size N x K is written.
+//
+// 16x16x4 operands: lane l supplies A[l & 15][l >> 4] and B[l >> 4][l & 15].
+// In the logit contraction, step j of a row tile takes feature 8NS q + j of
+// [x'^2 | x'] (2Dp = 32NS features) from the lanes of quarter q = l >> 4, so
+// that a lane's weights W[comp][8NS q, 8NS q + 8NS) are one contiguous run of
+// the table and its descriptor values are x'[8NS (q & 1), +8NS), squared for
+// q < 2.  In the moment contraction, element r of the logit accumulator of row
+// tile t (rows 16t + 4q + r, the lane's component) is the A operand as it
+// stands, and B is x' or x'^2 of the same rows: one product per 4 rows.
+
+constexpr int F32_STATS = 3 * GR;  // floats of row statistics per group
+template <int NS> struct F32LseWaves { static constexpr int value = NS <= 5 ? 8 : 4; };
+
+__device__ __forceinline__ f32x4 mfma4(float a, float b, const f32x4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Logits of 32 rows x 64 staged components (LDS rows of 32NS + 4 floats):
+// 2 row tiles x 4 component tiles x 8NS steps.
+template <int NS>
+__device__ __forceinline__ void f32_lse_products(f32x4 (&acc)[4][2], const float (&a)[2][8 * NS], const float* w,
+                                                 int lr, int lg) {
+  constexpr int WS = 32 * NS + 4;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) acc[c][0] = acc[c][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int j4 = 0; j4 < 2 * NS; ++j4) {
+    f32x4 bw[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) bw[c] = *reinterpret_cast<const f32x4*>(w + (16 * c + lr) * WS + 8 * NS * lg + 4 * j4);
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[c][t] = mfma4(a[t][4 * j4 + e], bw[c][e], acc[c][t]);
+  }
+}
+
+// First pass: wave = one group of 32 rows; the weights of 64 components at a
+// time are double-buffered in LDS (the next block fetched into registers while
+// the current one is multiplied); a lane-local online log-sum-exp per row,
+// merged over the 16 lanes that share the rows at the end.
+template <int NS, bool ARG>
+__global__ __launch_bounds__(F32LseWaves<NS>::value * 64, 1) void f32_lse_kernel(
+    const float* __restrict__ x, int D, int Kp, const float* __restrict__ W, const float* __restrict__ bias,
+    const float* __restrict__ g, float* __restrict__ stats, long long numGroups, const int4* __restrict__ gtab) {
+  constexpr int Dp = NS * 16;
+  constexpr int F = 2 * Dp;
+  constexpr int WS = F + 4;
+  constexpr int LG = F32LseWaves<NS>::value;
+  constexpr int NT = LG * 64;
+  constexpr int WORDS = MB_COMPS * F / 4;  // 16-byte words of one weight block
+  constexpr int STG = WORDS / NT;
+  static_assert(STG * NT == WORDS, "staging must divide evenly");
+  __shared__ __align__(16) float wsh[2][MB_COMPS][WS];
+  __shared__ float bsh[4][MB_COMPS];
+  static_assert(LG * GR * Dp <= 2 * MB_COMPS * WS, "row staging must fit the weight buffers");
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int lr = lane & 15, lg = lane >> 4;
+  const long long grp = (long long)blockIdx.x * LG + w;
+  const bool live = grp < numGroups;  // uniform per wave; dead waves still take part in barriers
+  long long row0 = 0, rowEnd = 0;
+  if (live) {
+    const int4 gr = gtab[grp];
+    row0 = gr.x;
+    rowEnd = gr.y;
+  }
+  int srcOff[STG], dstOff[STG];
+#pragma unroll
+  for (int q = 0; q < STG; ++q) {
+    const int i = tid + NT * q;
+    const int c = i / (F / 4), q4 = i - c * (F / 4);
+    srcOff[q] = c * F + 4 * q4;
+    dstOff[q] = c * WS + 4 * q4;
+  }
+  f32x4 wpre[STG];
+#define F32_FETCH(B)                                                                                  \
+  {                                                                                                   \
+    _Pragma("unroll") for (int q = 0; q < STG; ++q) wpre[q] =                                         \
+        *reinterpret_cast<const f32x4*>(W + (size_t)(B) * MB_COMPS * F + srcOff[q]);                  \
+    if (tid < MB_COMPS) bpre = bias[(B) * MB_COMPS + tid];                                            \
+  }
+#define F32_STAGE(B)                                                                                  \
+  {                                                                                                   \
+    _Pragma("unroll") for (int q = 0; q < STG; ++q)                                                   \
+        *reinterpret_cast<f32x4*>(&wsh[(B) & 1][0][0] + dstOff[q]) = wpre[q];                         \
+    if (tid < MB_COMPS) bsh[(B) & 3][tid] = bpre;                                                     \
+  }
+  float bpre = 0.f;
+  F32_FETCH(0)
+
+  // the wave's 32 centred rows x' = x - g (zero past D and past the image) in
+  // LDS, row stride Dp, inside the weight buffers (free until the sweep)
+  float* xsh = &wsh[0][0][0] + w * GR * Dp;
+  {
+    constexpr int XW = GR * (Dp / 4);
+    constexpr int XN = (XW + 63) / 64;
+    const bool vecx = (D & 3) == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0;
+    f32x4 xv[XN], gv[XN];
+#pragma unroll
+    for (int u = 0; u < XN; ++u) {
+      const int i = lane + 64 * u;
+      const int c4 = (i - (i / (Dp / 4)) * (Dp / 4)) * 4;
+      gv[u] = i < XW ? *reinterpret_cast<const f32x4*>(g + c4) : f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int u = 0; u < XN; ++u) {
+      const int i = lane + 64 * u;
+      const int r = i / (Dp / 4), c4 = (i - r * (Dp / 4)) * 4;
+      const long long row = row0 + r;
+      f32x4 v = f32x4{0.f, 0.f, 0.f, 0.f};
+      if (i < XW && row < rowEnd) {
+        if (vecx) {
+          if (c4 < D) v = *reinterpret_cast<const f32x4*>(x + row * D + c4);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) v[e] = c4 + e < D ? x[row * D + c4 + e] : 0.f;
+        }
+      }
+      xv[u] = v;
+    }
+#pragma unroll
+    for (int u = 0; u < XN; ++u) {
+      const int i = lane + 64 * u;
+      if (i >= XW) continue;
+      const int r = i / (Dp / 4), c4 = (i - r * (Dp / 4)) * 4;
+      f32x4 v = xv[u];
+      if (row0 + r < rowEnd) v = v - gv[u];
+      *reinterpret_cast<f32x4*>(xsh + r * Dp + c4) = v;
+    }
+  }
+  __syncthreads();
+  // A operands of the 2 row tiles, resident for the whole sweep
+  float a[2][8 * NS];
+  {
+    const bool sq = lg < 2;
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int j4 = 0; j4 < 2 * NS; ++j4) {
+        const f32x4 v = *reinterpret_cast<const f32x4*>(xsh + (16 * t + lr) * Dp + 8 * NS * (lg & 1) + 4 * j4);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) a[t][4 * j4 + e] = sq ? v[e] * v[e] : v[e];
+      }
+  }
+  __syncthreads();  // the staging area becomes the weight buffers
+
+  LseState st;
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) { st.m[t][r] = -INFINITY; st.s[t][r] = 0.f; st.bv[t][r] = -INFINITY; st.bi[t][r] = 0; }
+  if (!__syncthreads_or(live)) return;
+  const int nb = Kp / MB_COMPS;
+  F32_STAGE(0)
+  if (nb > 1) F32_FETCH(1)
+  __syncthreads();
+  for (int b = 0; b < nb; ++b) {
+    f32x4 acc[4][2];
+    if (live) f32_lse_products<NS>(acc, a, &wsh[b & 1][0][0], lr, lg);
+    // the other buffer held block b - 1, read before the last barrier
+    if (b + 1 < nb) {
+      F32_STAGE(b + 1)
+      if (b + 2 < nb) F32_FETCH(b + 2)
+    }
+    if (live) lse_update<ARG>(acc, &bsh[b & 3][0], b * MB_COMPS, lr, st);
+    __syncthreads();
+  }
+#undef F32_FETCH
+#undef F32_STAGE
+
+  float* Sf = stats + (size_t)grp * F32_STATS;
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float m = row16_max(st.m[t][r]);
+      const float ms = (m == -INFINITY) ? 0.f : m;
+      const float s = row16_sum(st.s[t][r] * __builtin_amdgcn_exp2f(st.m[t][r] - ms));
+      float bv = st.bv[t][r];
+      int bi = st.bi[t][r];
+      if (ARG) row16_argmax(bv, bi);
+      const int row = 16 * t + 4 * lg + r;
+      if (live && lr == 0) {
+        const bool pad = row0 + row >= rowEnd;
+        Sf[row] = pad ? 0.f : -m;
+        Sf[GR + row] = pad ? 0.f : 1.0f / s;
+        Sf[2 * GR + row] = __int_as_float(pad ? -1 : bi);
+      }
+    }
+}
+
+// Second pass: workgroup = (image, WAVES x 16CT components); wave w owns CT
+// component tiles with their weights in registers and walks the image's
+// groups: the group's rows x' and row statistics are staged in LDS (double
+// buffered, the next group prefetched into registers), then
+//   logits   2 row tiles x CT tiles x 8NS steps, accumulators starting at
+//            bias - max;
+//   q        2^(logit - max) / sum, FAST one-hot, < 1e-6 dropped
+//            (fisher.c:436-441); padding rows have 1 / sum = 0;
+//   moments  S0 (q against ones), S1 += q x', S2 += q x'^2, one product per
+//            4 rows and 16 features.
+// The finalisation is that of the compensated encoder (scales 1).
+template <int NS, int CT, int WAVES>
+__global__ __launch_bounds__(WAVES * 64, 1) void f32_fisher_kernel(
+    const float* __restrict__ x, int D, const float* __restrict__ g, const float* __restrict__ stats,
+    const uint64_t* __restrict__ offsets, const long long* __restrict__ gstart, int imgBegin, int numImages, int K,
+    const float* __restrict__ W, const float* __restrict__ bias, const MfmaScales* __restrict__ scp,
+    const float* __restrict__ mup, const float* __restrict__ isg, const float* __restrict__ sinv,
+    const float* __restrict__ priors, int flags, float* __restrict__ enc, double* __restrict__ normpart,
+    unsigned long long* __restrict__ numTerms, int numKB) {
+  constexpr int Dp = NS * 16;
+  constexpr int F = 2 * Dp;
+  constexpr int NT = WAVES * 64;
+  constexpr int BC = WAVES * 16 * CT;
+  constexpr int XS = Dp + 4;              // LDS row stride of a staged group
+  constexpr int XWORDS = GR * Dp / 4;     // 16-byte words of a group's rows
+  constexpr int NV = XWORDS + F32_STATS / 4;
+  constexpr int PER = (NV + NT - 1) / NT;
+  constexpr int BUF = GR * XS + F32_STATS;
+  __shared__ __align__(16) float xs[2][BUF];
+  __shared__ double red[WAVES];
+
+  int li, kb;
+  fisher_mfma_coords(numKB, numImages, li, kb);
+  if (li >= numImages) return;
+  const int img = imgBegin + li;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int lr = lane & 15, lg = lane >> 4;
+  const int kw = kb * BC + 16 * CT * w;
+  const long long r0 = (long long)offsets[img], r1 = (long long)offsets[img + 1];
+  const long long nImg = r1 - r0;
+  const long long gBase = gstart[imgBegin];
+  const long long gB = gstart[img] - gBase, gE = gstart[img + 1] - gBase;
+  const bool fast = (flags & 4) != 0;
+  const MfmaScales sc = *scp;
+  const bool vecx = (D & 3) == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0;
+
+  float wr[CT][8 * NS];
+  float bc[CT];
+#pragma unroll
+  for (int c = 0; c < CT; ++c) {
+    const int comp = kw + 16 * c + lr;
+    bc[c] = bias[comp];
+#pragma unroll
+    for (int j4 = 0; j4 < 2 * NS; ++j4) {
+      const f32x4 v = *reinterpret_cast<const f32x4*>(W + (size_t)comp * F + 8 * NS * lg + 4 * j4);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) wr[c][4 * j4 + e] = v[e];
+    }
+  }
+  // staging words of this thread: row words (minus g on real rows, g read
+  // from LDS) and the group's statistics
+  __shared__ __align__(16) float gsh[Dp];
+  for (int d = tid; d < Dp; d += NT) gsh[d] = g[d];
+  auto load_group = [&](long long gi, f32x4 (&pre)[PER]) {
+    const long long rg = r0 + (long long)GR * (gi - gB);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int i = tid + NT * q;
+      f32x4 v = f32x4{0.f, 0.f, 0.f, 0.f};
+      if (i < XWORDS) {
+        const int r = i / (Dp / 4), c4 = (i - r * (Dp / 4)) * 4;
+        const long long row = rg + r;
+        if (row < r1) {
+          if (vecx) {
+            if (c4 < D) v = *reinterpret_cast<const f32x4*>(x + row * D + c4);
+          } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[e] = c4 + e < D ? x[row * D + c4 + e] : 0.f;
+          }
+        }
+      } else if (i < NV) {
+        v = *reinterpret_cast<const f32x4*>(stats + (size_t)gi * F32_STATS + 4 * (i - XWORDS));
+      }
+      pre[q] = v;
+    }
+  };
+  auto store_group = [&](const f32x4 (&pre)[PER], float* dst, long long gi) {
+    const long long rg = r0 + (long long)GR * (gi - gB);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int i = tid + NT * q;
+      if (i < XWORDS) {
+        const int r = i / (Dp / 4), c4 = (i - r * (Dp / 4)) * 4;
+        *reinterpret_cast<f32x4*>(dst + r * XS + c4) =
+            rg + r < r1 ? pre[q] - *reinterpret_cast<const f32x4*>(gsh + c4) : pre[q];
+      } else if (i < NV) {
+        *reinterpret_cast<f32x4*>(dst + GR * XS + 4 * (i - XWORDS)) = pre[q];
+      }
+    }
+  };
+
+  f32x4 macc[CT][2][NS];
+  f32x4 macc0[CT];
+#pragma unroll
+  for (int c = 0; c < CT; ++c) {
+    macc0[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int ft = 0; ft < NS; ++ft) macc[c][s][ft] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+  unsigned terms = 0;
+  const float qmin = __uint_as_float(0x358637beu) * sc.sq;  // (double)q < 1e-6 for a float q
+  const bool sqLane = lg < 2;
+
+  f32x4 pre[PER];
+  __syncthreads();  // gsh
+  if (gB < gE) {
+    load_group(gB, pre);
+    store_group(pre, xs[0], gB);
+    if (gB + 1 < gE) load_group(gB + 1, pre);
+  }
+  __syncthreads();
+  for (long long gi = gB; gi < gE; ++gi) {
+    const int buf = (int)((gi - gB) & 1);
+    if (gi + 1 < gE) {
+      store_group(pre, xs[buf ^ 1], gi + 1);  // that buffer (group gi - 1) was read before the last barrier
+      if (gi + 2 < gE) load_group(gi + 2, pre);
+    }
+    const float* cur = xs[buf];
+    const float* Sf = cur + GR * XS;
+    f32x4 nm4[2], rs4[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      nm4[t] = *reinterpret_cast<const f32x4*>(&Sf[16 * t + 4 * lg]);
+      rs4[t] = *reinterpret_cast<const f32x4*>(&Sf[GR + 16 * t + 4 * lg]);
+    }
+    f32x4 acc[2][CT];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int c = 0; c < CT; ++c)
+        acc[t][c] = f32x4{bc[c] + nm4[t][0], bc[c] + nm4[t][1], bc[c] + nm4[t][2], bc[c] + nm4[t][3]};
+#pragma unroll
+    for (int j4 = 0; j4 < 2 * NS; ++j4) {
+      f32x4 av[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const f32x4 v = *reinterpret_cast<const f32x4*>(cur + (16 * t + lr) * XS + 8 * NS * (lg & 1) + 4 * j4);
+        av[t] = sqLane ? v * v : v;
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+          for (int c = 0; c < CT; ++c) acc[t][c] = mfma4(av[t][e], wr[c][4 * j4 + e], acc[t][c]);
+    }
+    float qv[CT][2][4];
+    if (fast) {
+      f32x4 ar4[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) ar4[t] = *reinterpret_cast<const f32x4*>(&Sf[2 * GR + 16 * t + 4 * lg]);
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < CT; ++c) {
+            const bool hit = kw + 16 * c + lr == __float_as_int(ar4[t][r]) && bc[c] != -INFINITY;
+            terms += (unsigned)__popcll(__ballot(hit));
+            qv[c][t][r] = hit ? sc.sq : 0.f;
+          }
+    } else {
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < CT; ++c) {
+            const float v = __builtin_amdgcn_exp2f(acc[t][c][r]) * rs4[t][r];
+            const bool keep = v >= qmin;  // false for NaN (padding rows: inf * 0)
+            terms += (unsigned)__popcll(__ballot(keep));
+            qv[c][t][r] = keep ? v : 0.f;
+          }
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int c = 0; c < CT; ++c) macc0[c] = mfma4(qv[c][t][r], 1.0f, macc0[c]);
+#pragma unroll
+        for (int ft = 0; ft < NS; ++ft) {
+          const float b1 = cur[(16 * t + 4 * lg + r) * XS + 16 * ft + lr];
+          const float b2 = b1 * b1;
+#pragma unroll
+          for (int c = 0; c < CT; ++c) {
+            macc[c][0][ft] = mfma4(qv[c][t][r], b1, macc[c][0][ft]);
+            macc[c][1][ft] = mfma4(qv[c][t][r], b2, macc[c][1][ft]);
+          }
+        }
+      }
+    __syncthreads();
+  }
+  fisher_slice_finalize<float, NS, CT, WAVES>(macc0, macc, sc, kw, lr, lg, tid, lane, img, kb, numKB, nImg, D, K, mup,
+                                              isg, sinv, priors, flags, enc, normpart, numTerms, terms, red);
+}
+
+// Bytes of row statistics materialised at once by the FP32 matrix-core encoder.
+constexpr size_t kF32StatsBudget = size_t(1) << 30;
+
+template <int NS, int CT, int WAVES>
+static void run_f32mfma(float* enc, const float* means, const float* covs, const float* priors, size_t D, size_t K,
+                        const float* x, const uint64_t* offsets, size_t numImages, int flags,
+                        unsigned long long* numTerms, const std::vector<uint64_t>& off) {
+  constexpr int Dp = NS * 16;
+  constexpr int BC = WAVES * 16 * CT;
+  static_assert(BC % MB_COMPS == 0, "component blocks of both passes must agree");
+  constexpr int LG = F32LseWaves<NS>::value;
+  const int Kp = (int)((K + BC - 1) / BC) * BC;
+  unsigned char* base = (unsigned char*)scratch(SLOT_PREP, mfma_tables_bytes<float>(Dp, Kp));
+  const MfmaTables<float> t = mfma_tables_carve<float>(base, Dp, Kp);
+  mfma_tables_launch<float>(t, means, covs, priors, (int)D, Dp, (int)K, Kp);
+  const int numKB = Kp / BC;
+  double* normpart = (double*)scratch(SLOT_PARTIAL, numImages * (size_t)numKB * sizeof(double));
+  if (numTerms) zero_words(numTerms, 2 * numImages);
+  std::vector<long long> gs(numImages + 1);
+  gs[0] = 0;
+  for (size_t i = 0; i < numImages; ++i) gs[i + 1] = gs[i] + (long long)((off[i + 1] - off[i] + GR - 1) / GR);
+  long long* gstart = (long long*)scratch(SLOT_GROUPS, (numImages + 1) * sizeof(long long));
+  upload(gstart, gs.data(), (numImages + 1) * sizeof(long long));
+  sync();  // gs is a local buffer
+  constexpr size_t kStatBytes = F32_STATS * sizeof(float);
+  size_t b0 = 0;
+  while (b0 < numImages) {
+    size_t b1 = b0 + 1;
+    while (b1 < numImages && (size_t)(gs[b1 + 1] - gs[b0]) * kStatBytes <= kF32StatsBudget &&
+           off[b1 + 1] - off[b0] < (uint64_t(1) << 31))
+      ++b1;
+    const uint64_t rowBase = off[b0], rows = off[b1] - off[b0];
+    VLHIP_REQUIRE(rows < (uint64_t(1) << 31), "an image of the FP32 matrix-core encoder exceeds 2^31 descriptors");
+    const size_t nG = (size_t)(gs[b1] - gs[b0]);
+    const int n = (int)(b1 - b0);
+    float* stats = (float*)scratch(SLOT_FISHER_WORK, (nG ? nG : 1) * kStatBytes);
+    if (nG > 0) {
+      int4* gtab = (int4*)scratch(SLOT_GROUPTAB, nG * sizeof(int4));
+      VLHIP_LAUNCH("mfma_group_table", mfma_group_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                   offsets, gstart, (int)b0, n, gtab);
+      const unsigned lb = (unsigned)((nG + LG - 1) / LG);
+      if (flags & 4)
+        VLHIP_LAUNCH("f32_lse", (f32_lse_kernel<NS, true>), dim3(lb), dim3(LG * 64), 0, x + rowBase * D, (int)D, Kp,
+                     (const float*)t.Whi, t.bias, t.g, stats, (long long)nG, gtab);
+      else
+        VLHIP_LAUNCH("f32_lse", (f32_lse_kernel<NS, false>), dim3(lb), dim3(LG * 64), 0, x + rowBase * D, (int)D, Kp,
+                     (const float*)t.Whi, t.bias, t.g, stats, (long long)nG, gtab);
+    }
+    const int padImages = n >= 8 ? ((n + 7) / 8) * 8 : n;
+    VLHIP_LAUNCH("f32_fisher", (f32_fisher_kernel<NS, CT, WAVES>), dim3((unsigned)((size_t)padImages * numKB)),
+                 dim3(WAVES * 64), 0, x, (int)D, t.g, stats, offsets, gstart, (int)b0, n, (int)K,
+                 (const float*)t.Whi, t.bias, t.scp, t.mup, t.isg, t.sinv, priors, flags, enc, normpart, numTerms,
+                 numKB);
+    b0 = b1;
+  }
+  if (flags & 2) launch_fisher_normalize(enc, normpart, numKB, numImages, (size_t)2 * K * D);
+}
+
+template <int NS>
+static void run_f32mfma_k(float* enc, const float* means, const float* covs, const float* priors, size_t D, size_t K,
+                          const float* x, const uint64_t* offsets, size_t numImages, int flags,
+                          unsigned long long* numTerms, const std::vector<uint64_t>& off) {
+  // wider component blocks when there are components enough (the weights of
+  // 2 tiles fit a wave's registers up to D = 80)
+  if constexpr (NS <= 5) {
+    if (K > 128) {
+      run_f32mfma<NS, 2, 8>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off);
+      return;
+    }
+    run_f32mfma<NS, 2, 4>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off);
+  } else {
+    run_f32mfma<NS, 1, 8>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off);
+  }
+}
+
+void fisher_f32mfma(float* enc, const float* means, const float* covs, const float* priors, size_t D, size_t K,
+                    const float* x, const uint64_t* offsets, size_t numImages, int flags,
+                    unsigned long long* numTerms) {
+  VLHIP_REQUIRE(K >= 1 && D >= 1, "numClusters and dimension must be positive");
+  if (numImages == 0) return;
+  VLHIP_REQUIRE(K <= (size_t)1 << 20, "the FP32 matrix-core encoder supports at most 2^20 components");
+  std::vector<uint64_t> off(numImages + 1);
+  download(off.data(), offsets, (numImages + 1) * sizeof(uint64_t));
+  switch ((D + 15) / 16) {
+    case 1: run_f32mfma_k<1>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off); break;
+    case 2: run_f32mfma_k<2>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off); break;
+    case 3: run_f32mfma_k<3>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off); break;
+    case 4: run_f32mfma_k<4>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off); break;
+    case 5: run_f32mfma_k<5>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off); break;
+    case 6: run_f32mfma_k<6>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off); break;
+    case 7: run_f32mfma_k<7>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off); break;
+    case 8: run_f32mfma_k<8>(enc, means, covs, priors, D, K, x, offsets, numImages, flags, numTerms, off); break;
+    default: fatal("vlhip_fisher_encode_batch", __FILE__, __LINE__,
+                   "the FP32 matrix-core Fisher encoder supports dimension <= 128");
+  }
+}
+
 
 // ---------------------------------------------------------------------------
 // EM on the matrix cores (vl_gmm_em with a compensated precision).  One EM

@@ -58,6 +58,12 @@ void fisher_split16(float* enc, const float* means, const float* covs, const flo
                     unsigned long long* numTerms, bool fp16,
                     const GateUse* gate);
 
+// FP32 matrix-core Fisher encoder (fisher_mfma.hip): plain FP32 operands on
+// v_mfma_f32_16x16x4_f32, D <= 128.
+void fisher_f32mfma(float* enc, const float* means, const float* covs, const float* priors, size_t D, size_t K,
+                    const float* x, const uint64_t* offsets, size_t numImages, int flags,
+                    unsigned long long* numTerms);
+
 // VLAD of a batch of images (vlad_kernels.hip); exactly one of assign / hard.
 template <typename T>
 void vlad_t(T* enc, const T* means, size_t D, size_t K, const T* x, const uint64_t* offsets, size_t numImages,

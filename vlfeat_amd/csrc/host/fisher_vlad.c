@@ -24,7 +24,8 @@ VL_EXPORT void
 vl_fisher_set_precision (int precision)
 {
   if (precision != VL_FISHER_PRECISION_FP32 && precision != VL_FISHER_PRECISION_BF16X3 &&
-      precision != VL_FISHER_PRECISION_F16X3 && precision != VL_FISHER_PRECISION_GATED) {
+      precision != VL_FISHER_PRECISION_F16X3 && precision != VL_FISHER_PRECISION_GATED &&
+      precision != VL_FISHER_PRECISION_FP32_MFMA) {
     vl_host_fatal("vl_fisher_set_precision: unknown precision %d\n", precision) ;
   }
   g_precision = precision ;
