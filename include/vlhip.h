@@ -207,6 +207,50 @@ VLHIP_API void vlhip_project (void * out, void const * data, size_t numData, siz
 VLHIP_API void vlhip_homkermap_evaluate (int dtype, void * out, void const * in, size_t n, double const * table,
                                          int order, int numSubdivisions, int minExponent, int maxExponent) ;
 
+/* ---- SVM solvers (vl/svm.c) ----------------------------------------------- */
+
+/* One run of SDCA (svm.c:1931-2016) or SGD (svm.c:2018-2142) iterations
+ * [begin, end) inside one epoch, on device arrays: point i of the epoch's
+ * iteration t is permutation[t % numData]; model, alpha (SDCA), scores and
+ * previousScores (SGD) are updated in place; state holds (bias, factor,
+ * biasFactor) on entry and receives them on return (host array; SGD's model
+ * is factor * model and its bias biasFactor * bias, as in the reference).
+ * norm2[i] (SDCA) = (<x_i, x_i> + biasMultiplier^2) / (lambda numData).
+ * workspace: vlhip_svm_workspace_bytes(dimension) bytes, cleared by
+ * vlhip_svm_reset_workspace before the first run of a training.  The runs of
+ * one workspace must not overlap.  One persistent cooperative launch of
+ * ceil(dimension / 4096) workgroups; dimension <= 4,194,304. */
+typedef struct VlhipSvmRun {
+  int dtype ;                      /* 1 float, 2 double points */
+  int solver ;                     /* 1 SGD, 2 SDCA (VlSvmSolverType) */
+  int loss ;                       /* VlSvmLossType */
+  void const * data ;              /* numData x dimension points */
+  size_t dimension, numData ;
+  double const * labels ;
+  double const * weights ;         /* NULL: every weight 1 */
+  double * model ;
+  double * alpha ;
+  double const * norm2 ;
+  double * scores, * previousScores ;
+  int64_t const * permutation ;
+  uint64_t begin, end ;
+  double lambda, biasMultiplier, biasLearningRate ;
+  int64_t sgdT0 ;                  /* max(2, ceil(1 / lambda)), svm.c:2026 */
+  double state [3] ;
+  void * workspace ;
+} VlhipSvmRun ;
+VLHIP_API size_t vlhip_svm_workspace_bytes (size_t dimension) ;
+VLHIP_API void vlhip_svm_reset_workspace (void * workspace, size_t dimension) ;
+VLHIP_API void vlhip_svm_run (VlhipSvmRun * run) ;
+/* scores[k] = <x_k, model> + biasTerm for every point (svm.c:1885-1886) */
+VLHIP_API void vlhip_svm_scores (int dtype, double * scores, void const * data, double const * model,
+                                 size_t dimension, size_t numData, double biasTerm) ;
+/* out[k] = <x_k, x_k> in double (svm.c:1943-1950) */
+VLHIP_API void vlhip_svm_square_norms (int dtype, double * out, void const * data, size_t dimension,
+                                       size_t numData) ;
+/* v *= factor elementwise (svm.c:2115-2116) */
+VLHIP_API void vlhip_svm_scale (double * v, size_t n, double factor) ;
+
 /* ---- VLAD (vl/vlad.c) ---------------------------------------------------- */
 
 /* VLAD of numImages images, vlad.c:163-242 per image.  Exactly one of

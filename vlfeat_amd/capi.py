@@ -112,7 +112,50 @@ _SIGS = {
     "vl_homogeneouskernelmap_get_dimension": (_sz, [_p]),
     "vl_homogeneouskernelmap_get_kernel_type": (C.c_int, [_p]),
     "vl_homogeneouskernelmap_get_window_type": (C.c_int, [_p]),
+    # svm.h / svmdataset.h
+    "vl_svmdataset_new": (_p, [C.c_uint32, _p, _sz, _sz]),
+    "vl_svmdataset_delete": (None, [_p]),
+    "vl_svmdataset_set_homogeneous_kernel_map": (None, [_p, _p]),
+    "vl_svmdataset_get_dimension": (_sz, [_p]),
+    "vl_svmdataset_get_inner_product_function": (_p, [_p]),
+    "vl_svmdataset_get_accumulate_function": (_p, [_p]),
+    "vl_svm_new": (_p, [C.c_int, _p, _sz, _sz, _p, C.c_double]),
+    "vl_svm_new_with_dataset": (_p, [C.c_int, _p, _p, C.c_double]),
+    "vl_svm_new_with_abstract_data": (_p, [C.c_int, _p, _sz, _sz, _p, C.c_double]),
+    "vl_svm_delete": (None, [_p]),
+    "vl_svm_train": (None, [_p]),
+    "vl_svm_set_loss": (None, [_p, C.c_int]),
+    "vl_svm_set_epsilon": (None, [_p, C.c_double]),
+    "vl_svm_set_bias_multiplier": (None, [_p, C.c_double]),
+    "vl_svm_set_bias_learning_rate": (None, [_p, C.c_double]),
+    "vl_svm_set_max_num_iterations": (None, [_p, _sz]),
+    "vl_svm_set_diagnostic_frequency": (None, [_p, _sz]),
+    "vl_svm_set_weights": (None, [_p, _p]),
+    "vl_svm_set_model": (None, [_p, _p]),
+    "vl_svm_set_bias": (None, [_p, C.c_double]),
+    "vl_svm_set_data_functions": (None, [_p, _p, _p]),
+    "vl_svm_set_diagnostic_function": (None, [_p, _p, _p]),
+    "vl_svm_get_model": (_p, [_p]),
+    "vl_svm_get_bias": (C.c_double, [_p]),
+    "vl_svm_get_scores": (_p, [_p]),
+    "vl_svm_get_statistics": (_p, [_p]),
+    "vl_svm_get_max_num_iterations": (_sz, [_p]),
+    "vl_svm_get_dimension": (_sz, [_p]),
+    "vl_svm_hinge_loss": (C.c_double, [C.c_double, C.c_double]),
+    "vl_svm_logistic_dca_update": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_double]),
 }
+
+
+class VlSvmStatistics(C.Structure):
+    """vl/svm.h:76-88"""
+    _fields_ = [("status", C.c_int), ("iteration", _sz), ("epoch", _sz), ("objective", C.c_double),
+                ("regularizer", C.c_double), ("loss", C.c_double), ("dualObjective", C.c_double),
+                ("dualLoss", C.c_double), ("dualityGap", C.c_double), ("scoresVariation", C.c_double),
+                ("elapsedTime", C.c_double)]
+
+
+SVM_STAT_FIELDS = ("status", "iteration", "epoch", "objective", "regularizer", "loss", "dualObjective",
+                   "dualLoss", "dualityGap", "scoresVariation")
 
 # Entry points present only in this library (batched forms and the device ABI).
 _EXT_SIGS = {
@@ -212,6 +255,81 @@ class CLib:
 
     def set_num_threads(self, n: int) -> None:
         self.vl_set_num_threads(n)
+
+    # ---- SVM -----------------------------------------------------------
+    def svm_train(self, data, labels, lam, *, solver=2, loss=0, weights=None, hom=None, bias_multiplier=None,
+                  epsilon=None, max_iter=None, diag_freq=None, bias_lr=None, abstract=False, seed=None,
+                  model=None, bias=None, diagnostics=False):
+        """Train an SVM on the rows of data (float32 or float64) through the C
+        API; returns a dict with model, bias, statistics, scores and, with
+        diagnostics, the statistics seen at every diagnostic call.  hom:
+        (kernel, gamma, order, period, window) of a homogeneous kernel map
+        applied by the dataset.  abstract: train through
+        vl_svm_new_with_abstract_data with the dataset's own data functions
+        (the solver then calls them as caller-set functions)."""
+        x = np.ascontiguousarray(data)
+        n, d = x.shape
+        y = np.ascontiguousarray(labels, np.float64)
+        w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+        if seed is not None:
+            self.seed(seed)
+        ds = self.vl_svmdataset_new(dtype_code(x), ptr(x), d, n)
+        hmap = None
+        if hom is not None:
+            hmap = self.vl_homogeneouskernelmap_new(*hom)
+            self.vl_svmdataset_set_homogeneous_kernel_map(ds, hmap)
+        if abstract:
+            svm = self.vl_svm_new_with_abstract_data(solver, ds, self.vl_svmdataset_get_dimension(ds), n,
+                                                     ptr(y), lam)
+            self.vl_svm_set_data_functions(svm, self.vl_svmdataset_get_inner_product_function(ds),
+                                           self.vl_svmdataset_get_accumulate_function(ds))
+        else:
+            svm = self.vl_svm_new_with_dataset(solver, ds, ptr(y), lam)
+        self.vl_svm_set_loss(svm, loss)
+        if bias_multiplier is not None:
+            self.vl_svm_set_bias_multiplier(svm, bias_multiplier)
+        if epsilon is not None:
+            self.vl_svm_set_epsilon(svm, epsilon)
+        if max_iter is not None:
+            self.vl_svm_set_max_num_iterations(svm, max_iter)
+        if diag_freq is not None:
+            self.vl_svm_set_diagnostic_frequency(svm, diag_freq)
+        if bias_lr is not None:
+            self.vl_svm_set_bias_learning_rate(svm, bias_lr)
+        if w is not None:
+            self.vl_svm_set_weights(svm, ptr(w))
+        D = int(self.vl_svm_get_dimension(svm))
+        if model is not None:
+            m0 = np.ascontiguousarray(model, np.float64)
+            self.vl_svm_set_model(svm, ptr(m0))
+        if bias is not None:
+            self.vl_svm_set_bias(svm, bias)
+        seen = []
+        cb = None
+        if diagnostics:
+            CB = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p)
+
+            def on_diag(handle, _):
+                st = C.cast(self.vl_svm_get_statistics(handle), C.POINTER(VlSvmStatistics)).contents
+                seen.append({f: getattr(st, f) for f in SVM_STAT_FIELDS})
+            cb = CB(on_diag)
+            self.vl_svm_set_diagnostic_function(svm, C.cast(cb, C.c_void_p), None)
+        self.vl_svm_train(svm)
+        st = C.cast(self.vl_svm_get_statistics(svm), C.POINTER(VlSvmStatistics)).contents
+        out = {
+            "model": np.ctypeslib.as_array(C.cast(self.vl_svm_get_model(svm), C.POINTER(C.c_double)), (D,)).copy(),
+            "bias": float(self.vl_svm_get_bias(svm)),
+            "scores": np.ctypeslib.as_array(C.cast(self.vl_svm_get_scores(svm), C.POINTER(C.c_double)), (n,)).copy(),
+            "statistics": {f: getattr(st, f) for f in SVM_STAT_FIELDS},
+            "max_iter": int(self.vl_svm_get_max_num_iterations(svm)),
+            "diagnostics": seen,
+        }
+        self.vl_svm_delete(svm)
+        self.vl_svmdataset_delete(ds)
+        if hmap is not None:
+            self.vl_homogeneouskernelmap_delete(hmap)
+        del cb
+        return out
 
     # ---- GMM -----------------------------------------------------------
     def gmm_posteriors(self, data, means, covs, priors):

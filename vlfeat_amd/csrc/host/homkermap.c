@@ -14,6 +14,7 @@
  */
 #include "vl/homkermap.h"
 #include "vlhip.h"
+#include "svm_private.h"
 
 #include <assert.h>
 #include <math.h>
@@ -227,13 +228,36 @@ vl_homogeneouskernelmap_get_window_type (VlHomogeneousKernelMap const * self) { 
 HKM_EVALUATE(vl_homogeneouskernelmap_evaluate_d, double)
 HKM_EVALUATE(vl_homogeneouskernelmap_evaluate_f, float)
 
-VL_EXPORT void
-vl_homogeneouskernelmap_evaluate_array (VlHomogeneousKernelMap const * self, vl_type dataType,
-                                        void * destination, void const * data, vl_size numData)
+/* the map's table on the device, uploaded on first use (caller holds the lock) */
+static double const *
+homkermap_device_table (VlHomogeneousKernelMap const * self)
 {
   VlHomogeneousKernelMap * mut = (VlHomogeneousKernelMap*) self ;
   vl_size const F = 2 * self->order + 1 ;
   vl_size const rows = self->numSubdivisions * (vl_size)(self->maxExponent - self->minExponent + 1) ;
+  if (!mut->deviceTable) {
+    mut->deviceTable = vlhip_malloc(sizeof(double) * F * rows) ;
+    vlhip_memcpy_h2d(mut->deviceTable, self->table, sizeof(double) * F * rows) ;
+  }
+  return (double const*) mut->deviceTable ;
+}
+
+void
+homkermap_evaluate_device (VlHomogeneousKernelMap const * self, vl_type dataType, void * dOut,
+                           void const * dIn, vl_size n)
+{
+  if (n == 0) return ;
+  vlhip_lock() ;
+  vlhip_homkermap_evaluate((int) dataType, dOut, dIn, n, homkermap_device_table(self), (int) self->order,
+                           (int) self->numSubdivisions, (int) self->minExponent, (int) self->maxExponent) ;
+  vlhip_unlock() ;
+}
+
+VL_EXPORT void
+vl_homogeneouskernelmap_evaluate_array (VlHomogeneousKernelMap const * self, vl_type dataType,
+                                        void * destination, void const * data, vl_size numData)
+{
+  vl_size const F = 2 * self->order + 1 ;
   vl_size es ;
   void * dIn, * dOut ;
   if (dataType != VL_TYPE_FLOAT && dataType != VL_TYPE_DOUBLE) {
@@ -243,15 +267,10 @@ vl_homogeneouskernelmap_evaluate_array (VlHomogeneousKernelMap const * self, vl_
   if (numData == 0) return ;
   es = vl_get_type_size(dataType) ;
   vlhip_lock() ;
-  if (!mut->deviceTable) {
-    mut->deviceTable = vlhip_malloc(sizeof(double) * F * rows) ;
-    vlhip_memcpy_h2d(mut->deviceTable, self->table, sizeof(double) * F * rows) ;
-  }
   dIn = vlhip_buffer(VLHIP_BUF_DATA, es * numData) ;
   dOut = vlhip_buffer(VLHIP_BUF_ENC, es * F * numData) ;
   vlhip_memcpy_h2d_async(dIn, data, es * numData) ;
-  vlhip_homkermap_evaluate((int) dataType, dOut, dIn, numData, (double const*) mut->deviceTable, (int) self->order,
-                           (int) self->numSubdivisions, (int) self->minExponent, (int) self->maxExponent) ;
+  homkermap_evaluate_device(self, dataType, dOut, dIn, numData) ;
   vlhip_memcpy_d2h(destination, dOut, es * F * numData) ;
   vlhip_unlock() ;
 }
