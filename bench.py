@@ -528,7 +528,8 @@ def dropin_config(args) -> int:
     seeded custom start (random data points, data variance, uniform priors);
     vl_kmeans_cluster (Lloyd, 10 iterations) on the same data; then
     vl_fisher_encode (improved), the homogeneous kernel map of the encodings,
-    and vl_vlad_encode (one-hot assignments from
+    a linear SVM (vl_svm_train, SDCA) on the encodings, and vl_vlad_encode
+    (one-hot assignments from
     vl_kmeans_quantize against the GMM means) of 1,000 images x 3,000 descriptors
     (seed 2).  The same calls run on the reference (oracle/_ref, OpenMP over the
     host cores) on the same inputs: wall-clock of both, and the parity of every
@@ -630,6 +631,26 @@ def dropin_config(args) -> int:
         homk["bit_identical"] = bool(np.array_equal(hk, hr))
     log("dropin: homogeneous kernel map of the encodings: " + ", ".join(f"{k}={v}" for k, v in homk.items()))
     del hk
+    # linear SVM on the encodings, as the recognition application trains one
+    # per class (traintest.m, vl_svmtrain): SDCA, hinge loss, lambda 1e-4, the
+    # reference's defaults otherwise (epsilon 1e-2, diagnostics once per
+    # epoch, at most max(n, 10 / lambda) iterations); one-vs-rest labels of a
+    # 10-class split of the images
+    svm = {}
+    labels = np.where(np.arange(n_img) % 10 == 0, 1.0, -1.0)
+    t0 = time.perf_counter()
+    sg = L.svm_train(fvs, labels, 1e-4, solver=2, loss=0, seed=0)
+    svm["gpu_library_s"] = time.perf_counter() - t0
+    svm.update({"points": n_img, "dimension": int(fvs.shape[1]), "iterations": sg["statistics"]["iteration"] + 1,
+                "status": sg["statistics"]["status"], "objective": sg["statistics"]["objective"],
+                "duality_gap": sg["statistics"]["dualityGap"]})
+    if R is not None:
+        t0 = time.perf_counter()
+        sr = R.svm_train(fvs, labels, 1e-4, solver=2, loss=0, seed=0)
+        svm["reference_s"] = time.perf_counter() - t0
+        svm["model_rel_diff"] = float(np.linalg.norm(sg["model"] - sr["model"]) / np.linalg.norm(sr["model"]))
+        svm["same_iterations"] = sg["statistics"]["iteration"] == sr["statistics"]["iteration"]
+    log("dropin: SVM (SDCA) on the encodings: " + ", ".join(f"{k}={v}" for k, v in svm.items()))
 
     result = {
         "metric": "FP32 drop-in at SIFT scale (config 2): host C API, GPU library vs reference",
@@ -642,6 +663,7 @@ def dropin_config(args) -> int:
                                f"{n_img} images x {per}", "N": N, "K": K, "D": D, "images": n_img},
         "gpu_library_s": g_t,
         "homkermap": homk,
+        "svm": svm,
     }
     if R is not None:
         rel = lambda a, b: float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(np.asarray(b, np.float64)))
