@@ -1,7 +1,7 @@
 /* svmdataset.h -- data sets of the SVM solver (drop-in for VLFeat 0.9.21
  * vl/svmdataset.h).
  *
- * Same declarations as vl/svmdataset.h:24-80: a VlSvmDataset wraps a dense
+ * Same declarations as vl/svmdataset.h:37-76: a VlSvmDataset wraps a dense
  * float or double array of numData points of `dimension` values (point i at
  * data + i*dimension), optionally expanded on the fly by a homogeneous kernel
  * map (then each value becomes 2*order+1 features, laid out value by value).
@@ -19,7 +19,7 @@
 
 struct VlSvm_ ;
 
-/* vl/svmdataset.h:24-26: the solver's data abstraction */
+/* vl/svmdataset.h:44-48: the solver's data abstraction */
 typedef double (*VlSvmInnerProductFunction)(const void *data,
                                             vl_uindex element,
                                             double *model) ;
