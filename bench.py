@@ -250,6 +250,11 @@ def batched_config(args) -> int:
     ms = (C.c_double * 2)()
     nb = C.c_uint64()
     L.vlhip_encode_plan_info(plan, C.byref(variant), ms, C.byref(nb))
+    cands = []
+    cw, wv, sc, cms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+    for ci in range(L.vlhip_encode_plan_candidates(plan, -1, None, None, None, None)):
+        L.vlhip_encode_plan_candidates(plan, ci, C.byref(cw), C.byref(wv), C.byref(sc), C.byref(cms))
+        cands.append({"components_per_wave": cw.value, "waves": wv.value, "schedule": sc.value, "ms": cms.value})
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -257,8 +262,9 @@ def batched_config(args) -> int:
                                 vlad.data_ptr(), terms.data_ptr())
     dt = (time.perf_counter() - t0) / args.steps
     L.vlhip_encode_plan_delete(plan)
-    log(f"batched: {dt * 1000:.2f} ms per replay, tiling {variant.value} ({ms[0]:.2f} / {ms[1]:.2f} ms), "
-        f"first run (autotune + capture) {first_s:.2f} s")
+    log(f"batched: {dt * 1000:.2f} ms per replay, tiling {variant.value}; candidates " +
+        ", ".join(f"{c['components_per_wave']}x{c['waves']}/s{c['schedule']}: {c['ms']:.2f} ms" for c in cands) +
+        f"; first run (autotune + capture) {first_s:.2f} s")
     # per-image latency: single-image plans over a sample of the images
     rng = np.random.default_rng(0)
     sample = np.sort(rng.choice(n, min(256, n), replace=False))
@@ -284,7 +290,9 @@ def batched_config(args) -> int:
                                "improved FV + hard VLAD (nearest mean), one encode plan",
                    "images": n, "descriptors": N, "K": K, "D": D},
         "descriptors_per_s": N / dt,
-        "tiling": {"chosen": variant.value, "candidate_ms": [ms[0], ms[1]], "batches": nb.value},
+        "tiling": {"chosen": variant.value, "chosen_ms": ms[0], "candidates": cands, "batches": nb.value,
+                   "search": "every component tiling (components per wave x waves) with schedule 0, then "
+                             "schedules 1 and 2 with the fastest tiling; one warm-up and one timed run each"},
         "first_run_s": first_s,
         "per_image_latency_ms": {"sample": len(lat), "p50": float(np.percentile(lat, 50)),
                                  "p99": float(np.percentile(lat, 99)), "max": float(lat.max()),

@@ -158,9 +158,12 @@ VLHIP_API void vlhip_fisher_gate_stats (unsigned long long * groups, unsigned lo
  * reference, whose callers loop over images).  A plan fixes the dimension, the
  * number of components, the images' descriptor counts (offsets_host, HOST
  * array of numImages+1 values starting at 0) and the flags; it prepares the
- * image-aligned group tables and device buffers once and picks the
- * matrix-core component tiling by timing the candidates on the first run
- * (cached per shape).  vlhip_encode_plan_run computes the compensated-FP16
+ * image-aligned group tables and device buffers once and, on the first run,
+ * picks the second pass's component tiling (16, 32 components per wave x 4, 8
+ * waves per workgroup) and component-block schedule (XCD-aware image-major,
+ * linear, XCD-aware component-major) by timing candidates on the plan's own
+ * shapes: every tiling with the first schedule, then the other schedules with
+ * the fastest tiling (choices cached per shape).  vlhip_encode_plan_run computes the compensated-FP16
  * Fisher vectors (fisher.c:363-488 per image) and, when vladFlags >= 0, the
  * VLAD of every image with each descriptor assigned to its nearest mean
  * (kmeans.c:655-702, vlad.c:163-242); its kernel sequence is captured into a
@@ -179,9 +182,16 @@ VLHIP_API void   vlhip_encode_plan_run (void * plan, void const * data, void con
  * and center (inDimension) are device arrays copied into the plan. */
 VLHIP_API void   vlhip_encode_plan_set_projection (void * plan, size_t inDimension, void const * projection,
                                                    void const * center, int renormalize) ;
-/* chosen tiling (0: 8 waves x 32 components per workgroup, 1: 4 waves x 32),
- * the measured milliseconds of each candidate (-1 if not tried), batches */
+/* chosen tiling (0: 8 waves x 32 components per workgroup, 1: 4 x 32,
+ * 2: 8 x 16, 3: 4 x 16), the milliseconds of the chosen candidate and of
+ * the default tiling with the first schedule (-1 if not timed: a choice
+ * reused from the cache), batches */
 VLHIP_API void   vlhip_encode_plan_info (void * plan, int * variant, double * variantMs, size_t * numBatches) ;
+/* The autotuner's candidates in the order timed: returns their number and,
+ * for 0 <= index < number, candidate index's components per wave, waves per
+ * workgroup, schedule (0, 1, 2 as above) and milliseconds per run. */
+VLHIP_API int    vlhip_encode_plan_candidates (void * plan, int index, int * componentsPerWave, int * waves,
+                                               int * schedule, double * ms) ;
 VLHIP_API void   vlhip_encode_plan_delete (void * plan) ;
 
 /* ---- pre-encoding projection (apps/recognition) ---------------------------- */

@@ -199,6 +199,8 @@ _EXT_SIGS = {
     "vlhip_encode_plan_run": (None, [_p, _p, _p, _p, _p, _p, _p, _p]),
     "vlhip_encode_plan_info": (None, [_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "vlhip_encode_plan_delete": (None, [_p]),
+    "vlhip_encode_plan_candidates": (C.c_int, [_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                               C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "vlhip_encode_plan_set_projection": (None, [_p, _sz, _p, _p, C.c_int]),
     "vl_homogeneouskernelmap_evaluate_array": (None, [_p, C.c_uint32, _p, _p, _sz]),
     "vlhip_homkermap_evaluate": (None, [C.c_int, _p, _p, _sz, _p, C.c_int, C.c_int, C.c_int, C.c_int]),

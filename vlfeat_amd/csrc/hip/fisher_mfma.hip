@@ -29,6 +29,7 @@
 #include "vlhip.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <type_traits>
 #include <vector>
@@ -802,18 +803,31 @@ __global__ __launch_bounds__(256) void mfma_sparse_kernel(
   }
 }
 
-// XCD-aware block mapping: workgroups are dealt to the 8 XCDs round robin, so
-// all component blocks of one image land on the same XCD and share its L2
-// copy of the image's group records.
-__device__ __forceinline__ void fisher_mfma_coords(int numKB, int numImages, int& img, int& kb) {
+// Block mapping of the second pass (component-block schedule).  Workgroups
+// are dealt to the 8 XCDs round robin.
+//   0  XCD-aware, image-major: all component blocks of one image land on the
+//      same XCD, one after the other, and share its L2 copy of the image's
+//      group records (the default);
+//   1  linear, image-major (blocks of one image consecutive in launch order,
+//      spread over the XCDs);
+//   2  XCD-aware, component-major: an XCD walks its images for component
+//      block 0, then block 1, ... (weights reused, records re-read).
+// With fewer than 8 images every schedule is the linear one.  The grid is
+// padImages x numKB (padImages: numImages rounded up to 8 from 8 images on);
+// blocks past the last image return.
+__device__ __forceinline__ void fisher_mfma_coords(int numKB, int numImages, int sched, int& img, int& kb) {
   const int b = blockIdx.x;
-  if (numImages >= 8) {
+  if (numImages < 8 || sched == 1) {
+    img = b / numKB;
+    kb = b % numKB;
+  } else if (sched == 2) {
+    const int xcd = b & 7, slot = b >> 3, perX = (numImages + 7) / 8;
+    kb = slot / perX;
+    img = (slot - kb * perX) * 8 + xcd;
+  } else {
     const int xcd = b & 7, slot = b >> 3;
     img = (slot / numKB) * 8 + xcd;
     kb = slot % numKB;
-  } else {
-    img = b / numKB;
-    kb = b % numKB;
   }
 }
 
@@ -931,7 +945,7 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
     const float* __restrict__ isg, const float* __restrict__ sinv, const T* __restrict__ priors, int flags,
     T* __restrict__ enc, double* __restrict__ normpart, unsigned long long* __restrict__ numTerms, int numKB,
     const unsigned char* __restrict__ gflag, const int2* __restrict__ list, int M, float emThr,
-    double* __restrict__ emOut) {
+    double* __restrict__ emOut, int sched) {
   typedef typename VecOf<H>::type V8;
   typedef GroupRec<NS> R;
   constexpr int Dp = NS * 16;
@@ -943,7 +957,7 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
   __shared__ double red[WAVES];
 
   int li, kb;
-  fisher_mfma_coords(numKB, numImages, li, kb);
+  fisher_mfma_coords(numKB, numImages, sched, li, kb);
   if (li >= numImages) return;
   const int img = imgBegin + li;
 
@@ -1849,21 +1863,22 @@ template <typename H, int NS, int CT, int WAVES>
 static void mfma_second_pass(const MfmaTables<H>& t, int D, int K, int flags, unsigned char* rec,
                              const uint64_t* offsets, const long long* gstart, int b0, int n, const float* priors,
                              float* enc, double* normpart, unsigned long long* numTerms, int numKB,
-                             const unsigned char* gflag, const int2* list, int M, size_t shmem) {
+                             const unsigned char* gflag, const int2* list, int M, size_t shmem, int sched = 0) {
   const int padImages = n >= 8 ? ((n + 7) / 8) * 8 : n;
   VLHIP_LAUNCH("mfma_fisher", (mfma_fisher_kernel<float, H, NS, CT, WAVES>), dim3((unsigned)((size_t)padImages * numKB)),
                dim3(WAVES * 64), shmem, rec, offsets, gstart, b0, n, D, K, t.Whi, t.Wlo, t.bias, t.scp, t.mup,
-               t.isg, t.sinv, priors, flags, enc, normpart, numTerms, numKB, gflag, list, M, 0.f, nullptr);
+               t.isg, t.sinv, priors, flags, enc, normpart, numTerms, numKB, gflag, list, M, 0.f, nullptr, sched);
 }
 
 template <typename H, int NS, int CT, int WAVES>
 static void mfma_batch_launch(const MfmaTables<H>& t, const float* xb, int D, int K, int Kp, int flags,
                               unsigned char* rec, size_t nG, const int4* gtab, const uint64_t* offsets,
                               const long long* gstart, int b0, int n, const float* priors, float* enc,
-                              double* normpart, unsigned long long* numTerms, int numKB, size_t shmem) {
+                              double* normpart, unsigned long long* numTerms, int numKB, size_t shmem,
+                              int sched = 0) {
   mfma_first_pass<H, NS>(t, xb, D, Kp, flags, rec, nG, gtab);
   mfma_second_pass<H, NS, CT, WAVES>(t, D, K, flags, rec, offsets, gstart, b0, n, priors, enc, normpart, numTerms,
-                                     numKB, nullptr, nullptr, 0, shmem);
+                                     numKB, nullptr, nullptr, 0, shmem, sched);
 }
 
 template <typename H, int NS, int CT, int WAVES>
@@ -2152,8 +2167,10 @@ struct PlanBatch {
 struct EncodePlan {
   size_t D = 0, K = 0, numImages = 0, totalRows = 0;
   int flags = 0, vladFlags = -1, NS = 0;
-  int variant = -1;            // component tiling: 0 = 8 waves x 32, 1 = 4 waves x 32
-  double variantMs[2] = {-1.0, -1.0};
+  int tiling = -1;             // chosen component tiling (kPlanTilings index) ...
+  int sched = 0;               // ... and component-block schedule (fisher_mfma_coords)
+  int chosen = -1;             // index of the chosen candidate in tried (-1: cached choice)
+  std::vector<std::array<double, 3>> tried;  // autotuner candidates {tiling, schedule, ms}
   std::vector<PlanBatch> batches;
   size_t maxGroups = 0;
   uint64_t* dOff = nullptr;
@@ -2176,16 +2193,31 @@ struct EncodePlan {
   float* dProjX = nullptr;  // totalRows x D, the projected descriptors
 };
 
+// Component tilings of the second pass the autotuner tries: CT tiles of 16
+// components per wave, WAVES waves per workgroup (a workgroup owns 16 CT WAVES
+// components).  {2, 8} needs D <= 80 (registers) and is tried only when
+// K > 128; the others for every shape.
+struct PlanTiling {
+  int ct, waves;
+};
+constexpr PlanTiling kPlanTilings[4] = {{2, 8}, {2, 4}, {1, 8}, {1, 4}};
+constexpr int kPlanSchedules = 3;
+constexpr int kPlanMinBlock = 64;  // smallest 16 CT WAVES of the tilings
+
+static bool plan_tiling_allowed(int tiling, int NS, size_t K) {
+  return tiling != 0 || (NS <= 5 && K > 128);
+}
+
 // autotuner cache: (D, K, Fisher flags, VLAD flags, load-stage input dimension,
-// renormalisation, rows bucket) -> variant
-static std::vector<std::pair<std::vector<long long>, int>>& tuning_cache() {
-  static std::vector<std::pair<std::vector<long long>, int>> c;
+// renormalisation, rows bucket, images bucket) -> (tiling, schedule)
+static std::vector<std::pair<std::vector<long long>, std::pair<int, int>>>& tuning_cache() {
+  static std::vector<std::pair<std::vector<long long>, std::pair<int, int>>> c;
   return c;
 }
 
 template <typename H, int NS, int CT, int WAVES>
-static void plan_enqueue(EncodePlan& P, const float* x, const float* means, const float* covs, const float* priors,
-                         float* enc, float* vladEnc, unsigned long long* numTerms) {
+static void plan_enqueue(EncodePlan& P, int sched, const float* x, const float* means, const float* covs,
+                         const float* priors, float* enc, float* vladEnc, unsigned long long* numTerms) {
   constexpr int Dp = NS * 16;
   constexpr int BC = WAVES * 16 * CT;
   const int Kp = (int)((P.K + BC - 1) / BC) * BC;
@@ -2197,7 +2229,7 @@ static void plan_enqueue(EncodePlan& P, const float* x, const float* means, cons
   for (const PlanBatch& b : P.batches)
     mfma_batch_launch<H, NS, CT, WAVES>(t, x + b.rowBase * P.D, (int)P.D, (int)P.K, Kp, P.flags, P.dRec, b.nG,
                                         P.dGtab + b.gOff, P.dOff, P.dGstart, (int)b.b0, (int)b.n, priors, enc,
-                                        P.dNorm, numTerms, numKB, shmem);
+                                        P.dNorm, numTerms, numKB, shmem, sched);
   if (P.flags & 2) launch_fisher_normalize(enc, P.dNorm, numKB, P.numImages, (size_t)2 * P.K * P.D);
   if (P.vladFlags >= 0 && vladEnc) {
     // the plan's own centre table: no library buffer is captured into the graph
@@ -2207,33 +2239,40 @@ static void plan_enqueue(EncodePlan& P, const float* x, const float* means, cons
 }
 
 template <typename H, int NS>
-static void plan_enqueue_variant(EncodePlan& P, int variant, const float* x, const float* means, const float* covs,
-                                 const float* priors, float* enc, float* vladEnc, unsigned long long* numTerms) {
-  if constexpr (NS <= 5) {
-    if (variant == 0) {
-      plan_enqueue<H, NS, 2, 8>(P, x, means, covs, priors, enc, vladEnc, numTerms);
-      return;
-    }
+static void plan_enqueue_variant(EncodePlan& P, int tiling, int sched, const float* x, const float* means,
+                                 const float* covs, const float* priors, float* enc, float* vladEnc,
+                                 unsigned long long* numTerms) {
+  switch (tiling) {
+    case 0:
+      if constexpr (NS <= 5) {
+        plan_enqueue<H, NS, 2, 8>(P, sched, x, means, covs, priors, enc, vladEnc, numTerms);
+        return;
+      }
+      break;
+    case 2: plan_enqueue<H, NS, 1, 8>(P, sched, x, means, covs, priors, enc, vladEnc, numTerms); return;
+    case 3: plan_enqueue<H, NS, 1, 4>(P, sched, x, means, covs, priors, enc, vladEnc, numTerms); return;
+    default: break;
   }
-  plan_enqueue<H, NS, 2, 4>(P, x, means, covs, priors, enc, vladEnc, numTerms);
+  plan_enqueue<H, NS, 2, 4>(P, sched, x, means, covs, priors, enc, vladEnc, numTerms);
 }
 
-static void plan_enqueue_any(EncodePlan& P, int variant, const float* x, const float* means, const float* covs,
-                             const float* priors, float* enc, float* vladEnc, unsigned long long* numTerms) {
+static void plan_enqueue_any(EncodePlan& P, int tiling, int sched, const float* x, const float* means,
+                             const float* covs, const float* priors, float* enc, float* vladEnc,
+                             unsigned long long* numTerms) {
   if (P.Din) {
     // load stage: the encoders below read the projected descriptors
     project_t(P.dProjX, x, P.totalRows, P.Din, P.D, P.dProjP, P.dProjC, P.renorm);
     x = P.dProjX;
   }
   switch (P.NS) {
-    case 1: plan_enqueue_variant<_Float16, 1>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
-    case 2: plan_enqueue_variant<_Float16, 2>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
-    case 3: plan_enqueue_variant<_Float16, 3>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
-    case 4: plan_enqueue_variant<_Float16, 4>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
-    case 5: plan_enqueue_variant<_Float16, 5>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
-    case 6: plan_enqueue_variant<_Float16, 6>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
-    case 7: plan_enqueue_variant<_Float16, 7>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
-    default: plan_enqueue_variant<_Float16, 8>(P, variant, x, means, covs, priors, enc, vladEnc, numTerms); break;
+    case 1: plan_enqueue_variant<_Float16, 1>(P, tiling, sched, x, means, covs, priors, enc, vladEnc, numTerms); break;
+    case 2: plan_enqueue_variant<_Float16, 2>(P, tiling, sched, x, means, covs, priors, enc, vladEnc, numTerms); break;
+    case 3: plan_enqueue_variant<_Float16, 3>(P, tiling, sched, x, means, covs, priors, enc, vladEnc, numTerms); break;
+    case 4: plan_enqueue_variant<_Float16, 4>(P, tiling, sched, x, means, covs, priors, enc, vladEnc, numTerms); break;
+    case 5: plan_enqueue_variant<_Float16, 5>(P, tiling, sched, x, means, covs, priors, enc, vladEnc, numTerms); break;
+    case 6: plan_enqueue_variant<_Float16, 6>(P, tiling, sched, x, means, covs, priors, enc, vladEnc, numTerms); break;
+    case 7: plan_enqueue_variant<_Float16, 7>(P, tiling, sched, x, means, covs, priors, enc, vladEnc, numTerms); break;
+    default: plan_enqueue_variant<_Float16, 8>(P, tiling, sched, x, means, covs, priors, enc, vladEnc, numTerms); break;
   }
 }
 
@@ -2243,16 +2282,23 @@ static void plan_set_attr() {
   VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)mfma_shmem<_Float16, NS, CT, WAVES>(false)));
 }
+template <int NS>
+static void plan_set_attrs_ns() {
+  if constexpr (NS <= 5) plan_set_attr<NS, 2, 8>();
+  plan_set_attr<NS, 2, 4>();
+  plan_set_attr<NS, 1, 8>();
+  plan_set_attr<NS, 1, 4>();
+}
 static void plan_set_attrs(int NS) {
   switch (NS) {
-    case 1: plan_set_attr<1, 2, 8>(); plan_set_attr<1, 2, 4>(); break;
-    case 2: plan_set_attr<2, 2, 8>(); plan_set_attr<2, 2, 4>(); break;
-    case 3: plan_set_attr<3, 2, 8>(); plan_set_attr<3, 2, 4>(); break;
-    case 4: plan_set_attr<4, 2, 8>(); plan_set_attr<4, 2, 4>(); break;
-    case 5: plan_set_attr<5, 2, 8>(); plan_set_attr<5, 2, 4>(); break;
-    case 6: plan_set_attr<6, 2, 4>(); break;
-    case 7: plan_set_attr<7, 2, 4>(); break;
-    default: plan_set_attr<8, 2, 4>(); break;
+    case 1: plan_set_attrs_ns<1>(); break;
+    case 2: plan_set_attrs_ns<2>(); break;
+    case 3: plan_set_attrs_ns<3>(); break;
+    case 4: plan_set_attrs_ns<4>(); break;
+    case 5: plan_set_attrs_ns<5>(); break;
+    case 6: plan_set_attrs_ns<6>(); break;
+    case 7: plan_set_attrs_ns<7>(); break;
+    default: plan_set_attrs_ns<8>(); break;
   }
 }
 
@@ -2515,7 +2561,7 @@ __global__ __launch_bounds__(WAVES * 64, 1) void f32_fisher_kernel(
   __shared__ double red[WAVES];
 
   int li, kb;
-  fisher_mfma_coords(numKB, numImages, li, kb);
+  fisher_mfma_coords(numKB, numImages, 0, li, kb);
   if (li >= numImages) return;
   const int img = imgBegin + li;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -2923,7 +2969,7 @@ static double run_em_step(double* out_host, const float* means, const float* cov
       VLHIP_LAUNCH("mfma_em_moments", (mfma_fisher_kernel<float, H, NS, CT, WAVES, true>), dim3((unsigned)((size_t)padImages * numKB)), dim3(WAVES * 64), shmem, rec,
                    offsets, gstart, (int)b0, n, (int)D, (int)K, t.Whi, t.Wlo, t.bias, t.scp, t.mup, t.isg, t.sinv,
                    priors, 0, (float*)nullptr, (double*)nullptr, (unsigned long long*)nullptr, numKB,
-                   (const unsigned char*)nullptr, (const int2*)nullptr, 0, emThr, part);
+                   (const unsigned char*)nullptr, (const int2*)nullptr, 0, emThr, part, 0);
     }
     const size_t o = llh.size();
     llh.resize(o + kLLBlocks);
@@ -3035,7 +3081,7 @@ void* vlhip_encode_plan_new(size_t D, size_t K, uint64_t const* offsets_host, si
   const int Kp = (int)((K + 127) / 128) * 128 > (int)((K + 255) / 256) * 256 ? (int)((K + 127) / 128) * 128
                                                                               : (int)((K + 255) / 256) * 256;
   VLHIP_CHECK(hipMalloc(&P->dTables, mfma_tables_bytes<_Float16>(P->NS * 16, Kp)));
-  VLHIP_CHECK(hipMalloc(&P->dNorm, numImages * (size_t)(Kp / 128) * sizeof(double)));
+  VLHIP_CHECK(hipMalloc(&P->dNorm, numImages * (size_t)(Kp / kPlanMinBlock) * sizeof(double)));
   if (vladFlags >= 0) {
     VLHIP_CHECK(hipMalloc(&P->dAssign, (P->totalRows ? P->totalRows : 1) * sizeof(uint32_t)));
     const size_t qb = quantize_screened_bytes(D, K);
@@ -3077,35 +3123,54 @@ void vlhip_encode_plan_run(void* plan, void const* data, void const* means, void
   hipStream_t saved = ctx().stream;
   const bool savedTiming = ctx().timing;
   ctx().stream = P->stream;
-  if (P->variant < 0) {
-    // autotuner: reuse a cached choice for this shape, else time both tilings
+  if (P->tiling < 0) {
+    // Autotuner, on the plan's own shapes: every allowed component tiling with
+    // the default schedule, then the other component-block schedules with the
+    // fastest tiling; each candidate is run once to warm up and once timed
+    // (hipEvents on the plan's stream).  Choices are cached per shape.
     std::vector<long long> shape = {(long long)P->D, (long long)P->K, (long long)P->flags,
                                     (long long)P->vladFlags, (long long)P->Din, (long long)P->renorm,
-                                    (long long)(64 - __builtin_clzll((unsigned long long)P->totalRows | 1))};
+                                    (long long)(64 - __builtin_clzll((unsigned long long)P->totalRows | 1)),
+                                    (long long)(64 - __builtin_clzll((unsigned long long)P->numImages | 1))};
     for (auto& e : tuning_cache())
-      if (e.first == shape) P->variant = e.second;
-    if (P->variant < 0) {
+      if (e.first == shape) {
+        P->tiling = e.second.first;
+        P->sched = e.second.second;
+      }
+    if (P->tiling < 0) {
       hipEvent_t e0, e1;
       VLHIP_CHECK(hipEventCreate(&e0));
       VLHIP_CHECK(hipEventCreate(&e1));
       ctx().timing = false;
-      for (int v = 0; v < 2; ++v) {
-        if (v == 0 && (P->NS > 5 || P->K <= 128)) continue;
-        plan_enqueue_any(*P, v, x, (const float*)means, (const float*)covs, (const float*)priors, (float*)fisherEnc,
-                         (float*)vladEnc, numTerms);  // warm-up
+      auto timed = [&](int tiling, int sched) {
+        plan_enqueue_any(*P, tiling, sched, x, (const float*)means, (const float*)covs, (const float*)priors,
+                         (float*)fisherEnc, (float*)vladEnc, numTerms);  // warm-up
         VLHIP_CHECK(hipEventRecord(e0, P->stream));
-        plan_enqueue_any(*P, v, x, (const float*)means, (const float*)covs, (const float*)priors, (float*)fisherEnc,
-                         (float*)vladEnc, numTerms);
+        plan_enqueue_any(*P, tiling, sched, x, (const float*)means, (const float*)covs, (const float*)priors,
+                         (float*)fisherEnc, (float*)vladEnc, numTerms);
         VLHIP_CHECK(hipEventRecord(e1, P->stream));
         VLHIP_CHECK(hipEventSynchronize(e1));
         float ms = 0.f;
         VLHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        P->variantMs[v] = ms;
-      }
+        P->tried.push_back({(double)tiling, (double)sched, (double)ms});
+      };
+      for (int ti = 0; ti < 4; ++ti)
+        if (plan_tiling_allowed(ti, P->NS, P->K)) timed(ti, 0);
+      auto best = [&]() {
+        size_t b = 0;
+        for (size_t i = 1; i < P->tried.size(); ++i)
+          if (P->tried[i][2] < P->tried[b][2]) b = i;
+        return b;
+      };
+      const int bestTiling = (int)P->tried[best()][0];
+      if (P->numImages >= 8)  // with fewer images every schedule is the linear one
+        for (int sc = 1; sc < kPlanSchedules; ++sc) timed(bestTiling, sc);
       VLHIP_CHECK(hipEventDestroy(e0));
       VLHIP_CHECK(hipEventDestroy(e1));
-      P->variant = (P->variantMs[0] >= 0.0 && (P->variantMs[1] < 0.0 || P->variantMs[0] <= P->variantMs[1])) ? 0 : 1;
-      tuning_cache().push_back({shape, P->variant});
+      P->chosen = (int)best();
+      P->tiling = (int)P->tried[P->chosen][0];
+      P->sched = (int)P->tried[P->chosen][1];
+      tuning_cache().push_back({shape, {P->tiling, P->sched}});
     }
   }
   bool same = P->exec != nullptr;
@@ -3118,7 +3183,7 @@ void vlhip_encode_plan_run(void* plan, void const* data, void const* means, void
     ctx().timing = false;  // no event records inside the capture
     VLHIP_CHECK(hipStreamBeginCapture(P->stream, hipStreamCaptureModeRelaxed));
     ctx().capturing = true;  // scratch() refuses to run: nothing library-owned enters the graph
-    plan_enqueue_any(*P, P->variant, x, (const float*)means, (const float*)covs, (const float*)priors,
+    plan_enqueue_any(*P, P->tiling, P->sched, x, (const float*)means, (const float*)covs, (const float*)priors,
                      (float*)fisherEnc, (float*)vladEnc, numTerms);
     ctx().capturing = false;
     VLHIP_CHECK(hipStreamEndCapture(P->stream, &P->graph));
@@ -3163,9 +3228,31 @@ void vlhip_encode_plan_set_projection(void* plan, size_t inDimension, void const
 
 void vlhip_encode_plan_info(void* plan, int* variant, double* variantMs, size_t* numBatches) {
   EncodePlan* P = (EncodePlan*)plan;
-  if (variant) *variant = P->variant;
-  if (variantMs) { variantMs[0] = P->variantMs[0]; variantMs[1] = P->variantMs[1]; }
+  if (variant) *variant = P->tiling;
+  if (variantMs) {
+    // time of the chosen candidate, and of the default (tiling 0 or 1 with
+    // schedule 0) when it was tried
+    variantMs[0] = variantMs[1] = -1.0;
+    if (P->chosen >= 0) variantMs[0] = P->tried[(size_t)P->chosen][2];
+    for (const auto& c : P->tried)
+      if (c[1] == 0.0 && c[0] == (plan_tiling_allowed(0, P->NS, P->K) ? 0.0 : 1.0)) variantMs[1] = c[2];
+  }
   if (numBatches) *numBatches = P->batches.size();
+}
+
+int vlhip_encode_plan_candidates(void* plan, int index, int* componentsPerWave, int* waves, int* schedule,
+                                 double* ms) {
+  EncodePlan* P = (EncodePlan*)plan;
+  const int n = (int)P->tried.size();
+  if (index >= 0 && index < n) {
+    const auto& c = P->tried[(size_t)index];
+    const PlanTiling& t = kPlanTilings[(int)c[0]];
+    if (componentsPerWave) *componentsPerWave = 16 * t.ct;
+    if (waves) *waves = t.waves;
+    if (schedule) *schedule = (int)c[1];
+    if (ms) *ms = c[2];
+  }
+  return n;
 }
 
 }  // extern "C"
