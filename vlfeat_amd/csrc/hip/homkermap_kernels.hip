@@ -6,11 +6,15 @@
 // maxExponent), otherwise linear interpolation between the two table rows that
 // bracket the mantissa, the row found by repeated subtraction of the
 // subdivision as the reference finds it.  Double arithmetic without
-// contraction, so each value is the reference's bit for bit.  One thread per
-// input value; the table (at most a few tens of KB) stays in the caches.
+// contraction, so each value is the reference's bit for bit.  The table (at
+// most a few tens of KB) stays in the caches.  homkermap_vec_kernel maps 16
+// bytes of input per thread; homkermap_kernel (one value per thread) serves
+// unaligned arrays and orders above 4.
 #include "vlhip_runtime.h"
 #include "vlhip_device.h"
 #include "vlhip.h"
+
+#include <algorithm>
 
 namespace vlhip {
 
@@ -54,9 +58,140 @@ __global__ __launch_bounds__(256) void homkermap_kernel(const T* __restrict__ in
   }
 }
 
+// The map of one value (same operations as homkermap_kernel).  With POW2 the
+// number of subdivisions is a power of two, so sub = 2^-s is exact and every
+// step of the reference's repeated subtraction, mantissa - k sub with
+// mantissa in [0, 1), is exact too (k sub is a multiple of 2^-s, itself a
+// multiple of the mantissa's ulp, and the difference is smaller than the
+// mantissa): k = floor(mantissa * numSub), computed exactly, gives the row and
+// the remainder the loop would give.
+template <typename T, int ORDER, bool POW2>
+__device__ __forceinline__ void homkermap_one(T x, const double* __restrict__ table, int numSub, double sub,
+                                              int minExp, int maxExp, T (&o)[2 * ORDER + 1]) {
+#pragma clang fp contract(off)
+  constexpr int F = 2 * ORDER + 1;
+  int exponent;
+  double mantissa = frexp((double)x, &exponent);
+  const double sign = (mantissa >= 0.0) ? +1.0 : -1.0;
+  mantissa *= 2 * sign;
+  exponent--;
+  if (mantissa == 0 || exponent <= minExp || exponent >= maxExp) {
+#pragma unroll
+    for (int j = 0; j < F; ++j) o[j] = T(0);
+    return;
+  }
+  const double* lo = table + (size_t)(exponent - minExp) * (size_t)numSub * F;
+  mantissa -= 1.0;
+  if constexpr (POW2) {
+    const int k = (int)(mantissa * (double)numSub);
+    mantissa -= (double)k * sub;
+    lo += (size_t)k * F;
+  } else {
+    while (mantissa >= sub) {
+      mantissa -= sub;
+      lo += F;
+    }
+  }
+  const double* hi = lo + F;
+  const double scale = (double)numSub * mantissa;
+  double l[F], h[F];
+#pragma unroll
+  for (int j = 0; j < F; ++j) { l[j] = lo[j]; h[j] = hi[j]; }
+#pragma unroll
+  for (int j = 0; j < F; ++j) o[j] = (T)((T)sign * ((h[j] - l[j]) * scale + l[j]));
+}
+
+// 16 bytes of input (VPT values) from base, mapped; outputs written at out + base F.
+template <typename T, int ORDER, bool POW2, int VPT, int W>
+__device__ __forceinline__ void homkermap_chunk(const T* __restrict__ in, long long n, long long base,
+                                                const double* table, int numSub, double sub, int minExp, int maxExp,
+                                                T* __restrict__ out) {
+  constexpr int F = 2 * ORDER + 1;
+  typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+  const bool whole = base + VPT <= n;
+  T v[VPT];
+  if (whole) {
+    const u32x4 w = *reinterpret_cast<const u32x4*>(in + base);
+    __builtin_memcpy(v, &w, 16);
+  } else {
+#pragma unroll
+    for (int j = 0; j < VPT; ++j) v[j] = base + j < n ? in[base + j] : T(0);
+  }
+  T o[VPT * F];
+#pragma unroll
+  for (int j = 0; j < VPT; ++j) {
+    T m[F];
+    homkermap_one<T, ORDER, POW2>(v[j], table, numSub, sub, minExp, maxExp, m);
+#pragma unroll
+    for (int f = 0; f < F; ++f) o[j * F + f] = m[f];
+  }
+  T* dst = out + base * F;
+  if (whole) {
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+      u32x4 w;
+      __builtin_memcpy(&w, &o[q * (16 / (int)sizeof(T))], 16);
+      *reinterpret_cast<u32x4*>(dst + q * (16 / (int)sizeof(T))) = w;
+    }
+  } else {
+    for (int j = 0; j < VPT && base + j < n; ++j)
+#pragma unroll
+      for (int f = 0; f < F; ++f) dst[j * F + f] = o[j * F + f];
+  }
+}
+
+// Persistent form: each workgroup copies the whole table (rows x F doubles,
+// at most 84 KB for order 4) into LDS once, then maps 16 bytes of input per
+// thread (4 floats or 2 doubles) in a grid-stride loop, reading the table rows
+// from LDS; the VPT x F outputs of a thread are contiguous and written as
+// 16-byte stores (VPT F sizeof(T) = 16 (2 ORDER + 1) bytes); a ragged end goes
+// value by value.  in and out 16-byte aligned.
+template <typename T, int ORDER, bool POW2>
+__global__ __launch_bounds__(256) void homkermap_vec_kernel(const T* __restrict__ in, long long n,
+                                                            const double* __restrict__ table, long long tableLen,
+                                                            int numSub, double sub, int minExp, int maxExp,
+                                                            T* __restrict__ out) {
+  extern __shared__ double tsh[];
+  constexpr int F = 2 * ORDER + 1;
+  constexpr int VPT = 16 / (int)sizeof(T);
+  constexpr int W = VPT * F * (int)sizeof(T) / 16;  // 16-byte output words per thread
+  for (long long i = threadIdx.x; i < tableLen; i += 256) tsh[i] = table[i];
+  __syncthreads();
+  const long long chunks = (n + VPT - 1) / VPT;
+  for (long long c = blockIdx.x * 256LL + threadIdx.x; c < chunks; c += (long long)gridDim.x * 256) {
+    homkermap_chunk<T, ORDER, POW2, VPT, W>(in, n, c * VPT, tsh, numSub, sub, minExp, maxExp, out);
+  }
+}
+
 template <typename T>
 static void homkermap_launch(const T* in, size_t n, const double* table, int order, int numSub, double sub, int minExp,
                              int maxExp, T* out) {
+  const bool aligned = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+  const bool pow2 = (numSub & (numSub - 1)) == 0;
+  const long long tableLen = (long long)numSub * (maxExp - minExp + 1) * (2 * order + 1);
+  const size_t lds = (size_t)tableLen * sizeof(double);
+  if (aligned && order >= 1 && order <= 4 && lds <= 96 * 1024) {
+    constexpr int VPT = 16 / (int)sizeof(T);
+    const long long blocksNeeded = ((long long)((n + VPT - 1) / VPT) + 255) / 256;
+    auto go = [&](auto kern) {
+      VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      int perCu = 0;
+      VLHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, (const void*)kern, 256, lds));
+      const long long G = std::min<long long>(blocksNeeded, (long long)std::max(perCu, 1) * ctx().num_cus);
+      VLHIP_LAUNCH("homkermap", kern, dim3((unsigned)G), dim3(256), lds, in, (long long)n, table, tableLen, numSub,
+                   sub, minExp, maxExp, out);
+    };
+    switch (order * 2 + (pow2 ? 1 : 0)) {
+      case 2: go(homkermap_vec_kernel<T, 1, false>); return;
+      case 3: go(homkermap_vec_kernel<T, 1, true>); return;
+      case 4: go(homkermap_vec_kernel<T, 2, false>); return;
+      case 5: go(homkermap_vec_kernel<T, 2, true>); return;
+      case 6: go(homkermap_vec_kernel<T, 3, false>); return;
+      case 7: go(homkermap_vec_kernel<T, 3, true>); return;
+      case 8: go(homkermap_vec_kernel<T, 4, false>); return;
+      default: go(homkermap_vec_kernel<T, 4, true>); return;
+    }
+  }
   const dim3 grid((unsigned)((n + 255) / 256));
   switch (order) {
 #define HK_CASE(O)                                                                                              \
