@@ -534,7 +534,7 @@ def dropin_config(args) -> int:
     caller of the reference uses it: vl_gmm_cluster on N = 1,000,000 SIFT-like
     descriptors projected to 80-d (seed 1), K = 256, 30 EM iterations from a
     seeded custom start (random data points, data variance, uniform priors);
-    vl_kmeans_cluster (Lloyd, 10 iterations) on the same data; then
+    vl_kmeans_cluster (Lloyd, and Elkan, 10 iterations each) on the same data; then
     vl_fisher_encode (improved), the homogeneous kernel map of the encodings,
     a linear SVM (vl_svm_train, SDCA) on the encodings, and vl_vlad_encode
     (one-hot assignments from
@@ -589,6 +589,14 @@ def dropin_config(args) -> int:
         out["kmeans_s"] = time.perf_counter() - t0
         kc = lib.kmeans_centers(km, np.float32)
         lib.vl_kmeans_delete(km)
+        # the same with Elkan's refinement (kmeans.c:1162-1745)
+        lib.seed(0)
+        km = lib.kmeans_new(np.float32, algorithm=va.VlKMeansElkan, max_iter=10)
+        t0 = time.perf_counter()
+        kee = lib.vl_kmeans_cluster(km, x.ctypes.data, D, N, K)
+        out["kmeans_elkan_s"] = time.perf_counter() - t0
+        kce = lib.kmeans_centers(km, np.float32)
+        lib.vl_kmeans_delete(km)
         t0 = time.perf_counter()
         vl = []
         for i in range(n_img):
@@ -596,7 +604,7 @@ def dropin_config(args) -> int:
             oh[np.arange(per), a[i * per:(i + 1) * per]] = 1
             vl.append(lib.vlad(xe[i * per:(i + 1) * per], m, oh, 0))
         out["vlad_s"] = time.perf_counter() - t0
-        return out, em + (fv, a, np.stack(vl), kc, ke)
+        return out, em + (fv, a, np.stack(vl), kc, ke, kce, kee)
 
     R = reference() if not args.no_cpu else None
     r_t = r = None
@@ -604,10 +612,11 @@ def dropin_config(args) -> int:
         R.set_num_threads(0)
         r_t, r = run(R)
         log(f"dropin: reference ({int(R.vl_get_max_threads())} threads): EM {r_t['em_s']:.2f} s, "
-            f"k-means {r_t['kmeans_s']:.2f} s, "
+            f"k-means {r_t['kmeans_s']:.2f} s (Elkan {r_t['kmeans_elkan_s']:.2f} s), "
             f"Fisher {r_t['fisher_s']:.2f} s, quantize {r_t['quantize_s']:.2f} s, VLAD {r_t['vlad_s']:.2f} s")
     g_t, g = run(L, model=r[:3] if r is not None else None)
-    log(f"dropin: GPU library: EM {g_t['em_s']:.2f} s, k-means {g_t['kmeans_s']:.2f} s, Fisher {g_t['fisher_s']:.2f} s, "
+    log(f"dropin: GPU library: EM {g_t['em_s']:.2f} s, k-means {g_t['kmeans_s']:.2f} s "
+        f"(Elkan {g_t['kmeans_elkan_s']:.2f} s), Fisher {g_t['fisher_s']:.2f} s, "
         f"quantize {g_t['quantize_s']:.2f} s, VLAD {g_t['vlad_s']:.2f} s")
     # homogeneous kernel map of the encodings, as the recognition application
     # applies it before its linear SVM (traintest.m:99: order 1, chi2, the
@@ -683,7 +692,8 @@ def dropin_config(args) -> int:
                   "fisher_max_rel_l2": float(fv_err.max()), "fisher_median_rel_l2": float(np.median(fv_err)),
                   "assignments_equal": int((g[5] == r[5]).sum()), "assignments_total": int(g[5].size),
                   "vlad_max_rel_l2": float(vl_err.max()),
-                  "kmeans_centers_equal": bool(np.array_equal(g[7], r[7])), "kmeans_energy_equal": g[8] == r[8]}
+                  "kmeans_centers_equal": bool(np.array_equal(g[7], r[7])), "kmeans_energy_equal": g[8] == r[8],
+                  "elkan_centers_equal": bool(np.array_equal(g[9], r[9])), "elkan_energy_equal": g[10] == r[10]}
         log("dropin: parity " + ", ".join(f"{k}={v:.3g}" for k, v in parity.items()))
         result["reference_s"] = r_t
         result["parity_vs_reference"] = parity

@@ -217,6 +217,24 @@ VLHIP_API void vlhip_project (void * out, void const * data, size_t numData, siz
 VLHIP_API void vlhip_homkermap_evaluate (int dtype, void * out, void const * in, size_t n, double const * table,
                                          int order, int numSubdivisions, int minExponent, int maxExponent) ;
 
+/* One step of Elkan's k-means refinement (kmeans.c:1162-1745) over device
+ * arrays, one thread per datum, the reference's operations in its order:
+ *   step 0  initial assignment and bounds (table: K x K center distances);
+ *   step 1  bounds moved by the centers' moves (aux: K move distances);
+ *   step 2  reassignment guided by the bounds (table: K x K center
+ *           distances, aux: each center's distance to its nearest other
+ *           center); *moved (device, cleared by the caller) becomes 1 when a
+ *           datum changes center;
+ *   step 3  distances of every datum to its center (distances, numData).
+ * upperBounds (numData), upperBoundIsExact (numData bytes) and lowerBounds
+ * (numData x numCenters) persist between steps; all distances are squared L2
+ * in the host's lane order (lanes 8, 4, 2 or 1), so the steps reproduce the
+ * reference's bounds, assignments and distances bit for bit. */
+VLHIP_API void vlhip_kmeans_elkan (int dtype, int step, void const * data, void const * centers, void const * table,
+                                   void const * aux, size_t numData, size_t dimension, size_t numCenters, int lanes,
+                                   uint32_t * assignments, void * upperBounds, unsigned char * upperBoundIsExact,
+                                   void * lowerBounds, unsigned * moved, void * distances) ;
+
 /* ---- SVM solvers (vl/svm.c) ----------------------------------------------- */
 
 /* One run of SDCA (svm.c:1931-2016) or SGD (svm.c:2018-2142) iterations

@@ -12,10 +12,12 @@
  *   ANN refinement              kmeans.c:1003-1156 (exhaustive search: as Lloyd plus
  *                               the keep-previous rule on the GPU; otherwise the
  *                               randomised kd-forest of kdtree.c on the host)
+ *   Elkan refinement            kmeans.c:1162-1745 (per-datum bound updates,
+ *                               reassignment and distances on the GPU, the
+ *                               reference's float bounds bit for bit; center
+ *                               update as Lloyd's)
  *   repetitions                 kmeans.c:2023-2093
- * Only the L2 distance is provided on this path.  Elkan refinement returns the
- * Lloyd solution (both are exact Lloyd iterations; Elkan only skips distance
- * evaluations on the CPU).
+ * Only the L2 distance is provided on this path.
  */
 #include "host_fatal.h"
 #include "vl/kmeans.h"
@@ -233,8 +235,12 @@ vl_kmeans_refine_centers (VlKMeans * self, void const * data, vl_size numData)
   assert(self->centers) ;
   require_l2(self, "vl_kmeans_refine_centers") ;
   switch (self->dataType) {
-    case VL_TYPE_FLOAT : return km_refine_f(self, (float const*)data, numData) ;
-    case VL_TYPE_DOUBLE : return km_refine_d(self, (double const*)data, numData) ;
+    case VL_TYPE_FLOAT :
+      return self->algorithm == VlKMeansElkan ? km_refine_elkan_f(self, (float const*)data, numData)
+                                              : km_refine_f(self, (float const*)data, numData) ;
+    case VL_TYPE_DOUBLE :
+      return self->algorithm == VlKMeansElkan ? km_refine_elkan_d(self, (double const*)data, numData)
+                                              : km_refine_d(self, (double const*)data, numData) ;
     default : abort() ;
   }
   return 0 ;

@@ -305,6 +305,171 @@ FN(km_refine) (VlKMeans * self, TYPE const * data, vl_size numData)
   return energy ;
 }
 
+/* Elkan refinement (kmeans.c:1162-1745).  The per-datum steps (initial
+ * assignment, moving the bounds, the bound-guided reassignment, the final
+ * distances) run on the GPU (vlhip_kmeans_elkan); the host does what the
+ * reference does once per iteration for all data: cluster masses, the new
+ * centers' data lists and restart draws (the same center update as Lloyd's,
+ * kmeans.c:1313-1345), each center's move, the center-center distances
+ * (kept in self->centerDistances, as the reference keeps them) and each
+ * center's nearest-center distance, all with the host's lane-order distance.
+ * The loop stops after maxNumIterations iterations or when no datum changed
+ * center; the returned energy is the sum of the final exact distances in data
+ * order. */
+static double
+FN(km_refine_elkan) (VlKMeans * self, TYPE const * data, vl_size numData)
+{
+  vl_size const K = self->numCenters, D = self->dimension, es = sizeof(TYPE) ;
+  vl_size c, j, x, iteration ;
+  int const lanes = host_lanes(self->dataType) ;
+  FN(km_dist_t) distFn = (FN(km_dist_t)) DISTFN(VlDistanceL2) ;
+  VlRand * rand = vl_get_rand() ;
+  vl_uint32 * assignments ;
+  vl_size * clusterMasses, * cursor ;
+  TYPE * oldCenters, * move, * nearest, * cd, * dist ;
+  vl_size memBytes, updBytes ;
+  unsigned char * upd ;
+  vl_uint32 * members ;
+  vl_uint64 * starts, * restartRow ;
+  void * dData, * dCenters, * dTable, * dMove, * dNearest, * dUB, * dLB, * dDist ;
+  unsigned char * dExact, * dUpd ;
+  vl_uint32 * dAssign ;
+  unsigned * dMoved ;
+  double energy = 0 ;
+
+  if (numData == 0) return 0 ;
+  if (numData > 0xffffffffu) {
+    VL_PRINTF("vl_kmeans_refine_centers: more than 2^32 data are not supported\n") ;
+    abort() ;
+  }
+  if (!self->centerDistances) self->centerDistances = vl_malloc(es * K * K) ;
+  cd = (TYPE*) self->centerDistances ;
+  assignments = (vl_uint32*) vl_malloc(sizeof(vl_uint32) * numData) ;
+  clusterMasses = (vl_size*) vl_malloc(sizeof(vl_size) * K) ;
+  cursor = (vl_size*) vl_malloc(sizeof(vl_size) * K) ;
+  oldCenters = (TYPE*) vl_malloc(es * D * K) ;
+  move = (TYPE*) vl_malloc(es * K) ;
+  nearest = (TYPE*) vl_malloc(es * K) ;
+  dist = (TYPE*) vl_malloc(es * numData) ;
+  memBytes = (sizeof(vl_uint32) * numData + 7) & ~(vl_size)7 ;
+  updBytes = memBytes + sizeof(vl_uint64) * (2 * K + 1) ;
+  upd = (unsigned char*) vl_malloc(updBytes) ;
+  members = (vl_uint32*) upd ;
+  starts = (vl_uint64*) (upd + memBytes) ;
+  restartRow = starts + K + 1 ;
+
+  dData = vlhip_malloc(es * D * numData) ;
+  dCenters = vlhip_malloc(es * D * K) ;
+  dTable = vlhip_malloc(es * K * K) ;
+  dMove = vlhip_malloc(es * K) ;
+  dNearest = vlhip_malloc(es * K) ;
+  dUB = vlhip_malloc(es * numData) ;
+  dLB = vlhip_malloc(es * numData * K) ;
+  dDist = vlhip_malloc(es * numData) ;
+  dExact = (unsigned char*) vlhip_malloc(numData) ;
+  dAssign = (vl_uint32*) vlhip_malloc(sizeof(vl_uint32) * numData) ;
+  dMoved = (unsigned*) vlhip_malloc(sizeof(unsigned)) ;
+  dUpd = (unsigned char*) vlhip_malloc(updBytes) ;
+  vlhip_memcpy_h2d(dData, data, es * D * numData) ;
+  vlhip_memcpy_h2d(dCenters, self->centers, es * D * K) ;
+
+#define KM_CENTER_DISTANCES()                                                        \
+  for (j = 0 ; j < K ; ++j) {                                                        \
+    for (c = 0 ; c <= j ; ++c) {                                                     \
+      TYPE z = distFn(D, (TYPE*) self->centers + c * D, (TYPE*) self->centers + j * D) ; \
+      cd[c + j * K] = z ;                                                            \
+      cd[j + c * K] = z ;                                                            \
+    }                                                                                \
+  }
+
+  /* iteration 0: assignment to the initial centers */
+  KM_CENTER_DISTANCES()
+  vlhip_memcpy_h2d(dTable, cd, es * K * K) ;
+  vlhip_kmeans_elkan(TYPE_CODE, 0, dData, dCenters, dTable, NULL, numData, D, K, lanes, dAssign, dUB, dExact, dLB,
+                     NULL, NULL) ;
+  if (self->verbosity) {
+    vlhip_memcpy_d2h(dist, dUB, es * numData) ;
+    for (energy = 0, x = 0 ; x < numData ; ++x) energy += dist[x] ;
+    VL_PRINTF("kmeans: Elkan iteration 0, energy bound %g\n", energy) ;
+  }
+
+  for (iteration = 1 ; ; ++iteration) {
+    unsigned anyMoved = 0 ;
+    vl_size numRestarted = 0 ;
+    /* new centers: sums in data order, divided by the mass; empty clusters
+     * restart at a random datum, drawn in cluster order */
+    vlhip_memcpy_d2h(assignments, dAssign, sizeof(vl_uint32) * numData) ;
+    memset(clusterMasses, 0, sizeof(vl_size) * K) ;
+    for (x = 0 ; x < numData ; ++x) clusterMasses[assignments[x]] ++ ;
+    starts[0] = 0 ;
+    for (c = 0 ; c < K ; ++c) {
+      cursor[c] = starts[c] ;
+      starts[c + 1] = starts[c] + clusterMasses[c] ;
+    }
+    for (x = 0 ; x < numData ; ++x) members[cursor[assignments[x]] ++] = (vl_uint32) x ;
+    for (c = 0 ; c < K ; ++c) {
+      restartRow[c] = 0 ;
+      if (clusterMasses[c] == 0) {
+        restartRow[c] = vl_rand_uindex(rand, numData) ;
+        numRestarted ++ ;
+      }
+    }
+    memcpy(oldCenters, self->centers, es * D * K) ;
+    vlhip_memcpy_h2d(dUpd, upd, updBytes) ;
+    vlhip_kmeans_update_centers(TYPE_CODE, dCenters, dData, (vl_uint32 const*) dUpd,
+                                (uint64_t const*) (dUpd + memBytes), (uint64_t const*) (dUpd + memBytes) + K + 1,
+                                K, D) ;
+    vlhip_memcpy_d2h(self->centers, dCenters, es * D * K) ;
+    /* each center's move, the new center distances and nearest-center distances */
+    for (c = 0 ; c < K ; ++c) move[c] = distFn(D, (TYPE*) self->centers + c * D, oldCenters + c * D) ;
+    KM_CENTER_DISTANCES()
+    for (c = 0 ; c < K ; ++c) {
+      nearest[c] = (TYPE) VL_INFINITY_D ;
+      for (j = 0 ; j < K ; ++j) {
+        if (j == c) continue ;
+        nearest[c] = VL_MIN(nearest[c], cd[j + c * K]) ;
+      }
+    }
+    vlhip_memcpy_h2d(dMove, move, es * K) ;
+    vlhip_memcpy_h2d(dTable, cd, es * K * K) ;
+    vlhip_memcpy_h2d(dNearest, nearest, es * K) ;
+    vlhip_kmeans_elkan(TYPE_CODE, 1, dData, dCenters, NULL, dMove, numData, D, K, lanes, dAssign, dUB, dExact, dLB,
+                       NULL, NULL) ;
+    vlhip_memset(dMoved, 0, sizeof(unsigned)) ;
+    vlhip_kmeans_elkan(TYPE_CODE, 2, dData, dCenters, dTable, dNearest, numData, D, K, lanes, dAssign, dUB, dExact,
+                       dLB, dMoved, NULL) ;
+    vlhip_memcpy_d2h(&anyMoved, dMoved, sizeof(unsigned)) ;
+    if (self->verbosity) {
+      vlhip_memcpy_d2h(dist, dUB, es * numData) ;
+      for (energy = 0, x = 0 ; x < numData ; ++x) energy += dist[x] ;
+      VL_PRINTF("kmeans: Elkan iteration %d, energy bound %g%s\n", (int) iteration, energy,
+                numRestarted ? ", empty centers restarted" : "") ;
+    }
+    if (iteration >= self->maxNumIterations) break ;
+    if (!anyMoved) break ;
+  }
+#undef KM_CENTER_DISTANCES
+
+  vlhip_kmeans_elkan(TYPE_CODE, 3, dData, dCenters, NULL, NULL, numData, D, K, lanes, dAssign, dUB, dExact, dLB,
+                     NULL, dDist) ;
+  vlhip_memcpy_d2h(dist, dDist, es * numData) ;
+  for (energy = 0, x = 0 ; x < numData ; ++x) energy += dist[x] ;
+
+  {
+    void * bufs [] = {dData, dCenters, dTable, dMove, dNearest, dUB, dLB, dDist, dExact, dAssign, dMoved, dUpd} ;
+    for (j = 0 ; j < sizeof(bufs) / sizeof(bufs[0]) ; ++j) vlhip_free(bufs[j]) ;
+  }
+  vl_free(assignments) ;
+  vl_free(clusterMasses) ;
+  vl_free(cursor) ;
+  vl_free(oldCenters) ;
+  vl_free(move) ;
+  vl_free(nearest) ;
+  vl_free(dist) ;
+  vl_free(upd) ;
+  return energy ;
+}
+
 #undef FN
 #undef TYPE
 #undef TYPE_CODE
