@@ -62,9 +62,17 @@ struct MfmaScales {
   float shift;      // logit origin of the fused encoder (mfma_shift_kernel)
 };
 
+// Rows per group of the matrix-core passes and components per staged block
+// of the first pass.
+constexpr int GR = 32;
+constexpr int MB_COMPS = 64;
+
 // ---------------------------------------------------------------------------
 // Tables.  Kp = K rounded up to 64, Dp = D rounded up to 16.
 //   Whi/Wlo [Kp][2Dp]   features [0,Dp) = w2 * 2^-e2, [Dp,2Dp) = w1 * 2^-e1
+//   Wst     [Kp/64][hi, lo][64][2Dp + 8]  the same split weights in the first
+//                       pass's LDS layout (rows padded by 16 bytes of zeros),
+//                       so that a block is one contiguous copy
 //   bias    [Kp]        b_k (FP32), -inf for skipped / padding components
 //   mup, isg, sinv [Kp][Dp]  mu', sigma^-1, sigma^-2 (FP32) for finalisation
 
@@ -139,8 +147,9 @@ __global__ __launch_bounds__(64) void mfma_tables_kernel(const T* __restrict__ m
                                                          const T* __restrict__ priors, int D, int Dp, int K, int Kp,
                                                          const float* __restrict__ g, MfmaScales* __restrict__ scp,
                                                          H* __restrict__ Whi, H* __restrict__ Wlo,
-                                                         float* __restrict__ bias, float* __restrict__ mup,
-                                                         float* __restrict__ isg, float* __restrict__ sinv) {
+                                                         H* __restrict__ Wst, float* __restrict__ bias,
+                                                         float* __restrict__ mup, float* __restrict__ isg,
+                                                         float* __restrict__ sinv) {
   const int k = blockIdx.x, lane = threadIdx.x;
   // the bound does not change here (the scales are stored with the same bits)
   const MfmaScales sc = mfma_scales(scp->bound, std::is_same<H, _Float16>::value);
@@ -149,6 +158,10 @@ __global__ __launch_bounds__(64) void mfma_tables_kernel(const T* __restrict__ m
   const float e1 = 1.0f / sc.s1, e2 = 1.0f / sc.s2;
   H* wh = Whi + (size_t)k * 2 * Dp;
   H* wl = Wlo + (size_t)k * 2 * Dp;
+  const int WS = 2 * Dp + 8;
+  H* sh = Wst + ((size_t)(k / MB_COMPS) * 2 * MB_COMPS + (size_t)(k % MB_COMPS)) * WS;
+  H* sl = sh + (size_t)MB_COMPS * WS;
+  if (lane < 8) sh[2 * Dp + lane] = sl[2 * Dp + lane] = H(0.f);
   const bool real = k < K;
   double logdet = 0.0, quad = 0.0;
   for (int d = lane; d < Dp; d += 64) {
@@ -166,10 +179,11 @@ __global__ __launch_bounds__(64) void mfma_tables_kernel(const T* __restrict__ m
       si = (float)s;
     }
     const H h2 = H(w2), h1 = H(w1);
-    wh[d] = h2;
-    wl[d] = H(w2 - (float)h2);
-    wh[Dp + d] = h1;
-    wl[Dp + d] = H(w1 - (float)h1);
+    const H l2 = H(w2 - (float)h2), l1 = H(w1 - (float)h1);
+    wh[d] = sh[d] = h2;
+    wl[d] = sl[d] = l2;
+    wh[Dp + d] = sh[Dp + d] = h1;
+    wl[Dp + d] = sl[Dp + d] = l1;
     mup[(size_t)k * Dp + d] = mu;
     isg[(size_t)k * Dp + d] = is;
     sinv[(size_t)k * Dp + d] = si;
@@ -228,16 +242,17 @@ __device__ __forceinline__ void make_frag(const float (&xc)[8], bool square, flo
 //           (x' s1 hi, x' s1 lo, x'^2 s2 hi, x'^2 s2 lo), lane l holds feature
 //           16ft + (l & 15) of rows rho(l >> 4, v), v = 0..7, where
 //           rho(lg, v) = 16(v >> 2) + 4 lg + (v & 3);
-//   stats   [3][32]: -max logit, sq / sum_k 2^(logit_k - max) (0 for the
-//           padding rows after an image's last descriptor), argmax (-1 for
-//           padding rows; written only when FAST encodings are requested).
+//   stats   [3][32]: -m, sq / sum_k 2^(logit_k - m) (0 for the padding rows
+//           after an image's last descriptor), argmax (-1 for padding rows;
+//           written only when FAST encodings are requested).  m is the row's
+//           largest logit with FAST encodings, otherwise a reference at most
+//           64 above it (LseLazy); either gives the posteriors
+//           2^(logit - m) * sq / sum.
 // Logits are in log2 units (the weight tables carry the factor log2 e).
 // rho is the order in which a lane holds rows of the logit accumulators
 // (C layout: rows 4(l >> 4) + r of each 16-row tile), so the posteriors become
 // A fragments of the moment contraction without leaving registers; the sum
 // over rows does not depend on the order, only on A and B agreeing on it.
-constexpr int GR = 32;
-constexpr int MB_COMPS = 64;
 // groups (waves) per first-pass workgroup: 8, or 4 when D > 80 (registers)
 template <int NS> struct LseGroups { static constexpr int value = NS <= 5 ? 8 : 4; };
 
@@ -264,19 +279,6 @@ __device__ __forceinline__ double block_sum_waves(double v, double* red) {
   return r;
 }
 
-// Weight staging of the first pass: STG 16-byte words per thread per block.
-template <typename H, int STG>
-__device__ __forceinline__ void w_load(u32x4 (&pre)[STG], const H* Whi, const H* Wlo, int cb, int Dp,
-                                       unsigned loMask, const int (&off)[STG]) {
-#pragma unroll
-  for (int q = 0; q < STG; ++q)
-    pre[q] = *reinterpret_cast<const u32x4*>((((loMask >> q) & 1) ? Wlo : Whi) + (size_t)cb * 2 * Dp + off[q]);
-}
-template <typename H, int STG>
-__device__ __forceinline__ void w_store(const u32x4 (&pre)[STG], H* base, const int (&off)[STG]) {
-#pragma unroll
-  for (int q = 0; q < STG; ++q) *reinterpret_cast<u32x4*>(base + off[q]) = pre[q];
-}
 
 // Group table of a batch: group gstart[i] - gstart[imgBegin] + k of image i
 // starts at batch-relative row offsets[i] - offsets[imgBegin] + 32k, its image
@@ -300,37 +302,132 @@ struct LseState {
   int bi[2][4];
 };
 
-// Logits of 32 rows x 64 components (one block of staged weights, layout
-// [hi, lo][comp][2Dp + 8]): 2 row tiles x 4 component tiles x NS steps x 3
-// products, eight independent accumulators between dependent matrix ops.
-template <typename H, int NS>
-__device__ __forceinline__ void lse_mfma(f32x4 (&acc)[4][2], const typename VecOf<H>::type (&ahi)[2][NS],
-                                         const typename VecOf<H>::type (&alo)[2][NS], const H* w, int lr, int lg) {
+// Logits of 32 rows x 64 components from one block of staged weights (layout
+// [hi, lo][comp][2Dp + 8]): 2 row tiles x 4 component tiles x NS feature
+// steps x 3 products, eight independent accumulators between dependent matrix
+// ops; the first product of each accumulator adds init[c] (the components'
+// biases, or zero).  The B fragments of step j + 1 are read into the
+// registers of step j as soon as its products are done with them (the hi part
+// after 16 products, the lo part after 24), so only the first step of a block
+// waits for LDS.  vwork(j) is vector work interleaved one to one with step j's
+// products; between(j) runs after step j (it may branch).
+template <typename H, int NS, typename VW, typename BW>
+__device__ __forceinline__ void lse_products(f32x4 (&acc)[4][2], const typename VecOf<H>::type (&ahi)[2][NS],
+                                             const typename VecOf<H>::type (&alo)[2][NS], const H* w,
+                                             const f32x4 (&init)[4], int lr, int lg, VW&& vwork, BW&& between) {
   typedef typename VecOf<H>::type V8;
   constexpr int WS = 2 * NS * 16 + 8;
+  const H* wh = w + lr * WS + 8 * lg;
+  const H* wl = wh + MB_COMPS * WS;
+  V8 bh[4], bl[4];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) acc[c][0] = acc[c][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int c = 0; c < 4; ++c) {
+    bh[c] = *reinterpret_cast<const V8*>(wh + 16 * c * WS);
+    bl[c] = *reinterpret_cast<const V8*>(wl + 16 * c * WS);
+  }
 #pragma unroll
   for (int j = 0; j < NS; ++j) {
-    V8 bh[4], bl[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      bh[c] = *reinterpret_cast<const V8*>(w + (16 * c + lr) * WS + 32 * j + 8 * lg);
-      bl[c] = *reinterpret_cast<const V8*>(w + (MB_COMPS + 16 * c + lr) * WS + 32 * j + 8 * lg);
-    }
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) acc[c][t] = mfma16(ahi[t][j], bh[c], acc[c][t]);
+      for (int c = 0; c < 4; ++c) acc[c][t] = mfma16(ahi[t][j], bh[c], j == 0 ? init[c] : acc[c][t]);
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
       for (int c = 0; c < 4; ++c) acc[c][t] = mfma16(alo[t][j], bh[c], acc[c][t]);
+    if (j + 1 < NS) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) bh[c] = *reinterpret_cast<const V8*>(wh + 16 * c * WS + 32 * (j + 1));
+    }
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
       for (int c = 0; c < 4; ++c) acc[c][t] = mfma16(ahi[t][j], bl[c], acc[c][t]);
+    if (j + 1 < NS) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) bl[c] = *reinterpret_cast<const V8*>(wl + 16 * c * WS + 32 * (j + 1));
+    }
+    vwork(j);
+    if (j == 0) __builtin_amdgcn_sched_group_barrier(0x100, 12, 0);  // the block's first fragment and bias reads
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one matrix op
+      __builtin_amdgcn_sched_group_barrier(0x002, 1, 0);  // one vector op
+    }
+    if (j + 1 < NS) __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);  // next step's hi fragments
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x002, 1, 0);
+    }
+    if (j + 1 < NS) __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);  // next step's lo fragments
+    __builtin_amdgcn_sched_barrier(0);
+    between(j);
   }
+}
+
+// Running log-sum-exp of a lane's 8 rows against a lazily moved reference
+// (log2 units): s = sum of 2^(logit - ref) over the components seen so far;
+// ref is the same in the 16 lanes that share a row and never below the row's
+// largest logit, so every term is at most 1 and a block costs one
+// subtraction, one exponential and one addition per logit, plus a check.
+// When a block lifts a row's largest logit above ref, the row's ref moves to
+// that logit + 64 (its term is then 2^-64, far from underflow; terms that
+// underflow are more than 62 binades below it) and s is rescaled.  ref starts
+// at -FLT_MAX, so the first finite logit of a row moves it.  The row
+// statistics are then -ref and sq / sum, in place of -max and sq / sum over
+// 2^(logit - max): the same posteriors 2^(logit - ref) * sq / sum.
+struct LseLazy {
+  float ref[2][4], s[2][4];
+};
+constexpr float kLseHeadroom = 64.f;
+
+__device__ __forceinline__ void lse_lazy_init(LseLazy& st) {
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) { st.ref[t][r] = -3.402823466e38f; st.s[t][r] = 0.f; }
+}
+__device__ __forceinline__ float lse_block_max(const f32x4 (&a)[4][2], int t, int r) {
+  return fmaxf(fmaxf(a[0][t][r], a[1][t][r]), fmaxf(a[2][t][r], a[3][t][r]));
+}
+// True when a block's logits a (bias included) lift a row of this lane above
+// its reference (plain comparisons: a maximum would first quiet its operands).
+__device__ __forceinline__ bool lse_lazy_check(const f32x4 (&a)[4][2], const LseLazy& st) {
+  bool up = false;
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float ref = st.ref[t][r];
+      up = up | (a[0][t][r] > ref) | (a[1][t][r] > ref) | (a[2][t][r] > ref) | (a[3][t][r] > ref);
+    }
+  return up;
+}
+// Moves the reference of every row whose largest logit in block a rose above
+// it; all 64 lanes take part (row maxima across lanes).
+__device__ __forceinline__ void lse_lazy_rescale(const f32x4 (&a)[4][2], LseLazy& st) {
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float rm = row16_max(lse_block_max(a, t, r));
+      const float nr = rm > st.ref[t][r] ? rm + kLseHeadroom : st.ref[t][r];
+      st.s[t][r] *= __builtin_amdgcn_exp2f(st.ref[t][r] - nr);
+      st.ref[t][r] = nr;
+    }
+}
+// The terms of row (t, r) of a block.
+__device__ __forceinline__ void lse_lazy_row(const f32x4 (&a)[4][2], int t, int r, LseLazy& st) {
+  const float ref = st.ref[t][r];
+  st.s[t][r] += (__builtin_amdgcn_exp2f(a[0][t][r] - ref) + __builtin_amdgcn_exp2f(a[1][t][r] - ref)) +
+                (__builtin_amdgcn_exp2f(a[2][t][r] - ref) + __builtin_amdgcn_exp2f(a[3][t][r] - ref));
+}
+// Whole update with a block's logits (the last block of a sweep).
+__device__ __forceinline__ void lse_lazy_update(const f32x4 (&a)[4][2], LseLazy& st) {
+  if (__ballot(lse_lazy_check(a, st)) != 0) lse_lazy_rescale(a, st);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) lse_lazy_row(a, q >> 2, q & 3, st);
 }
 
 // Online log-sum-exp update with the logits of components cb .. cb + 63
@@ -382,70 +479,21 @@ __device__ __forceinline__ void lse_row(const f32x4 (&acc)[4][2], const float (&
   st.m[t][r] = mn;
 }
 
-// One pipelined step: the products of a block into accNew and the
-// log-sum-exp update with the previous block's accOld, cut into NS chunks
-// (one feature step of products plus a share of the 8 row updates each) whose
-// matrix and vector instructions are interleaved one to two.
-template <typename H, int NS, bool ARG>
-__device__ __forceinline__ void lse_fused(f32x4 (&accNew)[4][2], const f32x4 (&accOld)[4][2],
-                                          const typename VecOf<H>::type (&ahi)[2][NS],
-                                          const typename VecOf<H>::type (&alo)[2][NS], const H* w, int lr, int lg,
-                                          const float* bblkOld, int cbOld, LseState& st) {
-  typedef typename VecOf<H>::type V8;
-  constexpr int WS = 2 * NS * 16 + 8;
-  float bc[4];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) bc[c] = bblkOld[16 * c + lr];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) accNew[c][0] = accNew[c][1] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int j = 0; j < NS; ++j) {
-    V8 bh[4], bl[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      bh[c] = *reinterpret_cast<const V8*>(w + (16 * c + lr) * WS + 32 * j + 8 * lg);
-      bl[c] = *reinterpret_cast<const V8*>(w + (MB_COMPS + 16 * c + lr) * WS + 32 * j + 8 * lg);
-    }
-#pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) accNew[c][t] = mfma16(ahi[t][j], bh[c], accNew[c][t]);
-#pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) accNew[c][t] = mfma16(alo[t][j], bh[c], accNew[c][t]);
-#pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) accNew[c][t] = mfma16(ahi[t][j], bl[c], accNew[c][t]);
-    // rows [8j/NS, 8(j+1)/NS) of the previous block
-#pragma unroll
-    for (int q = (8 * j) / NS; q < (8 * (j + 1)) / NS; ++q) lse_row<ARG>(accOld, bc, q >> 2, q & 3, cbOld, lr, st);
-    __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);  // the step's 8 LDS fragment reads
-    __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);  // vector work under their latency
-#pragma unroll
-    for (int i = 0; i < 24; ++i) {
-      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one matrix op
-      __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // two vector ops
-    }
-    __builtin_amdgcn_sched_barrier(0);
-  }
-}
-
 // ---------------------------------------------------------------------------
 // First pass.  Workgroup: 8 waves (4 when D > 80), wave = one group of 32 rows.  The wave
 // builds its A fragments from the fp32 descriptors, stores them and the B
 // fragments in the group record, then sweeps all components: the split
 // weights of 64 components at a time are double-buffered in LDS (the next
-// block is fetched into registers while the current one is multiplied), and
+// block is copied straight into LDS while the current one is multiplied), and
 // every wave runs 2 row tiles x 4 component tiles x NS steps x 3 products per
-// block, keeping a lane-local online log-sum-exp (and, with ARG, the first
-// index of the maximum) merged across the 16 lanes that share rows at the end.
-// The log-sum-exp update of a block is issued with the products of the next.
+// block.  Without FAST encodings the accumulators start at the components'
+// biases and each row keeps a log-sum-exp against a lazily moved reference
+// (LseLazy); with them, a lane-local online log-sum-exp and the first index
+// of the maximum.  Both are merged across the 16 lanes that share rows at the
+// end.  The update with a block is issued with the products of the next.
 template <typename H, int NS, bool ARG>
 __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
-    const float* __restrict__ x, int D, int Kp,
-    const H* __restrict__ Whi, const H* __restrict__ Wlo, const float* __restrict__ bias,
+    const float* __restrict__ x, int D, int Kp, const H* __restrict__ Wst, const float* __restrict__ bias,
     const MfmaScales* __restrict__ scp, const float* __restrict__ g,
     unsigned char* __restrict__ rec, long long numGroups, const int4* __restrict__ gtab,
     const unsigned char* __restrict__ gflag) {
@@ -455,16 +503,11 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
   constexpr int WS = 2 * Dp + 8;                 // LDS row stride (halves) of the staged weights
   constexpr int LG = LseGroups<NS>::value;
   constexpr int NT = LG * 64;
-  constexpr int PART = MB_COMPS * (2 * Dp / 8);  // 16-byte words of one weight part per block
-  constexpr int STG = 2 * PART / NT;             // words per thread per block (= NS)
-  static_assert(STG * NT == 2 * PART, "staging must divide evenly");
   __shared__ __align__(16) H wsh[2][2][MB_COMPS][WS];  // [buffer][hi, lo][comp][feature]
-  // Biases of the staged blocks (block B in slot B & 3), fetched and stored
-  // with the block's weights: the log-sum-exp update of block b - 1 reads them
-  // from LDS while block b + 1 is being stored, so no global load is waited
-  // for inside the sweep (a global load issued after the prefetch of block
-  // b + 2 would make the wave wait for that prefetch as well).
-  __shared__ float bsh[4][MB_COMPS];
+  // Biases of the staged blocks (block B in slot B & 3), four copies each,
+  // loaded with the block's copy and stored at the end of the step before it,
+  // so that no global load is waited for inside the sweep.
+  __shared__ __align__(16) f32x4 bsh[4][MB_COMPS];
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int lr = lane & 15, lg = lane >> 4;
@@ -482,21 +525,24 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
   V8* Bp = reinterpret_cast<V8*>(rec + (size_t)grp * R::kBytes + R::kA);
   float* Sf = reinterpret_cast<float*>(rec + (size_t)grp * R::kBytes + R::kA + R::kB);
 
-  // staging map of this thread (the same for every block)
-  int srcOff[STG], dstOff[STG];
-  unsigned srcLo = 0;  // bit q: word q comes from the lo part
+  // Weight staging: a block of Wst (its LDS layout [hi, lo][comp][2Dp + 8])
+  // is copied in 1 KiB pieces, wave w taking pieces w, w + LG, ... and lane l
+  // bytes 16l .. 16l + 15 of each.  Blocks after the first are copied straight
+  // into LDS (global_load_lds, 16 bytes per lane) a whole block ahead, through
+  // no registers.
+  constexpr size_t BLOCK_BYTES = sizeof(H) * 2 * MB_COMPS * WS;
+  constexpr int PIECES = (int)(BLOCK_BYTES / 1024);
+  constexpr int PW = (PIECES + LG - 1) / LG;       // pieces per wave, at most
+  static_assert(PIECES * 1024 == BLOCK_BYTES, "pieces must tile a block");
+  const unsigned voff = 1024u * (unsigned)w + 16u * (unsigned)lane;
+  // block 0 goes through registers, loaded now and stored once the staging
+  // area below is free
+  u32x4 w0[PW];
 #pragma unroll
-  for (int q = 0; q < STG; ++q) {
-    const int i = tid + NT * q;
-    const int part = i / PART, rem = i - part * PART;
-    const int c = rem / (2 * Dp / 8), q8 = rem - c * (2 * Dp / 8);
-    srcLo |= (part != 0 ? 1u : 0u) << q;
-    srcOff[q] = c * 2 * Dp + 8 * q8;
-    dstOff[q] = (part * MB_COMPS + c) * WS + 8 * q8;
-  }
-  u32x4 wpre[STG];
-  w_load<H, STG>(wpre, Whi, Wlo, 0, Dp, srcLo, srcOff);
-  float bpre = tid < MB_COMPS ? bias[tid] : 0.f;
+  for (int k = 0; k < PW; ++k)
+    if (w + LG * k < PIECES)
+      w0[k] = *reinterpret_cast<const u32x4*>(reinterpret_cast<const char*>(Wst) + voff + 1024u * LG * k);
+  float bnext = tid < MB_COMPS ? bias[tid] : 0.f;
 
   // Stage the wave's 32 centred descriptors x' = x - g (zero past D and past
   // the image) in LDS with 16-byte loads; the weight buffers are free until
@@ -600,12 +646,6 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
   }
   __syncthreads();  // the staging area becomes the weight buffers
 
-  LseState st;
-#pragma unroll
-  for (int t = 0; t < 2; ++t)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) { st.m[t][r] = -INFINITY; st.s[t][r] = 0.f; st.bv[t][r] = -INFINITY; st.bi[t][r] = 0; }
-
   // Software pipeline over the component blocks: the matrix products of block
   // b are issued together with the log-sum-exp update of block b - 1, so the
   // vector work overlaps the matrix cores; one barrier per block.
@@ -614,70 +654,181 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
   const bool need = live && !(gflag != nullptr && gflag[grp] != 0);
   if (!__syncthreads_or(need)) return;
   const int nb = Kp / MB_COMPS;
-  f32x4 accA[4][2], accB[4][2];
-#define LSE_STAGE(B)                                                                                  \
-  {                                                                                                   \
-    w_store<H, STG>(wpre, &wsh[(B) & 1][0][0][0], dstOff);                                            \
-    if (tid < MB_COMPS) bsh[(B) & 3][tid] = bpre;                                                     \
-  }
-#define LSE_FETCH(B)                                                                                  \
-  {                                                                                                   \
-    w_load<H, STG>(wpre, Whi, Wlo, (B) * MB_COMPS, Dp, srcLo, srcOff);                                \
-    if (tid < MB_COMPS) bpre = bias[(B) * MB_COMPS + tid];                                            \
-  }
-  LSE_STAGE(0)
-  if (nb > 1) LSE_FETCH(1)
-  __syncthreads();
-  if (need) lse_mfma<H, NS>(accA, ahi, alo, &wsh[0][0][0][0], lr, lg);
+  auto buf = [&](int b) __attribute__((always_inline)) { return &wsh[b & 1][0][0][0]; };
+  // copy of block b into its buffer (every wave takes part, needed or not)
+  auto copy_block = [&](int b) __attribute__((always_inline)) {
+    const char* src = reinterpret_cast<const char*>(Wst) + (size_t)b * BLOCK_BYTES;
+    unsigned char* dst = reinterpret_cast<unsigned char*>(buf(b));
+#pragma unroll
+    for (int k = 0; k < PW; ++k)
+      if (w + LG * k < PIECES)
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + voff + 1024u * LG * k),
+                                         (__attribute__((address_space(3))) void*)(dst + 1024 * (w + LG * k)), 16, 0,
+                                         0);
+  };
+  // biases of block b, four copies each (an accumulator's initial value)
+  auto store_bias = [&](int b) __attribute__((always_inline)) {
+    if (tid < MB_COMPS) bsh[b & 3][tid] = f32x4{bnext, bnext, bnext, bnext};
+  };
+#pragma unroll
+  for (int k = 0; k < PW; ++k)
+    if (w + LG * k < PIECES) reinterpret_cast<u32x4*>(buf(0))[64 * (w + LG * k) + lane] = w0[k];
+  store_bias(0);
   if (nb > 1) {
-    LSE_STAGE(1)
-    if (nb > 2) LSE_FETCH(2)
+    copy_block(1);
+    if (tid < MB_COMPS) bnext = bias[MB_COMPS + tid];
   }
   __syncthreads();
-  // step b: stage block b + 1 (its buffer held block b - 1, read before the
-  // last barrier), fetch block b + 2, multiply block b, update with block b - 1
-#define LSE_STEP(ACC_NEW, ACC_OLD, B)                                                                 \
-  {                                                                                                   \
-    const int b_ = (B);                                                                               \
-    if (b_ + 1 < nb) {                                                                                \
-      LSE_STAGE(b_ + 1)                                                                               \
-      if (b_ + 2 < nb) LSE_FETCH(b_ + 2)                                                              \
-    }                                                                                                 \
-    if (need)                                                                                         \
-      lse_fused<H, NS, ARG>(ACC_NEW, ACC_OLD, ahi, alo, &wsh[b_ & 1][0][0][0], lr, lg,                \
-                            &bsh[(b_ - 1) & 3][0], (b_ - 1) * MB_COMPS, st);                          \
-    __syncthreads();                                                                                  \
-  }
-  int b = 1;
-  for (; b + 1 < nb; b += 2) {
-    LSE_STEP(accB, accA, b)
-    LSE_STEP(accA, accB, b + 1)
-  }
-  if (b < nb) {
-    LSE_STEP(accB, accA, b)
-    if (need) lse_update<ARG>(accB, &bsh[(nb - 1) & 3][0], (nb - 1) * MB_COMPS, lr, st);
-  } else {
-    if (need) lse_update<ARG>(accA, &bsh[(nb - 1) & 3][0], (nb - 1) * MB_COMPS, lr, st);
-  }
-#undef LSE_STEP
-#undef LSE_STAGE
-#undef LSE_FETCH
+  f32x4 accA[4][2], accB[4][2];
+  const f32x4 zero4[4] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f},
+                          f32x4{0.f, 0.f, 0.f, 0.f}};
+  auto no_work = [](int) __attribute__((always_inline)) {};
+  // block b's initial accumulators: its biases (or zero with ARG, whose
+  // update adds them, so that its first maximum compares the same values as
+  // before)
+  auto init_of = [&](int b, f32x4 (&bq)[4]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) bq[c] = ARG ? zero4[c] : bsh[b & 3][16 * c + lr];
+  };
+  // end of step b: block b + 1 and its biases are in place for every wave
+  auto finish_step = [&](int b) __attribute__((always_inline)) {
+    if (b + 1 < nb) store_bias(b + 1);
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed
+    __syncthreads();
+  };
 
-  // merge the 16 lanes that hold the same rows (lanes lg*16 .. lg*16+15)
   const float sq = scp->sq;
+  if constexpr (!ARG) {
+    LseLazy st;
+    lse_lazy_init(st);
+    // the update of block b - 1 with the products of block b: row maxima and
+    // the reference check with step 0, the branch (rare) after it, then the
+    // terms of the 8 rows spread over steps 1 .. NS - 1
+    auto step = [&](f32x4 (&accNew)[4][2], f32x4 (&accOld)[4][2], int b) __attribute__((always_inline)) {
+      if (b + 1 < nb) {
+        copy_block(b + 1);
+        if (tid < MB_COMPS) bnext = bias[(b + 1) * MB_COMPS + tid];
+      }
+      if (need) {
+        f32x4 bq[4];
+        init_of(b, bq);
+        bool up = false;
+        lse_products<H, NS>(
+            accNew, ahi, alo, buf(b), bq, lr, lg,
+            [&](int j) __attribute__((always_inline)) {
+              if (j == 0) {
+                up = lse_lazy_check(accOld, st);
+              } else {
 #pragma unroll
-  for (int t = 0; t < 2; ++t)
+                for (int q = (8 * (j - 1)) / (NS - 1 > 0 ? NS - 1 : 1); q < (8 * j) / (NS - 1 > 0 ? NS - 1 : 1); ++q)
+                  lse_lazy_row(accOld, q >> 2, q & 3, st);
+              }
+            },
+            [&](int j) __attribute__((always_inline)) {
+              if (j == 0 && __ballot(up) != 0) lse_lazy_rescale(accOld, st);
+            });
+        if (NS == 1) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      float m = st.m[t][r];
+          for (int q = 0; q < 8; ++q) lse_lazy_row(accOld, q >> 2, q & 3, st);
+        }
+      }
+      finish_step(b);
+    };
+    if (need) {
+      f32x4 bq[4];
+      init_of(0, bq);
+      lse_products<H, NS>(accA, ahi, alo, buf(0), bq, lr, lg, no_work, no_work);
+    }
+    finish_step(0);
+    int b = 1;
+    for (; b + 1 < nb; b += 2) {
+      step(accB, accA, b);
+      step(accA, accB, b + 1);
+    }
+    if (b < nb) {
+      step(accB, accA, b);
+      if (need) lse_lazy_update(accB, st);
+    } else {
+      if (need) lse_lazy_update(accA, st);
+    }
+    // sums over the 16 lanes that hold the same rows (one reference per row)
 #pragma unroll
-      for (int o = 1; o < 16; o <<= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-      const float ms = (m == -INFINITY) ? 0.f : m;
-      float s = st.s[t][r] * __builtin_amdgcn_exp2f(st.m[t][r] - ms);
+    for (int t = 0; t < 2; ++t)
 #pragma unroll
-      for (int o = 1; o < 16; o <<= 1) s += __shfl_xor(s, o, 64);
-      int bi = st.bi[t][r];
-      if (ARG) {
+      for (int r = 0; r < 4; ++r) {
+        const float sum = row16_sum(st.s[t][r]);
+        const int row = 16 * t + 4 * lg + r;
+        if (need && lr == 0) {
+          const bool pad = row0 + row >= rowEnd;
+          Sf[row] = pad ? 0.f : -st.ref[t][r];
+          Sf[GR + row] = pad ? 0.f : sq / sum;
+        }
+      }
+  } else {
+    LseState st;
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) { st.m[t][r] = -INFINITY; st.s[t][r] = 0.f; st.bv[t][r] = -INFINITY; st.bi[t][r] = 0; }
+    auto bias_of = [&](int b, float (&bc)[4]) __attribute__((always_inline)) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) bc[c] = bsh[b & 3][16 * c + lr][0];
+    };
+    // the update of block b - 1 (the online log-sum-exp and first maximum),
+    // its 8 rows spread over the NS steps of block b's products
+    auto step = [&](f32x4 (&accNew)[4][2], f32x4 (&accOld)[4][2], int b) __attribute__((always_inline)) {
+      if (b + 1 < nb) {
+        copy_block(b + 1);
+        if (tid < MB_COMPS) bnext = bias[(b + 1) * MB_COMPS + tid];
+      }
+      if (need) {
+        float bc[4];
+        bias_of(b - 1, bc);
+        lse_products<H, NS>(
+            accNew, ahi, alo, buf(b), zero4, lr, lg,
+            [&](int j) __attribute__((always_inline)) {
+#pragma unroll
+              for (int q = (8 * j) / NS; q < (8 * (j + 1)) / NS; ++q)
+                lse_row<true>(accOld, bc, q >> 2, q & 3, (b - 1) * MB_COMPS, lr, st);
+            },
+            no_work);
+      }
+      finish_step(b);
+    };
+    if (need) lse_products<H, NS>(accA, ahi, alo, buf(0), zero4, lr, lg, no_work, no_work);
+    finish_step(0);
+    int b = 1;
+    for (; b + 1 < nb; b += 2) {
+      step(accB, accA, b);
+      step(accA, accB, b + 1);
+    }
+    float bc[4];
+    bias_of(nb - 1, bc);
+    if (b < nb) {
+      step(accB, accA, b);
+      if (need) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) lse_row<true>(accB, bc, q >> 2, q & 3, (nb - 1) * MB_COMPS, lr, st);
+      }
+    } else {
+      if (need) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) lse_row<true>(accA, bc, q >> 2, q & 3, (nb - 1) * MB_COMPS, lr, st);
+      }
+    }
+    // merge the 16 lanes that hold the same rows (lanes lg*16 .. lg*16+15)
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float m = st.m[t][r];
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        const float ms = (m == -INFINITY) ? 0.f : m;
+        float s = st.s[t][r] * __builtin_amdgcn_exp2f(st.m[t][r] - ms);
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) s += __shfl_xor(s, o, 64);
+        int bi = st.bi[t][r];
         float bv = st.bv[t][r];
 #pragma unroll
         for (int o = 1; o < 16; o <<= 1) {
@@ -685,22 +836,22 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
           const int oi = __shfl_xor(bi, o, 64);
           if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
         }
+        const int row = 16 * t + 4 * lg + r;
+        if (need && lr == 0) {
+          const bool pad = row0 + row >= rowEnd;
+          Sf[row] = pad ? 0.f : -m;
+          Sf[GR + row] = pad ? 0.f : sq / s;
+          Sf[2 * GR + row] = __int_as_float(pad ? -1 : bi);
+        }
       }
-      const int row = 16 * t + 4 * lg + r;
-      if (need && lr == 0) {
-        const bool pad = row0 + row >= rowEnd;
-        Sf[row] = pad ? 0.f : -m;
-        Sf[GR + row] = pad ? 0.f : sq / s;
-        if (ARG) Sf[2 * GR + row] = __int_as_float(pad ? -1 : bi);
-      }
-    }
+  }
 }
 
 // Gated groups.  One wave per group, after the first pass has written every
-// group's exact row statistics (-max logit, sq / sum over all K components).
+// group's exact row statistics (-m, sq / sum over all K components).
 // For each row the exact logits of its M <= 16 candidate components (the same
 // FP32 tables as the dense path: bias + log2 e * sum_d sigma^-2 (mu' x' - x'^2 / 2))
-// give exact posteriors q = 2^(logit - max) / sum -- normalised over all
+// give exact posteriors q = 2^(logit - m) / sum -- normalised over all
 // components by the first pass, not over the candidates -- and their mass.
 // The group is served sparsely only if every row passed the network's
 // pre-screen (conf) and the candidates' exact mass of every row is >= need;
@@ -922,7 +1073,7 @@ __device__ __forceinline__ void fisher_slice_finalize(
 // walks the image's groups (records double-buffered in LDS, the next one
 // prefetched into registers):
 //   logits   2 row tiles x CT component tiles x NS steps x 3 products;
-//   q        2^(logit - max) / sum, FAST one-hot, < 1e-6 dropped
+//   q        2^(logit - m) / sum, FAST one-hot, < 1e-6 dropped
 //            (fisher.c:436-441), rows outside the image masked, rounded to
 //            16 bits (scaled) -- the same rounded value enters S0;
 //   moments  q (A, straight from the accumulators) x the 4 planes of B:
@@ -1048,7 +1199,7 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
             aq[c][4 * t + r] = H(tile[(16 * t + 4 * lg + r) * BC + 16 * CT * w + 16 * c + lr]);
     } else {
       const float* Sf = reinterpret_cast<const float*>(cur + R::kA + R::kB);
-      // rows 16t + 4lg + r of this lane: -max, sq / sum
+      // rows 16t + 4lg + r of this lane: -m, sq / sum
       f32x4 nm4[2], rs4[2];
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
@@ -1056,7 +1207,7 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
         rs4[t] = *reinterpret_cast<const f32x4*>(&Sf[GR + 16 * t + 4 * lg]);
       }
 
-      // ---- logits - max (the accumulators start at bias - max)
+      // ---- logits - m (the accumulators start at bias - m)
       f32x4 acc[2][CT];
 #pragma unroll
       for (int t = 0; t < 2; ++t)
@@ -1789,12 +1940,12 @@ constexpr size_t kRecordBudget = size_t(8) << 30;
 constexpr size_t kGateProbeRows = 65536;
 
 // Device tables of one encode, carved from one buffer:
-// [scales][g Dp][bias Kp][mup, isg, sinv Kp*Dp][Whi, Wlo Kp*2Dp].
+// [scales][g Dp][bias Kp][mup, isg, sinv Kp*Dp][Whi, Wlo Kp*2Dp][Wst Kp*2(2Dp+8)].
 template <typename H>
 struct MfmaTables {
   MfmaScales* scp;
   float *g, *bias, *mup, *isg, *sinv;
-  H *Whi, *Wlo;
+  H *Whi, *Wlo, *Wst;
 };
 
 static inline size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
@@ -1802,7 +1953,8 @@ static inline size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
 template <typename H>
 static size_t mfma_tables_bytes(int Dp, int Kp) {
   return al256(sizeof(MfmaScales)) + al256(sizeof(float) * Dp) + al256(sizeof(float) * Kp) +
-         3 * al256(sizeof(float) * (size_t)Kp * Dp) + 2 * al256(sizeof(H) * (size_t)Kp * 2 * Dp);
+         3 * al256(sizeof(float) * (size_t)Kp * Dp) + 2 * al256(sizeof(H) * (size_t)Kp * 2 * Dp) +
+         al256(sizeof(H) * (size_t)Kp * 2 * (2 * Dp + 8));
 }
 
 template <typename H>
@@ -1818,6 +1970,7 @@ static MfmaTables<H> mfma_tables_carve(unsigned char* base, int Dp, int Kp) {
   t.sinv = t.isg + bT / sizeof(float);
   t.Whi = (H*)(base + bS + bG + bB + 3 * bT);
   t.Wlo = (H*)(base + bS + bG + bB + 3 * bT + bW);
+  t.Wst = (H*)(base + bS + bG + bB + 3 * bT + 2 * bW);
   return t;
 }
 
@@ -1839,7 +1992,7 @@ static void mfma_tables_launch(const MfmaTables<H>& t, const float* means, const
   VLHIP_LAUNCH("mfma_bound", (mfma_bound_kernel<float>), dim3((unsigned)(((size_t)K * D + 255) / 256)), dim3(256), 0,
                means, covs, D, K, t.g, t.scp);
   VLHIP_LAUNCH("mfma_tables", (mfma_tables_kernel<float, H>), dim3(Kp), dim3(64), 0, means, covs, priors, D, Dp, K,
-               Kp, t.g, t.scp, t.Whi, t.Wlo, t.bias, t.mup, t.isg, t.sinv);
+               Kp, t.g, t.scp, t.Whi, t.Wlo, t.Wst, t.bias, t.mup, t.isg, t.sinv);
 }
 
 // The two passes over one batch of images [b0, b0 + n) whose descriptors start
@@ -1851,10 +2004,10 @@ static void mfma_first_pass(const MfmaTables<H>& t, const float* xb, int D, int 
   constexpr int LG = LseGroups<NS>::value;
   const unsigned lb = (unsigned)((nG + LG - 1) / LG);
   if (flags & 4) {
-    VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, true>), dim3(lb), dim3(LG * 64), 0, xb, D, Kp, t.Whi, t.Wlo,
+    VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, true>), dim3(lb), dim3(LG * 64), 0, xb, D, Kp, t.Wst,
                  t.bias, t.scp, t.g, rec, (long long)nG, gtab, (const unsigned char*)nullptr);
   } else {
-    VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, false>), dim3(lb), dim3(LG * 64), 0, xb, D, Kp, t.Whi, t.Wlo,
+    VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, false>), dim3(lb), dim3(LG * 64), 0, xb, D, Kp, t.Wst,
                  t.bias, t.scp, t.g, rec, (long long)nG, gtab, (const unsigned char*)nullptr);
   }
 }
@@ -2961,7 +3114,7 @@ static double run_em_step(double* out_host, const float* means, const float* cov
                  gstart, (int)b0, n, gtab);
     constexpr int LG = LseGroups<NS>::value;
     VLHIP_LAUNCH("mfma_lse", (mfma_lse_kernel<H, NS, false>), dim3((unsigned)((nG + LG - 1) / LG)), dim3(LG * 64), 0,
-                 x + rowBase * D, (int)D, Kp, t.Whi, t.Wlo, t.bias, t.scp, t.g, rec, (long long)nG, gtab,
+                 x + rowBase * D, (int)D, Kp, t.Wst, t.bias, t.scp, t.g, rec, (long long)nG, gtab,
                  (const unsigned char*)nullptr);
     VLHIP_LAUNCH("mfma_em_ll", mfma_em_ll_kernel<NS>, dim3(kLLBlocks), dim3(256), 0, rec, (long long)nG, t.scp, llp);
     if (out_host) {
