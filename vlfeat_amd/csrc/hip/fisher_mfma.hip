@@ -1325,6 +1325,278 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Pipelined second pass (Fisher vectors without the gate).  The arithmetic of
+// mfma_fisher_kernel, group by group in the same order, so the encodings are
+// the same bit for bit; but iteration i of a wave issues the logit products of
+// group i and then the moment products of group i - 1, with the posteriors of
+// group i (exponentials, threshold, rounding) computed under those moment
+// products, so that the vector work of one group overlaps the matrix work of
+// its neighbour instead of alternating with it.  Group records are copied
+// straight into LDS (global_load_lds, 16 bytes per lane, 1 KiB per wave-wide
+// copy; the last copy of a record repeats its last word into the buffer's
+// padding) into three buffers: group i - 1 (moments), group i (logits) and the
+// copy of group i + 1 in flight.  One barrier per group.
+template <int NS>
+constexpr size_t pipe_buf_bytes() {
+  return (GroupRec<NS>::kBytes + 1023) / 1024 * 1024;
+}
+// three buffers fit the 160 KiB of LDS up to D = 96
+template <int NS>
+constexpr bool pipe_fits() {
+  return 3 * pipe_buf_bytes<NS>() <= 160 * 1024;
+}
+
+template <typename T, typename H, int NS, int CT, int WAVES, bool FAST>
+__global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_pipe_kernel(
+    const unsigned char* __restrict__ rec, const uint64_t* __restrict__ offsets, const long long* __restrict__ gstart,
+    int imgBegin, int numImages, int D, int K, const H* __restrict__ Whi, const H* __restrict__ Wlo,
+    const float* __restrict__ bias, const MfmaScales* __restrict__ scp, const float* __restrict__ mup,
+    const float* __restrict__ isg, const float* __restrict__ sinv, const T* __restrict__ priors, int flags,
+    T* __restrict__ enc, double* __restrict__ normpart, unsigned long long* __restrict__ numTerms, int numKB,
+    int sched) {
+  typedef typename VecOf<H>::type V8;
+  typedef GroupRec<NS> R;
+  constexpr int BC = WAVES * 16 * CT;
+  constexpr int NV = (int)(R::kBytes / 16);   // 16-byte words per record
+  constexpr int PIECES = (NV + 63) / 64;      // 1 KiB copies per record
+  constexpr int PW = (PIECES + WAVES - 1) / WAVES;
+  constexpr size_t RS = pipe_buf_bytes<NS>();  // LDS buffer stride
+  static_assert(RS == (size_t)PIECES * 1024, "buffers hold whole copies");
+  extern __shared__ __align__(16) unsigned char smem_raw[];  // 3 buffers
+  __shared__ double red[WAVES];
+
+  int li, kb;
+  fisher_mfma_coords(numKB, numImages, sched, li, kb);
+  if (li >= numImages) return;
+  const int img = imgBegin + li;
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int lr = lane & 15, lg = lane >> 4;
+  const int kw = kb * BC + 16 * CT * w;  // first component of this wave
+  const long long nImg = (long long)offsets[img + 1] - (long long)offsets[img];
+  const long long gBase = gstart[imgBegin];
+  const long long gB = gstart[img] - gBase;  // this image's records
+  const int n = (int)(gstart[img + 1] - gstart[img]);
+  const MfmaScales sc = *scp;
+
+  // copy of group gB + i into buffer i % 3 (every wave takes part)
+  auto copy_rec = [&](int i) __attribute__((always_inline)) {
+    const unsigned char* src = rec + (size_t)(gB + i) * R::kBytes;
+    unsigned char* dst = smem_raw + (size_t)(i % 3) * RS;
+#pragma unroll
+    for (int k = 0; k < PW; ++k) {
+      const int q = w + WAVES * k;
+      if (q < PIECES) {
+        const int word = 64 * q + lane < NV ? 64 * q + lane : NV - 1;
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + 16 * (size_t)word),
+                                         (__attribute__((address_space(3))) void*)(dst + 1024 * q), 16, 0, 0);
+      }
+    }
+  };
+  if (n > 0) copy_rec(0);
+  if (n > 1) copy_rec(1);
+
+  V8 whi[CT][NS], wlo[CT][NS];
+  float bc[CT];
+#pragma unroll
+  for (int c = 0; c < CT; ++c) {
+    const int comp = kw + 16 * c + lr;
+    bc[c] = bias[comp];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      whi[c][j] = *reinterpret_cast<const V8*>(Whi + (size_t)comp * 2 * NS * 16 + 32 * j + 8 * lg);
+      wlo[c][j] = *reinterpret_cast<const V8*>(Wlo + (size_t)comp * 2 * NS * 16 + 32 * j + 8 * lg);
+    }
+  }
+  f32x4 macc[CT][2][NS];  // [comp tile][S1, S2][feature tile]
+#pragma unroll
+  for (int c = 0; c < CT; ++c)
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int ft = 0; ft < NS; ++ft) macc[c][s][ft] = f32x4{0.f, 0.f, 0.f, 0.f};
+  f32x4 macc0[CT];  // posterior mass S0 (every column)
+#pragma unroll
+  for (int c = 0; c < CT; ++c) macc0[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  V8 ones;
+#pragma unroll
+  for (int v = 0; v < 8; ++v) ones[v] = H(1.f);
+  unsigned terms = 0;  // kept (row, component) pairs of this wave, counted from lane masks
+  // (double)q < 1e-6 for a float q is q < the next float above float(1e-6),
+  // compared after the exact power-of-two scaling by sq
+  const float qminS = __uint_as_float(0x358637beu) * sc.sq;
+
+  // logits - m of the group in buffer b (the accumulators start at bias - m)
+  auto logits = [&](const unsigned char* b, f32x4 (&acc)[2][CT]) __attribute__((always_inline)) {
+    const V8* Ap = reinterpret_cast<const V8*>(b);
+    const float* Sf = reinterpret_cast<const float*>(b + R::kA + R::kB);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const f32x4 nm = *reinterpret_cast<const f32x4*>(&Sf[16 * t + 4 * lg]);
+#pragma unroll
+      for (int c = 0; c < CT; ++c) acc[t][c] = f32x4{bc[c] + nm[0], bc[c] + nm[1], bc[c] + nm[2], bc[c] + nm[3]};
+    }
+    // the A fragments of step j + 1 are read while step j's products run
+    V8 ah[2][2], al[2][2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      ah[0][t] = Ap[((t * NS + 0) * 2 + 0) * 64 + lane];
+      al[0][t] = Ap[((t * NS + 0) * 2 + 1) * 64 + lane];
+    }
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      const int u = j & 1;
+      if (j + 1 < NS) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          ah[u ^ 1][t] = Ap[((t * NS + j + 1) * 2 + 0) * 64 + lane];
+          al[u ^ 1][t] = Ap[((t * NS + j + 1) * 2 + 1) * 64 + lane];
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int c = 0; c < CT; ++c) acc[t][c] = mfma16(ah[u][t], whi[c][j], acc[t][c]);
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int c = 0; c < CT; ++c) acc[t][c] = mfma16(al[u][t], whi[c][j], acc[t][c]);
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int c = 0; c < CT; ++c) acc[t][c] = mfma16(ah[u][t], wlo[c][j], acc[t][c]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  };
+  // scaled posteriors q * sq of the group in buffer b as A fragments of the
+  // moment contraction (padding rows have sq / sum = 0 and FAST argmax -1)
+  auto posteriors = [&](const unsigned char* b, const f32x4 (&acc)[2][CT], V8 (&aq)[CT])
+                        __attribute__((always_inline)) {
+    const float* Sf = reinterpret_cast<const float*>(b + R::kA + R::kB);
+    if constexpr (FAST) {
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const f32x4 ar = *reinterpret_cast<const f32x4*>(&Sf[2 * GR + 16 * t + 4 * lg]);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < CT; ++c) {
+            const bool hit = kw + 16 * c + lr == __float_as_int(ar[r]) && bc[c] != -INFINITY;
+            terms += (unsigned)__popcll(__ballot(hit));
+            aq[c][4 * t + r] = hit ? H(sc.sq) : H(0.f);
+          }
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const f32x4 rs = *reinterpret_cast<const f32x4*>(&Sf[GR + 16 * t + 4 * lg]);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < CT; ++c) {
+            const float v = __builtin_amdgcn_exp2f(acc[t][c][r]) * rs[r];
+            const bool keep = v >= qminS;
+            terms += (unsigned)__popcll(__ballot(keep));
+            aq[c][4 * t + r] = H(keep ? v : 0.f);
+          }
+      }
+    }
+  };
+  // moments of the group in buffer b with its posteriors aq: 4 planes x NS
+  // feature tiles, each fragment read three steps ahead of its products; with
+  // post, the posteriors of the next group (acc, buffer pb) are formed one
+  // element per step under these products (steps 2 ..), the rest after them
+  constexpr int MS = 4 * NS;       // moment steps
+  constexpr int PREF = 3;          // fragment reads in flight
+  auto moments = [&](const unsigned char* b, const V8 (&aq)[CT], bool post, const unsigned char* pb,
+                     const f32x4 (&acc)[2][CT], V8 (&aqn)[CT]) __attribute__((always_inline)) {
+    const V8* Bp = reinterpret_cast<const V8*>(b + R::kA);
+    const float* Sn = reinterpret_cast<const float*>(pb + R::kA + R::kB);
+    V8 bx[PREF];
+#pragma unroll
+    for (int s = 0; s < PREF && s < MS; ++s) bx[s] = Bp[s * 64 + lane];
+    f32x4 rs[2];
+    if (!FAST && post) {
+#pragma unroll
+      for (int t = 0; t < 2; ++t) rs[t] = *reinterpret_cast<const f32x4*>(&Sn[GR + 16 * t + 4 * lg]);
+    }
+    f32x4 ar[2];
+    if (FAST && post) {
+#pragma unroll
+      for (int t = 0; t < 2; ++t) ar[t] = *reinterpret_cast<const f32x4*>(&Sn[2 * GR + 16 * t + 4 * lg]);
+    }
+    // one posterior element (t, r, c) of the next group
+    auto element = [&](int e) __attribute__((always_inline)) {
+      const int c = e % CT, r = (e / CT) & 3, t = e / (4 * CT);
+      if constexpr (FAST) {
+        const bool hit = kw + 16 * c + lr == __float_as_int(ar[t][r]) && bc[c] != -INFINITY;
+        terms += (unsigned)__popcll(__ballot(hit));
+        aqn[c][4 * t + r] = hit ? H(sc.sq) : H(0.f);
+      } else {
+        const float v = __builtin_amdgcn_exp2f(acc[t][c][r]) * rs[t][r];
+        const bool keep = v >= qminS;
+        terms += (unsigned)__popcll(__ballot(keep));
+        aqn[c][4 * t + r] = H(keep ? v : 0.f);
+      }
+    };
+    constexpr int NE = 8 * CT;  // posterior elements per lane
+    // posterior mass: every column of q x ones is the row sum
+#pragma unroll
+    for (int c = 0; c < CT; ++c) macc0[c] = mfma16(aq[c], ones, macc0[c]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+      const int p = s / NS, ft = s - (s / NS) * NS;
+      const V8 cur = bx[s % PREF];
+      if (s + PREF < MS) bx[s % PREF] = Bp[(s + PREF) * 64 + lane];
+#pragma unroll
+      for (int c = 0; c < CT; ++c) macc[c][p >> 1][ft] = mfma16(aq[c], cur, macc[c][p >> 1][ft]);
+      if (post && s >= 2 && s - 2 < NE) element(s - 2);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (post) {
+#pragma unroll
+      for (int e = MS - 2 > 0 ? MS - 2 : 0; e < NE; ++e) element(e);
+    }
+  };
+  auto buf = [&](int i) __attribute__((always_inline)) { return smem_raw + (size_t)(i % 3) * RS; };
+  // end of iteration i: every wave is done with buffer (i - 1) % 3 and the copy
+  // of group i + 1 has landed; the buffer takes group i + 2
+  auto next = [&](int i) __attribute__((always_inline)) {
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed
+    __syncthreads();
+    if (i + 2 < n) copy_rec(i + 2);
+  };
+
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  V8 aqPrev[CT];
+  if (n > 0) {
+    f32x4 acc[2][CT];
+    if constexpr (!FAST) logits(buf(0), acc);
+    posteriors(buf(0), acc, aqPrev);
+    next(0);
+  }
+  for (int i = 1; i < n; ++i) {
+    f32x4 acc[2][CT];
+    V8 aq[CT];
+    if constexpr (!FAST) logits(buf(i), acc);
+    moments(buf(i - 1), aqPrev, true, buf(i), acc, aq);
+#pragma unroll
+    for (int c = 0; c < CT; ++c) aqPrev[c] = aq[c];
+    next(i);
+  }
+  if (n > 0) {
+    f32x4 acc[2][CT];
+    V8 aq[CT];
+    moments(buf(n - 1), aqPrev, false, buf(n - 1), acc, aq);
+  }
+
+  fisher_slice_finalize<T, NS, CT, WAVES>(macc0, macc, sc, kw, lr, lg, tid, lane, img, kb, numKB, nImg, D, K, mup,
+                                          isg, sinv, priors, flags, enc, normpart, numTerms, terms, red);
+}
+
+// ---------------------------------------------------------------------------
 // Fused single-pass encoder.  The two-pass form computes every logit twice:
 // once for the row log-sum-exp (first pass) and once more for the posteriors
 // (second pass), because a workgroup owns only a block of components and the
@@ -2018,9 +2290,40 @@ static void mfma_second_pass(const MfmaTables<H>& t, int D, int K, int flags, un
                              float* enc, double* normpart, unsigned long long* numTerms, int numKB,
                              const unsigned char* gflag, const int2* list, int M, size_t shmem, int sched = 0) {
   const int padImages = n >= 8 ? ((n + 7) / 8) * 8 : n;
-  VLHIP_LAUNCH("mfma_fisher", (mfma_fisher_kernel<float, H, NS, CT, WAVES>), dim3((unsigned)((size_t)padImages * numKB)),
-               dim3(WAVES * 64), shmem, rec, offsets, gstart, b0, n, D, K, t.Whi, t.Wlo, t.bias, t.scp, t.mup,
-               t.isg, t.sinv, priors, flags, enc, normpart, numTerms, numKB, gflag, list, M, 0.f, nullptr, sched);
+  const dim3 grid((unsigned)((size_t)padImages * numKB));
+  if constexpr (pipe_fits<NS>()) {
+    if (gflag == nullptr) {
+      // no gate-served group: the pipelined kernel (same encodings)
+      const size_t sh = 3 * pipe_buf_bytes<NS>();
+      if (flags & 4)
+        VLHIP_LAUNCH("mfma_fisher", (mfma_fisher_pipe_kernel<float, H, NS, CT, WAVES, true>), grid, dim3(WAVES * 64),
+                     sh, rec, offsets, gstart, b0, n, D, K, t.Whi, t.Wlo, t.bias, t.scp, t.mup, t.isg, t.sinv, priors,
+                     flags, enc, normpart, numTerms, numKB, sched);
+      else
+        VLHIP_LAUNCH("mfma_fisher", (mfma_fisher_pipe_kernel<float, H, NS, CT, WAVES, false>), grid, dim3(WAVES * 64),
+                     sh, rec, offsets, gstart, b0, n, D, K, t.Whi, t.Wlo, t.bias, t.scp, t.mup, t.isg, t.sinv, priors,
+                     flags, enc, normpart, numTerms, numKB, sched);
+      return;
+    }
+  }
+  VLHIP_LAUNCH("mfma_fisher", (mfma_fisher_kernel<float, H, NS, CT, WAVES>), grid, dim3(WAVES * 64), shmem, rec,
+               offsets, gstart, b0, n, D, K, t.Whi, t.Wlo, t.bias, t.scp, t.mup, t.isg, t.sinv, priors, flags, enc,
+               normpart, numTerms, numKB, gflag, list, M, 0.f, nullptr, sched);
+}
+
+// Dynamic LDS limits of the second-pass kernels of one tiling (shmem: that of
+// mfma_fisher_kernel, which serves gated launches and D > 96).
+template <typename H, int NS, int CT, int WAVES>
+static void second_pass_attrs(size_t shmem) {
+  if constexpr (pipe_fits<NS>()) {
+    const int sh = (int)(3 * pipe_buf_bytes<NS>());
+    VLHIP_CHECK(hipFuncSetAttribute((const void*)mfma_fisher_pipe_kernel<float, H, NS, CT, WAVES, true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, sh));
+    VLHIP_CHECK(hipFuncSetAttribute((const void*)mfma_fisher_pipe_kernel<float, H, NS, CT, WAVES, false>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, sh));
+  }
+  VLHIP_CHECK(hipFuncSetAttribute((const void*)mfma_fisher_kernel<float, H, NS, CT, WAVES>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
 }
 
 template <typename H, int NS, int CT, int WAVES>
@@ -2056,8 +2359,7 @@ static void run_mfma(float* enc, const float* means, const float* covs, const fl
   if (numTerms) zero_words(numTerms, 2 * numImages);
   // two group records, plus the posterior tile of gate-served groups
   const size_t shmem = mfma_shmem<H, NS, CT, WAVES>(gu != nullptr);
-  auto kern = mfma_fisher_kernel<float, H, NS, CT, WAVES>;
-  VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+  second_pass_attrs<H, NS, CT, WAVES>(shmem);
   // image-aligned groups: gstart[i] = sum_{j<i} ceil(n_j / GR)
   std::vector<long long> gs(numImages + 1);
   gs[0] = 0;
@@ -2431,9 +2733,7 @@ static void plan_enqueue_any(EncodePlan& P, int tiling, int sched, const float* 
 
 template <int NS, int CT, int WAVES>
 static void plan_set_attr() {
-  auto kern = mfma_fisher_kernel<float, _Float16, NS, CT, WAVES>;
-  VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)mfma_shmem<_Float16, NS, CT, WAVES>(false)));
+  second_pass_attrs<_Float16, NS, CT, WAVES>(mfma_shmem<_Float16, NS, CT, WAVES>(false));
 }
 template <int NS>
 static void plan_set_attrs_ns() {
