@@ -90,6 +90,22 @@ def host_description(R) -> str:
     return f"{model}, {len(os.sched_getaffinity(0))} logical CPUs visible, {simd} distance kernels"
 
 
+def workload_descriptors(N: int, dim: int, seed: int, device: str = "cuda", chunk: int = 1 << 20):
+    """The SIFT-like workload's N descriptors from the library's seeded C
+    generator (libvlsynth, include/vlsynth.h: a C caller of
+    vlsynth_sift_like(out, N, dim, seed, 0) gets the same bits), generated on
+    the host in chunks and copied to a device tensor."""
+    import torch
+    from vlfeat_amd import synth
+    out = torch.empty((N, dim), dtype=torch.float32, device=device)
+    buf = np.empty((min(chunk, max(N, 1)), dim), np.float32)
+    for s0 in range(0, N, chunk):
+        n = min(chunk, N - s0)
+        synth.c_sift_like(n, dim, seed=seed, row_offset=s0, out=buf[:n])
+        out[s0:s0 + n].copy_(torch.from_numpy(buf[:n]))
+    return out
+
+
 def fit_gmm(x_host: np.ndarray, K: int, iters: int, seed: int, cov_bound: float = 1e-6):
     """GMM for the workload: EM on the GPU from random data points."""
     import vlfeat_amd as va
@@ -176,8 +192,8 @@ def projection_leg(L, x, N: int, steps: int) -> dict:
     HBM and FP32 matrix peaks."""
     import torch
     from vlfeat_amd import synth
-    raw = synth.sift_like_torch(N, 128, seed=1)
-    P = torch.as_tensor(synth.projection(DIM, 128, 1), dtype=torch.float32, device=raw.device)
+    raw = workload_descriptors(N, 128, seed=1)
+    P = torch.as_tensor(synth.c_projection(DIM, 128, 1), dtype=torch.float32, device=raw.device)
     cen = torch.zeros(128, dtype=torch.float32, device=raw.device)
     out = torch.empty((N, DIM), dtype=torch.float32, device=raw.device)
     torch.cuda.synchronize()
@@ -228,11 +244,11 @@ def batched_config(args) -> int:
     K, D = 512, DIM
     L = va.lib()
     L.vlhip_init()
-    counts = synth.lognormal_counts(args.images, median=3000, sigma=0.5, lo=200, hi=20000, seed=4)
+    counts = synth.c_lognormal_counts(args.images, median=3000, sigma=0.5, lo=200, hi=20000, seed=4)
     off = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint64)
     N = int(off[-1])
     log(f"batched: {args.images} images, {N} descriptors (median {int(np.median(counts))}, max {counts.max()})")
-    x = synth.sift_like_torch(N, D, seed=2)
+    x = workload_descriptors(N, D, seed=2)
     torch.cuda.synchronize()
     m, c, p, ll = fit_gmm(x[:300_000].cpu().numpy(), K, iters=10, seed=0)
     dev = torch.device("cuda")
@@ -285,7 +301,7 @@ def batched_config(args) -> int:
         "metric": "batched uneven Fisher+VLAD encoding (K=512, D=80, log-normal image sizes)",
         "value": n / dt, "unit": "images/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
         "ms_per_step": dt * 1000, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp16", "data": "synthetic (SIFT-like projected to 80-d, seed 2; K=512 GMM fitted by GPU EM)",
+        "dtype": "fp16", "data": "synthetic (seeded C generator vlsynth_sift_like: SIFT-like projected to 80-d, seed 2; K=512 GMM fitted by GPU EM)",
         "config": {"workload": "4096 images, n_i ~ LogNormal(log 3000, 0.5) clamped to [200, 20000], "
                                "improved FV + hard VLAD (nearest mean), one encode plan",
                    "images": n, "descriptors": N, "K": K, "D": D},
@@ -351,8 +367,8 @@ def em_config(args) -> int:
     L = va.lib()
     L.vlhip_init()
     dev = torch.device("cuda")
-    log(f"em: generating {N} descriptors (D={D}) on the GPU")
-    x = synth.sift_like_torch(N, D, seed=1)
+    log(f"em: generating {N} descriptors (D={D})")
+    x = workload_descriptors(N, D, seed=1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     m, c, p, ll_fit = fit_gmm(x[:500_000].cpu().numpy(), K, iters=5, seed=0, cov_bound=lb)
@@ -514,7 +530,7 @@ def em_config(args) -> int:
         "value": timing["f16x3"]["descriptors_per_s"], "unit": "descriptors/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": timing["f16x3"]["ms_per_iteration"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "fp16",
-        "data": "synthetic (SIFT-like 128-d, not projected, seed 1; model: 5-iteration FP32 EM fit)",
+        "data": "synthetic (seeded C generator vlsynth_sift_like: SIFT-like 128-d, not projected, seed 1; model: 5-iteration FP32 EM fit)",
         "config": {"workload": "one EM iteration (LL + M-step moments) over 4M descriptors, K=1024, D=128",
                    "N": N, "K": K, "D": D, "variance_floor": lb},
         "modes": timing,
@@ -553,8 +569,8 @@ def dropin_config(args) -> int:
     L = va.lib()
     L.vlhip_init()
     log(f"dropin: generating {N} + {n_img * per} descriptors")
-    x = synth.sift_like_torch(N, D, seed=1).cpu().numpy()
-    xe = synth.sift_like_torch(n_img * per, D, seed=2).cpu().numpy()
+    x = synth.c_sift_like(N, D, seed=1)
+    xe = synth.c_sift_like(n_img * per, D, seed=2)
     torch.cuda.synchronize()
     rng = np.random.default_rng(7)
     m0 = x[rng.choice(N, K, replace=False)].copy()
@@ -675,7 +691,7 @@ def dropin_config(args) -> int:
         "unit": "descriptors/s (Fisher + quantize + VLAD through the host API, PCIe included)",
         "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": 1000 * (g_t["fisher_s"] + g_t["vlad_s"] + g_t["quantize_s"]),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
-        "data": "synthetic (SIFT-like projected to 80-d; EM data seed 1, images seed 2)",
+        "data": "synthetic (seeded C generator vlsynth_sift_like: SIFT-like projected to 80-d; EM data seed 1, images seed 2)",
         "config": {"workload": "vl_gmm_cluster N=1M K=256 D=80 30 iterations; improved FV + VLAD of "
                                f"{n_img} images x {per}", "N": N, "K": K, "D": D, "images": n_img},
         "gpu_library_s": g_t,
@@ -751,8 +767,8 @@ def main() -> int:
     L.vlhip_init()
     n_img = args.images
     N = n_img * PER_IMAGE
-    log(f"generating {N} descriptors (D={DIM}) on the GPU")
-    x = synth.sift_like_torch(N, DIM, seed=1)
+    log(f"generating {N} descriptors (D={DIM}) with the seeded C generator")
+    x = workload_descriptors(N, DIM, seed=1)
     torch.cuda.synchronize()
     train = x[: 500_000].cpu().numpy()
     log("fitting the K=1024 GMM (EM on the GPU, 500k descriptors)")
@@ -865,7 +881,7 @@ def main() -> int:
         # gated: the gate network runs in bf16, the exact posteriors and moments of
         # every group it does not serve in compensated fp16 (all groups of this workload)
         "dtype": {"fp32": "fp32", "fp32mfma": "fp32", "bf16x3": "bf16", "f16x3": "fp16", "gated": "fp16"}[args.mode],
-        "data": "synthetic (SIFT-like rectified/clipped Gaussian projected to 80-d, seed 1; GMM fitted by GPU EM)",
+        "data": "synthetic (seeded C generator vlsynth_sift_like: SIFT-like rectified/clipped Gaussian projected to 80-d, seed 1; GMM fitted by GPU EM)",
         "config": {"workload": "improved Fisher vectors, 4096 images x 3000 descriptors, K=1024, D=80",
                    "K": K_COMP, "D": DIM, "images": n_img, "descriptors_per_image": PER_IMAGE,
                    "mode": args.mode, **({"gate": "80-256-256-1024 GELU perceptron (bf16), top-16 candidates, "

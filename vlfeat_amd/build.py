@@ -24,6 +24,8 @@ HOST_SRC = PKG / "csrc" / "host"
 HIP_SRC = PKG / "csrc" / "hip"
 BUILD = PKG / "_build"
 LIB = PKG / "lib" / "libvl.so"
+SYNTH_SRC = PKG / "csrc" / "synth" / "vlsynth.c"
+SYNTH_LIB = PKG / "lib" / "libvlsynth.so"
 
 OFFLOAD_ARCH = "gfx950"
 
@@ -79,7 +81,18 @@ def build(verbose: bool = False) -> Path:
     _run([hipcc, f"--offload-arch={OFFLOAD_ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs,
           "-Wl,-Bsymbolic", "-lgomp", "-lpthread", "-lm"])
     os.replace(tmp, LIB)
+    build_synth()
     return LIB
+
+
+def build_synth() -> Path:
+    """The seeded data generator of the benchmark workloads (host C, OpenMP):
+    ``libvlsynth.so`` beside ``libvl.so``."""
+    LIB.parent.mkdir(exist_ok=True)
+    tmp = SYNTH_LIB.with_suffix(".so.tmp")
+    _run(["gcc", *CC_FLAGS, "-fopenmp", f"-I{INCLUDE}", "-shared", str(SYNTH_SRC), "-o", str(tmp), "-lm"])
+    os.replace(tmp, SYNTH_LIB)
+    return SYNTH_LIB
 
 
 if __name__ == "__main__":

@@ -9,12 +9,85 @@
 * ``lognormal_counts`` -- per-image descriptor counts, round(exp(ln m + s z))
                    clamped to [lo, hi].
 
-numpy versions serve tests and small runs; ``sift_like_torch`` generates the
-same distribution on the GPU for the large benchmark inputs.
+The benchmark inputs come from the C generator (``libvlsynth.so``, header
+``include/vlsynth.h``): ``c_sift_like``, ``c_mixture``, ``c_projection``,
+``c_lognormal_counts``.  Every value depends only on (seed, row), from
+per-row SplitMix64 streams, the polar method and a logarithm built from basic
+IEEE operations, so a C caller of the same functions gets the same bits, on
+any thread count and for any chunking of the rows.  The numpy versions (same
+distributions, other draws) serve the tests; ``sift_like_torch`` generates
+the distribution on the GPU.
 """
 from __future__ import annotations
 
+import ctypes as _C
+from pathlib import Path as _Path
+
 import numpy as np
+
+_SYNTH = None
+
+
+def _synth():
+    """The C generator library, built by vlfeat_amd.build.build_synth."""
+    global _SYNTH
+    if _SYNTH is None:
+        path = _Path(__file__).resolve().parent / "lib" / "libvlsynth.so"
+        if not path.exists():
+            raise RuntimeError(f"{path} is not built (vlfeat_amd.build.build_synth())")
+        lib = _C.CDLL(str(path))
+        lib.vlsynth_sift_like.argtypes = [_C.c_void_p, _C.c_size_t, _C.c_int, _C.c_uint64, _C.c_uint64]
+        lib.vlsynth_sift_like.restype = None
+        lib.vlsynth_projection.argtypes = [_C.c_void_p, _C.c_int, _C.c_int, _C.c_uint64]
+        lib.vlsynth_projection.restype = None
+        lib.vlsynth_mixture.argtypes = [_C.c_void_p, _C.c_size_t, _C.c_int, _C.c_int, _C.c_uint64, _C.c_uint64,
+                                        _C.c_void_p, _C.c_void_p, _C.c_void_p]
+        lib.vlsynth_mixture.restype = None
+        lib.vlsynth_lognormal_counts.argtypes = [_C.c_void_p, _C.c_size_t, _C.c_double, _C.c_double, _C.c_int64,
+                                                 _C.c_int64, _C.c_uint64]
+        lib.vlsynth_lognormal_counts.restype = None
+        for name in ("vlsynth_log", "vlsynth_exp"):
+            getattr(lib, name).argtypes = [_C.c_double]
+            getattr(lib, name).restype = _C.c_double
+        lib.vlsynth_uniform.argtypes = [_C.c_uint64, _C.c_uint64, _C.c_uint64, _C.c_uint64]
+        lib.vlsynth_uniform.restype = _C.c_double
+        _SYNTH = lib
+    return _SYNTH
+
+
+def c_sift_like(n: int, dim: int = 80, seed: int = 1, row_offset: int = 0, out: np.ndarray | None = None) -> np.ndarray:
+    """Rows [row_offset, row_offset + n) of the SIFT-like workload (C generator), float32 n x dim."""
+    if out is None:
+        out = np.empty((n, dim), np.float32)
+    assert out.dtype == np.float32 and out.shape == (n, dim) and out.flags.c_contiguous
+    _synth().vlsynth_sift_like(out.ctypes.data, n, dim, seed, row_offset)
+    return out
+
+
+def c_projection(dim_out: int = 80, dim_in: int = 128, seed: int = 1) -> np.ndarray:
+    """The SIFT-like workload's projection (orthonormal rows), float64 dim_out x dim_in."""
+    P = np.empty((dim_out, dim_in), np.float64)
+    _synth().vlsynth_projection(P.ctypes.data, dim_out, dim_in, seed)
+    return P
+
+
+def c_mixture(n: int, dim: int = 64, k_true: int = 64, seed: int = 0, row_offset: int = 0):
+    """(x float32 n x dim, means, sigmas, weights) of the mixture workload (C generator)."""
+    x = np.empty((n, dim), np.float32)
+    mu = np.empty((k_true, dim), np.float64)
+    sg = np.empty((k_true, dim), np.float64)
+    w = np.empty(k_true, np.float64)
+    _synth().vlsynth_mixture(x.ctypes.data, n, dim, k_true, seed, row_offset, mu.ctypes.data, sg.ctypes.data,
+                             w.ctypes.data)
+    return x, mu, sg, w
+
+
+def c_lognormal_counts(num_images: int, median: float = 3000, sigma: float = 0.5, lo: int = 200, hi: int = 20000,
+                       seed: int = 4) -> np.ndarray:
+    """Per-image descriptor counts of the uneven-image workload (C generator)."""
+    c = np.empty(num_images, np.int64)
+    _synth().vlsynth_lognormal_counts(c.ctypes.data, num_images, float(median), float(sigma), lo, hi, seed)
+    return c
 
 
 def mixture(n, dim=64, k_true=64, seed=0, dtype=np.float32):
