@@ -3,7 +3,9 @@ optionally renormalised to unit L2 norm, as a device kernel and as the load
 stage of an encode plan.
 
 Checks: against the closed form in float64 (per-row relative L2 <= 2e-6: FP32
-products and sums over at most 200 terms), unit norms after renormalisation,
+products and sums over at most 200 terms; shapes for both kernels — 16-byte
+aligned data with 32 <= Din <= 128 a multiple of 16 streamed from global
+memory, the rest through LDS chunks — with ragged row counts), unit norms after renormalisation,
 the zero row (norm floor 1e-12), a PCA-whitening projection learned as
 trainEncoder.m:150-166 learns it whitening its own training data (identity
 covariance), and a plan with the load stage giving bit for bit the encodings of
@@ -43,7 +45,8 @@ def closed_form(x, P, c, renorm):
     return y
 
 
-@pytest.mark.parametrize("N,Din,D", [(1000, 128, 80), (777, 37, 13), (130, 200, 128), (5, 8, 1), (64, 32, 16)])
+@pytest.mark.parametrize("N,Din,D", [(1000, 128, 80), (777, 37, 13), (130, 200, 128), (5, 8, 1), (64, 32, 16),
+                                     (31, 128, 80), (4097, 64, 128), (3001, 48, 7), (20000, 128, 80), (33, 16, 5)])
 @pytest.mark.parametrize("renorm", [0, 1])
 def test_project_matches_closed_form(N, Din, D, renorm):
     rng = np.random.default_rng(N + Din + D)
