@@ -1333,17 +1333,24 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_kernel(
 // products, so that the vector work of one group overlaps the matrix work of
 // its neighbour instead of alternating with it.  Group records are copied
 // straight into LDS (global_load_lds, 16 bytes per lane, 1 KiB per wave-wide
-// copy; the last copy of a record repeats its last word into the buffer's
-// padding) into three buffers: group i - 1 (moments), group i (logits) and the
-// copy of group i + 1 in flight.  One barrier per group.
+// copy) by parts: iteration i reads the A part and statistics of group i
+// (logits, posteriors) and the B part of group i - 1 (moments) while the A part
+// of group i + 1 and the B part of group i are copied, so two slots of each
+// part suffice (the statistics' copy repeats the record's last word into its
+// slot's padding).  One barrier per group.
 template <int NS>
-constexpr size_t pipe_buf_bytes() {
-  return (GroupRec<NS>::kBytes + 1023) / 1024 * 1024;
+constexpr size_t pipe_a_bytes() {  // A part + statistics, whole KiB
+  return GroupRec<NS>::kA + 1024;
 }
-// three buffers fit the 160 KiB of LDS up to D = 96
+template <int NS>
+constexpr size_t pipe_lds_bytes() {  // two A slots, two B slots
+  return 2 * (pipe_a_bytes<NS>() + GroupRec<NS>::kB);
+}
+// D > 112 keeps the double-buffered kernel: with 32 components per wave the
+// pipelined form needs more than 512 registers there
 template <int NS>
 constexpr bool pipe_fits() {
-  return 3 * pipe_buf_bytes<NS>() <= 160 * 1024;
+  return NS <= 7 && pipe_lds_bytes<NS>() <= 160 * 1024;
 }
 
 template <typename T, typename H, int NS, int CT, int WAVES, bool FAST>
@@ -1358,11 +1365,13 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_pipe_kernel(
   typedef GroupRec<NS> R;
   constexpr int BC = WAVES * 16 * CT;
   constexpr int NV = (int)(R::kBytes / 16);   // 16-byte words per record
-  constexpr int PIECES = (NV + 63) / 64;      // 1 KiB copies per record
-  constexpr int PW = (PIECES + WAVES - 1) / WAVES;
-  constexpr size_t RS = pipe_buf_bytes<NS>();  // LDS buffer stride
-  static_assert(RS == (size_t)PIECES * 1024, "buffers hold whole copies");
-  extern __shared__ __align__(16) unsigned char smem_raw[];  // 3 buffers
+  static_assert(R::kA == (size_t)4 * NS * 1024 && R::kB == (size_t)4 * NS * 1024 && R::kStats <= 1024,
+                "record parts in whole KiB, statistics in one");
+  constexpr int PA = 4 * NS + 1;              // 1 KiB copies of an A slot (the last: statistics)
+  constexpr int PB = 4 * NS;                  // 1 KiB copies of a B slot
+  constexpr int PWA = (PA + WAVES - 1) / WAVES, PWB = (PB + WAVES - 1) / WAVES;
+  constexpr size_t AS = pipe_a_bytes<NS>();
+  extern __shared__ __align__(16) unsigned char smem_raw[];  // A slots 0, 1, then B slots 0, 1
   __shared__ double red[WAVES];
 
   int li, kb;
@@ -1379,22 +1388,34 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_pipe_kernel(
   const int n = (int)(gstart[img + 1] - gstart[img]);
   const MfmaScales sc = *scp;
 
-  // copy of group gB + i into buffer i % 3 (every wave takes part)
-  auto copy_rec = [&](int i) __attribute__((always_inline)) {
+  auto abuf = [&](int i) __attribute__((always_inline)) { return smem_raw + (size_t)(i & 1) * AS; };
+  auto bbuf = [&](int i) __attribute__((always_inline)) {
+    return smem_raw + 2 * AS + (size_t)(i & 1) * R::kB;
+  };
+  // 1 KiB copy q of group gB + i's record into LDS at dst (every wave takes part)
+  auto copy_piece = [&](int i, int q, unsigned char* dst) __attribute__((always_inline)) {
     const unsigned char* src = rec + (size_t)(gB + i) * R::kBytes;
-    unsigned char* dst = smem_raw + (size_t)(i % 3) * RS;
+    const int word = 64 * q + lane < NV ? 64 * q + lane : NV - 1;
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + 16 * (size_t)word),
+                                     (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+  };
+  // A part and statistics of group i into A slot i % 2
+  auto copy_a = [&](int i) __attribute__((always_inline)) {
 #pragma unroll
-    for (int k = 0; k < PW; ++k) {
-      const int q = w + WAVES * k;
-      if (q < PIECES) {
-        const int word = 64 * q + lane < NV ? 64 * q + lane : NV - 1;
-        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + 16 * (size_t)word),
-                                         (__attribute__((address_space(3))) void*)(dst + 1024 * q), 16, 0, 0);
-      }
+    for (int k = 0; k < PWA; ++k) {
+      const int u = w + WAVES * k;
+      if (u < PA) copy_piece(i, u < 4 * NS ? u : 8 * NS, abuf(i) + 1024 * u);
     }
   };
-  if (n > 0) copy_rec(0);
-  if (n > 1) copy_rec(1);
+  // B part of group i into B slot i % 2
+  auto copy_b = [&](int i) __attribute__((always_inline)) {
+#pragma unroll
+    for (int k = 0; k < PWB; ++k) {
+      const int u = w + WAVES * k;
+      if (u < PB) copy_piece(i, 4 * NS + u, bbuf(i) + 1024 * u);
+    }
+  };
+  if (n > 0) copy_a(0);
 
   V8 whi[CT][NS], wlo[CT][NS];
   float bc[CT];
@@ -1426,10 +1447,10 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_pipe_kernel(
   // compared after the exact power-of-two scaling by sq
   const float qminS = __uint_as_float(0x358637beu) * sc.sq;
 
-  // logits - m of the group in buffer b (the accumulators start at bias - m)
+  // logits - m of the group in A slot b (the accumulators start at bias - m)
   auto logits = [&](const unsigned char* b, f32x4 (&acc)[2][CT]) __attribute__((always_inline)) {
     const V8* Ap = reinterpret_cast<const V8*>(b);
-    const float* Sf = reinterpret_cast<const float*>(b + R::kA + R::kB);
+    const float* Sf = reinterpret_cast<const float*>(b + R::kA);
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       const f32x4 nm = *reinterpret_cast<const f32x4*>(&Sf[16 * t + 4 * lg]);
@@ -1468,11 +1489,11 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_pipe_kernel(
       __builtin_amdgcn_sched_barrier(0);
     }
   };
-  // scaled posteriors q * sq of the group in buffer b as A fragments of the
+  // scaled posteriors q * sq of the group in A slot b as A fragments of the
   // moment contraction (padding rows have sq / sum = 0 and FAST argmax -1)
   auto posteriors = [&](const unsigned char* b, const f32x4 (&acc)[2][CT], V8 (&aq)[CT])
                         __attribute__((always_inline)) {
-    const float* Sf = reinterpret_cast<const float*>(b + R::kA + R::kB);
+    const float* Sf = reinterpret_cast<const float*>(b + R::kA);
     if constexpr (FAST) {
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
@@ -1502,16 +1523,16 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_pipe_kernel(
       }
     }
   };
-  // moments of the group in buffer b with its posteriors aq: 4 planes x NS
+  // moments of the group in B slot b with its posteriors aq: 4 planes x NS
   // feature tiles, each fragment read three steps ahead of its products; with
-  // post, the posteriors of the next group (acc, buffer pb) are formed one
+  // post, the posteriors of the next group (acc, A slot pa) are formed one
   // element per step under these products (steps 2 ..), the rest after them
   constexpr int MS = 4 * NS;       // moment steps
   constexpr int PREF = 3;          // fragment reads in flight
-  auto moments = [&](const unsigned char* b, const V8 (&aq)[CT], bool post, const unsigned char* pb,
+  auto moments = [&](const unsigned char* b, const V8 (&aq)[CT], bool post, const unsigned char* pa,
                      const f32x4 (&acc)[2][CT], V8 (&aqn)[CT]) __attribute__((always_inline)) {
-    const V8* Bp = reinterpret_cast<const V8*>(b + R::kA);
-    const float* Sn = reinterpret_cast<const float*>(pb + R::kA + R::kB);
+    const V8* Bp = reinterpret_cast<const V8*>(b);
+    const float* Sn = reinterpret_cast<const float*>(pa + R::kA);
     V8 bx[PREF];
 #pragma unroll
     for (int s = 0; s < PREF && s < MS; ++s) bx[s] = Bp[s * 64 + lane];
@@ -1559,29 +1580,32 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_pipe_kernel(
       for (int e = MS - 2 > 0 ? MS - 2 : 0; e < NE; ++e) element(e);
     }
   };
-  auto buf = [&](int i) __attribute__((always_inline)) { return smem_raw + (size_t)(i % 3) * RS; };
-  // end of iteration i: every wave is done with buffer (i - 1) % 3 and the copy
-  // of group i + 1 has landed; the buffer takes group i + 2
+  // end of iteration i (logits and posteriors of group i, moments of group
+  // i - 1): every wave is done with A slot i % 2 and B slot (i - 1) % 2, and the
+  // copies of A(i + 1) and B(i) have landed; the slots take A(i + 2) and B(i + 1)
   auto next = [&](int i) __attribute__((always_inline)) {
     __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed
     __syncthreads();
-    if (i + 2 < n) copy_rec(i + 2);
+    if (i + 2 < n) copy_a(i + 2);
+    if (i + 1 < n) copy_b(i + 1);
   };
 
+  if (n > 1) copy_a(1);
+  if (n > 0) copy_b(0);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   V8 aqPrev[CT];
   if (n > 0) {
     f32x4 acc[2][CT];
-    if constexpr (!FAST) logits(buf(0), acc);
-    posteriors(buf(0), acc, aqPrev);
+    if constexpr (!FAST) logits(abuf(0), acc);
+    posteriors(abuf(0), acc, aqPrev);
     next(0);
   }
   for (int i = 1; i < n; ++i) {
     f32x4 acc[2][CT];
     V8 aq[CT];
-    if constexpr (!FAST) logits(buf(i), acc);
-    moments(buf(i - 1), aqPrev, true, buf(i), acc, aq);
+    if constexpr (!FAST) logits(abuf(i), acc);
+    moments(bbuf(i - 1), aqPrev, true, abuf(i), acc, aq);
 #pragma unroll
     for (int c = 0; c < CT; ++c) aqPrev[c] = aq[c];
     next(i);
@@ -1589,7 +1613,7 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_pipe_kernel(
   if (n > 0) {
     f32x4 acc[2][CT];
     V8 aq[CT];
-    moments(buf(n - 1), aqPrev, false, buf(n - 1), acc, aq);
+    moments(bbuf(n - 1), aqPrev, false, abuf(n - 1), acc, aq);
   }
 
   fisher_slice_finalize<T, NS, CT, WAVES>(macc0, macc, sc, kw, lr, lg, tid, lane, img, kb, numKB, nImg, D, K, mup,
@@ -2294,7 +2318,7 @@ static void mfma_second_pass(const MfmaTables<H>& t, int D, int K, int flags, un
   if constexpr (pipe_fits<NS>()) {
     if (gflag == nullptr) {
       // no gate-served group: the pipelined kernel (same encodings)
-      const size_t sh = 3 * pipe_buf_bytes<NS>();
+      const size_t sh = pipe_lds_bytes<NS>();
       if (flags & 4)
         VLHIP_LAUNCH("mfma_fisher", (mfma_fisher_pipe_kernel<float, H, NS, CT, WAVES, true>), grid, dim3(WAVES * 64),
                      sh, rec, offsets, gstart, b0, n, D, K, t.Whi, t.Wlo, t.bias, t.scp, t.mup, t.isg, t.sinv, priors,
@@ -2312,11 +2336,11 @@ static void mfma_second_pass(const MfmaTables<H>& t, int D, int K, int flags, un
 }
 
 // Dynamic LDS limits of the second-pass kernels of one tiling (shmem: that of
-// mfma_fisher_kernel, which serves gated launches and D > 96).
+// mfma_fisher_kernel, which serves gated launches).
 template <typename H, int NS, int CT, int WAVES>
 static void second_pass_attrs(size_t shmem) {
   if constexpr (pipe_fits<NS>()) {
-    const int sh = (int)(3 * pipe_buf_bytes<NS>());
+    const int sh = (int)pipe_lds_bytes<NS>();
     VLHIP_CHECK(hipFuncSetAttribute((const void*)mfma_fisher_pipe_kernel<float, H, NS, CT, WAVES, true>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, sh));
     VLHIP_CHECK(hipFuncSetAttribute((const void*)mfma_fisher_pipe_kernel<float, H, NS, CT, WAVES, false>,
