@@ -1004,16 +1004,16 @@ __device__ __forceinline__ void rec_store(const u32x4 (&pre)[PER], unsigned char
 
 // Fisher-vector slice of one (image, component block) from the moment
 // accumulators in registers (fisher.c:419-485 in FP64): lane l holds features
-// 16ft + (l & 15) of components kw + 16c + 4(l >> 4) + r.  Writes u and v,
-// the slice's squared norm into normpart[img * numKB + kb] and the
-// workgroup's kept terms (per-wave lane-mask counts) into numTerms[img].
-template <typename T, int NS, int CT, int WAVES>
+// 16(ft0 + ft) + (l & 15), ft < NF, of components kw + 16c + 4(l >> 4) + r.
+// Writes u and v, the slice's squared norm into normpart[img * numKB + kb] and
+// the workgroup's kept terms (per-wave lane-mask counts) into numTerms[img].
+template <typename T, int NS, int CT, int WAVES, int NF = NS>
 __device__ __forceinline__ void fisher_slice_finalize(
-    const f32x4 (&macc0)[CT], const f32x4 (&macc)[CT][2][NS], const MfmaScales& sc, int kw, int lr, int lg, int tid,
+    const f32x4 (&macc0)[CT], const f32x4 (&macc)[CT][2][NF], const MfmaScales& sc, int kw, int lr, int lg, int tid,
     int lane, int img, int kb, int numKB, long long nImg, int D, int K, const float* __restrict__ mup,
     const float* __restrict__ isg, const float* __restrict__ sinv, const T* __restrict__ priors, int flags,
     T* __restrict__ enc, double* __restrict__ normpart, unsigned long long* __restrict__ numTerms, unsigned terms,
-    double* red) {
+    double* red, int ft0 = 0) {
   constexpr int Dp = NS * 16;
   const double invq = 1.0 / (double)sc.sq;
   const double inv1 = invq / (double)sc.s1, inv2 = invq / (double)sc.s2;
@@ -1035,8 +1035,8 @@ __device__ __forceinline__ void fisher_slice_finalize(
         vp = 1.0 / ((double)nImg * sqrt(2.0 * prior));
       }
 #pragma unroll
-      for (int ft = 0; ft < NS; ++ft) {
-        const int d = 16 * ft + lr;
+      for (int ft = 0; ft < NF; ++ft) {
+        const int d = 16 * (ft0 + ft) + lr;  // >= D past the last tile
         if (k >= K || d >= D) continue;
         T u = T(0), v = T(0);
         if (use) {
@@ -1618,6 +1618,296 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_pipe_kernel(
 
   fisher_slice_finalize<T, NS, CT, WAVES>(macc0, macc, sc, kw, lr, lg, tid, lane, img, kb, numKB, nImg, D, K, mup,
                                           isg, sinv, priors, flags, enc, normpart, numTerms, terms, red);
+}
+
+// ---------------------------------------------------------------------------
+// Paired second pass: 128-component blocks on eight waves.  With 32
+// components per wave, D > 80 needs more than 256 registers per lane for the
+// split weights and the moment accumulators, so the four-wave tilings run one
+// wave per SIMD and nothing hides a wave's LDS reads and vector work.  Here
+// waves w and w + 4 form a pair that owns 32 components: wave h of the pair
+// forms the logits and posteriors of components 16h .. 16h + 15 (every
+// feature, its split weights in registers), the pair exchanges posteriors
+// through LDS (1 KiB per wave and group, two slots), and each wave
+// accumulates the moments of all 32 components over its half of the feature
+// tiles.  Per lane: weights 8 NS and moment accumulators 16 ceil(NS / 2)
+// registers -- two waves per SIMD at D = 128.  Records are copied in parts as
+// in the pipelined kernel, and iteration i issues the logits of group i, then
+// the moments of group i - 1 with this wave's posteriors of group i formed
+// one element per product.  Logits, posteriors and moment sums are those of
+// mfma_fisher_kernel operation for operation, so the unnormalised encodings
+// and the EM moments are the same bit for bit (the slice norm is summed over
+// the waves in another order).  With EM the kernel writes the EM moments of
+// a chunk as mfma_fisher_kernel<..., EM = true> does.
+template <int NS>
+constexpr size_t pair_lds_bytes() {  // A and B slots, two posterior slots of 8 KiB
+  return pipe_lds_bytes<NS>() + 2 * 8 * 1024;
+}
+
+template <typename T, typename H, int NS, bool FAST, bool EM>
+__global__ __launch_bounds__(512, 1) void mfma_fisher_pair_kernel(
+    const unsigned char* __restrict__ rec, const uint64_t* __restrict__ offsets, const long long* __restrict__ gstart,
+    int imgBegin, int numImages, int D, int K, const H* __restrict__ Whi, const H* __restrict__ Wlo,
+    const float* __restrict__ bias, const MfmaScales* __restrict__ scp, const float* __restrict__ mup,
+    const float* __restrict__ isg, const float* __restrict__ sinv, const T* __restrict__ priors, int flags,
+    T* __restrict__ enc, double* __restrict__ normpart, unsigned long long* __restrict__ numTerms, int numKB,
+    float emThr, double* __restrict__ emOut, int sched) {
+  static_assert(!(FAST && EM), "EM takes soft posteriors");
+  typedef typename VecOf<H>::type V8;
+  typedef GroupRec<NS> R;
+  constexpr int WAVES = 8, CT = 2;
+  constexpr int BC = 128;              // four pairs x 32 components
+  constexpr int NF = (NS + 1) / 2;     // feature tiles of a wave's moments (the second half may lack one)
+  constexpr int NV = (int)(R::kBytes / 16);
+  static_assert(R::kA == (size_t)4 * NS * 1024 && R::kB == (size_t)4 * NS * 1024 && R::kStats <= 1024,
+                "record parts in whole KiB, statistics in one");
+  constexpr int PA = 4 * NS + 1, PB = 4 * NS;
+  constexpr int PWA = (PA + WAVES - 1) / WAVES, PWB = (PB + WAVES - 1) / WAVES;
+  constexpr size_t AS = pipe_a_bytes<NS>();
+  extern __shared__ __align__(16) unsigned char smem_raw[];  // A slots 0, 1, B slots 0, 1, posterior slots 0, 1
+  __shared__ double red[WAVES];
+
+  int li, kb;
+  fisher_mfma_coords(numKB, numImages, sched, li, kb);
+  if (li >= numImages) return;
+  const int img = imgBegin + li;
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int lr = lane & 15, lg = lane >> 4;
+  const int h = w >> 2;                      // half of the pair
+  const int kw = kb * BC + 32 * (w & 3);     // first component of the pair
+  const int kown = kw + 16 * h;              // this wave's logit components kown + lr
+  const int ft0 = h * NF;                    // this wave's first moment feature tile
+  const long long nImg = (long long)offsets[img + 1] - (long long)offsets[img];
+  const long long gB = gstart[img] - gstart[imgBegin];
+  const int n = (int)(gstart[img + 1] - gstart[img]);
+  const MfmaScales sc = *scp;
+
+  auto abuf = [&](int i) __attribute__((always_inline)) { return smem_raw + (size_t)(i & 1) * AS; };
+  auto bbuf = [&](int i) __attribute__((always_inline)) {
+    return smem_raw + 2 * AS + (size_t)(i & 1) * R::kB;
+  };
+  auto xbuf = [&](int i) __attribute__((always_inline)) {
+    return reinterpret_cast<V8*>(smem_raw + 2 * AS + 2 * R::kB + (size_t)(i & 1) * 8192);
+  };
+  auto copy_piece = [&](int i, int q, unsigned char* dst) __attribute__((always_inline)) {
+    const unsigned char* src = rec + (size_t)(gB + i) * R::kBytes;
+    const int word = 64 * q + lane < NV ? 64 * q + lane : NV - 1;
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + 16 * (size_t)word),
+                                     (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+  };
+  auto copy_a = [&](int i) __attribute__((always_inline)) {
+#pragma unroll
+    for (int k = 0; k < PWA; ++k) {
+      const int u = w + WAVES * k;
+      if (u < PA) copy_piece(i, u < 4 * NS ? u : 8 * NS, abuf(i) + 1024 * u);
+    }
+  };
+  auto copy_b = [&](int i) __attribute__((always_inline)) {
+#pragma unroll
+    for (int k = 0; k < PWB; ++k) {
+      const int u = w + WAVES * k;
+      if (u < PB) copy_piece(i, 4 * NS + u, bbuf(i) + 1024 * u);
+    }
+  };
+  if (n > 0) copy_a(0);
+
+  V8 whi[NS], wlo[NS];
+  const float bc = bias[kown + lr];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    whi[j] = *reinterpret_cast<const V8*>(Whi + (size_t)(kown + lr) * 2 * NS * 16 + 32 * j + 8 * lg);
+    wlo[j] = *reinterpret_cast<const V8*>(Wlo + (size_t)(kown + lr) * 2 * NS * 16 + 32 * j + 8 * lg);
+  }
+  f32x4 macc[CT][2][NF];  // [comp tile][S1, S2][feature tile ft0 + f]
+#pragma unroll
+  for (int c = 0; c < CT; ++c)
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int f = 0; f < NF; ++f) macc[c][s][f] = f32x4{0.f, 0.f, 0.f, 0.f};
+  f32x4 macc0[CT];  // posterior mass S0 of both component tiles (both waves)
+#pragma unroll
+  for (int c = 0; c < CT; ++c) macc0[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  V8 ones;
+#pragma unroll
+  for (int v = 0; v < 8; ++v) ones[v] = H(1.f);
+  unsigned terms = 0;  // kept (row, component) pairs of this wave's components
+  double massd = 0.0;  // EM: unthresholded mass of component kown + lr over this lane's rows
+  float msum = 0.f;
+  const float qminS = EM ? emThr : __uint_as_float(0x358637beu) * sc.sq;  // 1.0000001e-6 * sq
+
+  // logits - m of this wave's components for the group in A slot b
+  auto logits = [&](const unsigned char* b, f32x4 (&acc)[2]) __attribute__((always_inline)) {
+    const V8* Ap = reinterpret_cast<const V8*>(b);
+    const float* Sf = reinterpret_cast<const float*>(b + R::kA);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const f32x4 nm = *reinterpret_cast<const f32x4*>(&Sf[16 * t + 4 * lg]);
+      acc[t] = f32x4{bc + nm[0], bc + nm[1], bc + nm[2], bc + nm[3]};
+    }
+    V8 ah[2][2], al[2][2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      ah[0][t] = Ap[((t * NS + 0) * 2 + 0) * 64 + lane];
+      al[0][t] = Ap[((t * NS + 0) * 2 + 1) * 64 + lane];
+    }
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      const int u = j & 1;
+      if (j + 1 < NS) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          ah[u ^ 1][t] = Ap[((t * NS + j + 1) * 2 + 0) * 64 + lane];
+          al[u ^ 1][t] = Ap[((t * NS + j + 1) * 2 + 1) * 64 + lane];
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 2; ++t) acc[t] = mfma16(ah[u][t], whi[j], acc[t]);
+#pragma unroll
+      for (int t = 0; t < 2; ++t) acc[t] = mfma16(al[u][t], whi[j], acc[t]);
+#pragma unroll
+      for (int t = 0; t < 2; ++t) acc[t] = mfma16(ah[u][t], wlo[j], acc[t]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  };
+  // statistics of this lane's rows 16t + 4lg + r of the group in A slot b
+  auto row_stats = [&](const unsigned char* b, f32x4 (&rs)[2]) __attribute__((always_inline)) {
+    const float* Sf = reinterpret_cast<const float*>(b + R::kA);
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+      rs[t] = *reinterpret_cast<const f32x4*>(&Sf[(FAST ? 2 * GR : GR) + 16 * t + 4 * lg]);
+  };
+  // posterior element e = 4t + r of this wave's component for this lane
+  auto element = [&](int e, const f32x4 (&acc)[2], const f32x4 (&rs)[2], V8& aq) __attribute__((always_inline)) {
+    const int t = e >> 2, r = e & 3;
+    if constexpr (FAST) {
+      const bool hit = kown + lr == __float_as_int(rs[t][r]) && bc != -INFINITY;
+      terms += (unsigned)__popcll(__ballot(hit));
+      aq[e] = hit ? H(sc.sq) : H(0.f);
+    } else {
+      const float v = __builtin_amdgcn_exp2f(acc[t][r]) * rs[t][r];
+      const bool keep = v >= qminS;
+      terms += (unsigned)__popcll(__ballot(keep));
+      // padding rows (sq / sum = 0) may have 2^acc = inf: keep them out of the mass
+      if (EM) msum += rs[t][r] > 0.f ? v : 0.f;
+      aq[e] = H(keep ? v : 0.f);
+    }
+  };
+  auto mass_done = [&]() __attribute__((always_inline)) {
+    if (EM) {
+      massd += (double)msum;
+      msum = 0.f;
+    }
+  };
+  // moments of the group in B slot b with the posteriors q0, q1 of the pair's
+  // two component tiles over this wave's feature tiles; with post, this wave's
+  // posteriors of the next group are formed one element per product (steps
+  // 2 ..), the rest after them
+  constexpr int MS = 4 * NF;
+  constexpr int PREF = 3;
+  auto moments = [&](const unsigned char* b, const V8& q0, const V8& q1, bool post, const f32x4 (&acc)[2],
+                     const f32x4 (&rs)[2], V8& aqn) __attribute__((always_inline)) {
+    const V8* Bp = reinterpret_cast<const V8*>(b);
+    // fragment of step s (plane s / NF, tile ft0 + s % NF; past the last tile, the last one, unused)
+    auto frag = [&](int s) __attribute__((always_inline)) {
+      const int ft = ft0 + s % NF < NS ? ft0 + s % NF : NS - 1;
+      return Bp[((s / NF) * NS + ft) * 64 + lane];
+    };
+    V8 bx[PREF];
+#pragma unroll
+    for (int s = 0; s < PREF && s < MS; ++s) bx[s] = frag(s);
+    macc0[0] = mfma16(q0, ones, macc0[0]);
+    macc0[1] = mfma16(q1, ones, macc0[1]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+      const int p = s / NF, f = s % NF;
+      const V8 cur = bx[s % PREF];
+      if (s + PREF < MS) bx[s % PREF] = frag(s + PREF);
+      if ((NS & 1) == 0 || ft0 + f < NS) {
+        macc[0][p >> 1][f] = mfma16(q0, cur, macc[0][p >> 1][f]);
+        macc[1][p >> 1][f] = mfma16(q1, cur, macc[1][p >> 1][f]);
+      }
+      if (post && s >= 2 && s - 2 < 8) element(s - 2, acc, rs, aqn);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (post) {
+#pragma unroll
+      for (int e = MS - 2 > 0 ? MS - 2 : 0; e < 8; ++e) element(e, acc, rs, aqn);
+      mass_done();
+    }
+  };
+  auto next = [&](int i) __attribute__((always_inline)) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (i + 2 < n) copy_a(i + 2);
+    if (i + 1 < n) copy_b(i + 1);
+  };
+
+  if (n > 1) copy_a(1);
+  if (n > 0) copy_b(0);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  V8 own;  // this wave's posteriors of the previous group
+  if (n > 0) {
+    f32x4 acc[2], rs[2];
+    if constexpr (!FAST) logits(abuf(0), acc);
+    row_stats(abuf(0), rs);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) element(e, acc, rs, own);
+    mass_done();
+    xbuf(0)[w * 64 + lane] = own;
+    next(0);
+  }
+  for (int i = 1; i < n; ++i) {
+    const V8 other = xbuf(i - 1)[(w ^ 4) * 64 + lane];
+    f32x4 acc[2], rs[2];
+    V8 aq;
+    if constexpr (!FAST) logits(abuf(i), acc);
+    row_stats(abuf(i), rs);
+    moments(bbuf(i - 1), h ? other : own, h ? own : other, true, acc, rs, aq);
+    xbuf(i)[w * 64 + lane] = aq;
+    own = aq;
+    next(i);
+  }
+  if (n > 0) {
+    const V8 other = xbuf(n - 1)[(w ^ 4) * 64 + lane];
+    f32x4 acc[2], rs[2];
+    V8 aq;
+    moments(bbuf(n - 1), h ? other : own, h ? own : other, false, acc, rs, aq);
+  }
+
+  if constexpr (EM) {
+    const double invq = 1.0 / (double)sc.sq;
+    const double inv1 = invq / (double)sc.s1, inv2 = invq / (double)sc.s2;
+    const size_t KD = (size_t)K * D;
+    double* P = emOut + (size_t)img * (2 * (size_t)K + 2 * KD);
+    double m = massd;
+    m += __shfl_xor(m, 16, 64);
+    m += __shfl_xor(m, 32, 64);
+    if (lg == 0 && kown + lr < K) P[K + kown + lr] = m * invq;
+#pragma unroll
+    for (int c = 0; c < CT; ++c)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = kw + 16 * c + 4 * lg + r;
+        if (k >= K) continue;
+        if (lr == 0 && c == h) P[k] = (double)macc0[c][r] * invq;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          const int d = 16 * (ft0 + f) + lr;
+          if (d >= D) continue;
+          P[2 * (size_t)K + (size_t)k * D + d] = (double)macc[c][0][f][r] * inv1;
+          P[2 * (size_t)K + KD + (size_t)k * D + d] = (double)macc[c][1][f][r] * inv2;
+        }
+      }
+    return;
+  } else {
+    fisher_slice_finalize<T, NS, CT, WAVES, NF>(macc0, macc, sc, kw, lr, lg, tid, lane, img, kb, numKB, nImg, D, K,
+                                                mup, isg, sinv, priors, flags, enc, normpart, numTerms, terms, red,
+                                                ft0);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2315,6 +2605,21 @@ static void mfma_second_pass(const MfmaTables<H>& t, int D, int K, int flags, un
                              const unsigned char* gflag, const int2* list, int M, size_t shmem, int sched = 0) {
   const int padImages = n >= 8 ? ((n + 7) / 8) * 8 : n;
   const dim3 grid((unsigned)((size_t)padImages * numKB));
+  if constexpr (CT == 2 && WAVES == 4) {
+    if (gflag == nullptr) {
+      // 128-component blocks without the gate: the paired kernel on eight waves
+      const size_t sh = pair_lds_bytes<NS>();
+      if (flags & 4)
+        VLHIP_LAUNCH("mfma_fisher", (mfma_fisher_pair_kernel<float, H, NS, true, false>), grid, dim3(512), sh, rec,
+                     offsets, gstart, b0, n, D, K, t.Whi, t.Wlo, t.bias, t.scp, t.mup, t.isg, t.sinv, priors, flags,
+                     enc, normpart, numTerms, numKB, 0.f, (double*)nullptr, sched);
+      else
+        VLHIP_LAUNCH("mfma_fisher", (mfma_fisher_pair_kernel<float, H, NS, false, false>), grid, dim3(512), sh, rec,
+                     offsets, gstart, b0, n, D, K, t.Whi, t.Wlo, t.bias, t.scp, t.mup, t.isg, t.sinv, priors, flags,
+                     enc, normpart, numTerms, numKB, 0.f, (double*)nullptr, sched);
+      return;
+    }
+  }
   if constexpr (pipe_fits<NS>()) {
     if (gflag == nullptr) {
       // no gate-served group: the pipelined kernel (same encodings)
@@ -2339,6 +2644,13 @@ static void mfma_second_pass(const MfmaTables<H>& t, int D, int K, int flags, un
 // mfma_fisher_kernel, which serves gated launches).
 template <typename H, int NS, int CT, int WAVES>
 static void second_pass_attrs(size_t shmem) {
+  if constexpr (CT == 2 && WAVES == 4) {
+    const int sh = (int)pair_lds_bytes<NS>();
+    VLHIP_CHECK(hipFuncSetAttribute((const void*)mfma_fisher_pair_kernel<float, H, NS, true, false>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, sh));
+    VLHIP_CHECK(hipFuncSetAttribute((const void*)mfma_fisher_pair_kernel<float, H, NS, false, false>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, sh));
+  }
   if constexpr (pipe_fits<NS>()) {
     const int sh = (int)pipe_lds_bytes<NS>();
     VLHIP_CHECK(hipFuncSetAttribute((const void*)mfma_fisher_pipe_kernel<float, H, NS, CT, WAVES, true>,
@@ -3414,9 +3726,15 @@ static double run_em_step(double* out_host, const float* means, const float* cov
   double* part = out_host ? (double*)scratch(SLOT_PARTIAL, numChunks * PS * sizeof(double)) : nullptr;
   float qthr = (float)(0.01 / (double)K);  // smallest float q with !((double)q < 0.01 / K)
   if ((double)qthr < 0.01 / (double)K) qthr = std::nextafter(qthr, INFINITY);
-  const size_t shmem = mfma_shmem<H, NS, CT, WAVES>(false);
-  auto kern = mfma_fisher_kernel<float, H, NS, CT, WAVES, true>;
-  VLHIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+  // 128-component blocks: the paired kernel on eight waves
+  constexpr bool kPair = CT == 2 && WAVES == 4;
+  const size_t shmem = kPair ? pair_lds_bytes<NS>() : mfma_shmem<H, NS, CT, WAVES>(false);
+  if constexpr (kPair)
+    VLHIP_CHECK(hipFuncSetAttribute((const void*)mfma_fisher_pair_kernel<float, H, NS, false, true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+  else
+    VLHIP_CHECK(hipFuncSetAttribute((const void*)mfma_fisher_kernel<float, H, NS, CT, WAVES, true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
   // the scaled threshold needs the device scale sq: read it once
   MfmaScales hs;
   download(&hs, t.scp, sizeof(MfmaScales));
@@ -3443,10 +3761,18 @@ static double run_em_step(double* out_host, const float* means, const float* cov
     VLHIP_LAUNCH("mfma_em_ll", mfma_em_ll_kernel<NS>, dim3(kLLBlocks), dim3(256), 0, rec, (long long)nG, t.scp, llp);
     if (out_host) {
       const int padImages = n >= 8 ? ((n + 7) / 8) * 8 : n;
-      VLHIP_LAUNCH("mfma_em_moments", (mfma_fisher_kernel<float, H, NS, CT, WAVES, true>), dim3((unsigned)((size_t)padImages * numKB)), dim3(WAVES * 64), shmem, rec,
-                   offsets, gstart, (int)b0, n, (int)D, (int)K, t.Whi, t.Wlo, t.bias, t.scp, t.mup, t.isg, t.sinv,
-                   priors, 0, (float*)nullptr, (double*)nullptr, (unsigned long long*)nullptr, numKB,
-                   (const unsigned char*)nullptr, (const int2*)nullptr, 0, emThr, part, 0);
+      const dim3 grid((unsigned)((size_t)padImages * numKB));
+      if constexpr (kPair)
+        VLHIP_LAUNCH("mfma_em_moments", (mfma_fisher_pair_kernel<float, H, NS, false, true>), grid, dim3(512),
+                     shmem, rec, offsets, gstart, (int)b0, n, (int)D, (int)K, t.Whi, t.Wlo, t.bias, t.scp, t.mup,
+                     t.isg, t.sinv, priors, 0, (float*)nullptr, (double*)nullptr, (unsigned long long*)nullptr,
+                     numKB, emThr, part, 0);
+      else
+        VLHIP_LAUNCH("mfma_em_moments", (mfma_fisher_kernel<float, H, NS, CT, WAVES, true>), grid,
+                     dim3(WAVES * 64), shmem, rec, offsets, gstart, (int)b0, n, (int)D, (int)K, t.Whi, t.Wlo, t.bias,
+                     t.scp, t.mup, t.isg, t.sinv, priors, 0, (float*)nullptr, (double*)nullptr,
+                     (unsigned long long*)nullptr, numKB, (const unsigned char*)nullptr, (const int2*)nullptr, 0,
+                     emThr, part, 0);
     }
     const size_t o = llh.size();
     llh.resize(o + kLLBlocks);
