@@ -613,11 +613,20 @@ def dropin_config(args) -> int:
         out["kmeans_elkan_s"] = time.perf_counter() - t0
         kce = lib.kmeans_centers(km, np.float32)
         lib.vl_kmeans_delete(km)
+        # one dense one-hot matrix, rewritten in place per image (3,000 entries
+        # cleared, 3,000 set), so that the timing is that of the encoder calls
+        # rather than of allocating and zeroing 3 MB per image
+        oh = np.zeros((per, K), np.float32)
+        rows = np.arange(per)
+        prev = None
         t0 = time.perf_counter()
         vl = []
         for i in range(n_img):
-            oh = np.zeros((per, K), np.float32)
-            oh[np.arange(per), a[i * per:(i + 1) * per]] = 1
+            ai = a[i * per:(i + 1) * per]
+            if prev is not None:
+                oh[rows, prev] = 0
+            oh[rows, ai] = 1
+            prev = ai
             vl.append(lib.vlad(xe[i * per:(i + 1) * per], m, oh, 0))
         out["vlad_s"] = time.perf_counter() - t0
         return out, em + (fv, a, np.stack(vl), kc, ke, kce, kee)
