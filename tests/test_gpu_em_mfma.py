@@ -44,6 +44,7 @@ CASES = {
     "mix64_k64": (lambda: synth.mixture(30000, 64, 64, seed=0), 64),
     "mix24_k37": (lambda: synth.mixture(9000, 24, 10, seed=3), 37),
     "mix48_k160": (lambda: synth.mixture(20000, 48, 32, seed=4), 160),
+    "mix112_k90": (lambda: synth.mixture(9000, 112, 12, seed=8), 90),
 }
 
 

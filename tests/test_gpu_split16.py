@@ -36,6 +36,9 @@ CASES = {
     "mix24_k37": (lambda: synth.mixture(1500, 24, 10, seed=3), 37),
     "sift96_k160": (lambda: synth.sift_like(2000, 96, seed=5), 160),
     "sift128_k200": (lambda: synth.sift_like(2000, 128, seed=6, project=False), 200),
+    # odd numbers of 16-feature tiles on the paired second pass (K <= 128 or D > 80)
+    "mix112_k90": (lambda: synth.mixture(1500, 112, 12, seed=8), 90),
+    "mix16_k20": (lambda: synth.mixture(1500, 16, 6, seed=9), 20),
 }
 
 
