@@ -360,26 +360,29 @@ __global__ __launch_bounds__(256) void fisher_split_combine_kernel(
 
 // fisher.c:474-485: enc /= max(||enc||_2, 1e-12).  Workgroup (image = x,
 // slice of FN_SLICE entries = y): each workgroup of an image sums the same norm partials in
-// the same order, so they all divide by the same value.
+// the same order, so they all divide by the same value.  The slice's loads are
+// issued before the norm is formed, so that they overlap its latency.
 constexpr int FN_SLICE = 8192;
 template <typename T>
 __global__ __launch_bounds__(256) void fisher_normalize_kernel(T* __restrict__ enc, const double* __restrict__ normpart,
                                                               int numKB, size_t len) {
   __shared__ T nsh;
   const int img = blockIdx.x;
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int b = 0; b < numKB; ++b) s += normpart[(size_t)img * numKB + b];
-    const T n = (T)sqrt(s);
-    nsh = (T)(((double)n > 1e-12) ? (double)n : 1e-12);
-  }
-  __syncthreads();
-  const T n = nsh;
   T* e = enc + (size_t)img * len;
   const size_t s0 = (size_t)blockIdx.y * FN_SLICE;
   const size_t end = min(len, s0 + FN_SLICE);
   constexpr int V = 16 / sizeof(T);
   typedef T vec __attribute__((ext_vector_type(V)));
+  auto norm_of_image = [&]() {
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int b = 0; b < numKB; ++b) s += normpart[(size_t)img * numKB + b];
+      const T n = (T)sqrt(s);
+      nsh = (T)(((double)n > 1e-12) ? (double)n : 1e-12);
+    }
+    __syncthreads();
+    return nsh;
+  };
   if (len % V == 0 && ((uintptr_t)e & 15) == 0) {
     // 16-byte pieces, every load of the thread issued before the first store
     constexpr int U = FN_SLICE / V / 256;
@@ -390,6 +393,7 @@ __global__ __launch_bounds__(256) void fisher_normalize_kernel(T* __restrict__ e
       const size_t i = s0 / V + threadIdx.x + 256 * u;
       if (i * V < end) v[u] = ev[i];
     }
+    const T n = norm_of_image();
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const size_t i = s0 / V + threadIdx.x + 256 * u;
@@ -400,6 +404,7 @@ __global__ __launch_bounds__(256) void fisher_normalize_kernel(T* __restrict__ e
       }
     }
   } else {
+    const T n = norm_of_image();
     for (size_t i = s0 + threadIdx.x; i < end; i += blockDim.x) e[i] = e[i] / n;
   }
 }
