@@ -101,22 +101,26 @@ __device__ __forceinline__ void homkermap_one(T x, const double* __restrict__ ta
   for (int j = 0; j < F; ++j) o[j] = (T)((T)sign * ((h[j] - l[j]) * scale + l[j]));
 }
 
-// 16 bytes of input (VPT values) from base, mapped; outputs written at out + base F.
-template <typename T, int ORDER, bool POW2, int VPT, int W>
-__device__ __forceinline__ void homkermap_chunk(const T* __restrict__ in, long long n, long long base,
-                                                const double* table, int numSub, double sub, int minExp, int maxExp,
-                                                T* __restrict__ out) {
-  constexpr int F = 2 * ORDER + 1;
+// 16 bytes of input (VPT values) at base (zero past n).
+template <typename T, int VPT>
+__device__ __forceinline__ void homkermap_load(const T* __restrict__ in, long long n, long long base, T (&v)[VPT]) {
   typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-  const bool whole = base + VPT <= n;
-  T v[VPT];
-  if (whole) {
+  if (base + VPT <= n) {
     const u32x4 w = *reinterpret_cast<const u32x4*>(in + base);
     __builtin_memcpy(v, &w, 16);
   } else {
 #pragma unroll
     for (int j = 0; j < VPT; ++j) v[j] = base + j < n ? in[base + j] : T(0);
   }
+}
+
+// The VPT values v loaded from base, mapped; outputs written at out + base F.
+template <typename T, int ORDER, bool POW2, int VPT, int W>
+__device__ __forceinline__ void homkermap_map_store(const T (&v)[VPT], long long n, long long base,
+                                                    const double* table, int numSub, double sub, int minExp,
+                                                    int maxExp, T* __restrict__ out) {
+  constexpr int F = 2 * ORDER + 1;
+  typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
   T o[VPT * F];
 #pragma unroll
   for (int j = 0; j < VPT; ++j) {
@@ -126,7 +130,7 @@ __device__ __forceinline__ void homkermap_chunk(const T* __restrict__ in, long l
     for (int f = 0; f < F; ++f) o[j * F + f] = m[f];
   }
   T* dst = out + base * F;
-  if (whole) {
+  if (base + VPT <= n) {
 #pragma unroll
     for (int q = 0; q < W; ++q) {
       u32x4 w;
@@ -143,9 +147,14 @@ __device__ __forceinline__ void homkermap_chunk(const T* __restrict__ in, long l
 // Persistent form: each workgroup copies the whole table (rows x F doubles,
 // at most 84 KB for order 4) into LDS once, then maps 16 bytes of input per
 // thread (4 floats or 2 doubles) in a grid-stride loop, reading the table rows
-// from LDS; the VPT x F outputs of a thread are contiguous and written as
-// 16-byte stores (VPT F sizeof(T) = 16 (2 ORDER + 1) bytes); a ragged end goes
-// value by value.  in and out 16-byte aligned.
+// from LDS.  The VPT x F outputs of a thread are contiguous (W = 2 ORDER + 1
+// 16-byte words) and so are a wave's: they pass through LDS so that store q of
+// lane l writes word l + 64 q of the wave's run, each store one contiguous
+// KiB; a wave that reaches the ragged end writes value by value.  Each pass of
+// the loop issues the loads of HU chunks (one grid stride apart) before
+// mapping the first, so that more bytes are in flight per wave.  in and out
+// 16-byte aligned.
+constexpr int HU = 4;
 template <typename T, int ORDER, bool POW2>
 __global__ __launch_bounds__(256) void homkermap_vec_kernel(const T* __restrict__ in, long long n,
                                                             const double* __restrict__ table, long long tableLen,
@@ -155,11 +164,49 @@ __global__ __launch_bounds__(256) void homkermap_vec_kernel(const T* __restrict_
   constexpr int F = 2 * ORDER + 1;
   constexpr int VPT = 16 / (int)sizeof(T);
   constexpr int W = VPT * F * (int)sizeof(T) / 16;  // 16-byte output words per thread
+  typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+  __shared__ u32x4 stage[4][64 * W];  // one run of output words per wave
   for (long long i = threadIdx.x; i < tableLen; i += 256) tsh[i] = table[i];
   __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long long chunks = (n + VPT - 1) / VPT;
-  for (long long c = blockIdx.x * 256LL + threadIdx.x; c < chunks; c += (long long)gridDim.x * 256) {
-    homkermap_chunk<T, ORDER, POW2, VPT, W>(in, n, c * VPT, tsh, numSub, sub, minExp, maxExp, out);
+  const long long G = (long long)gridDim.x * 256;
+  for (long long c0 = blockIdx.x * 256LL + threadIdx.x; c0 < chunks; c0 += G * HU) {
+    T v[HU][VPT];
+#pragma unroll
+    for (int u = 0; u < HU; ++u)
+      if (c0 + G * u < chunks) homkermap_load<T, VPT>(in, n, (c0 + G * u) * VPT, v[u]);
+#pragma unroll
+    for (int u = 0; u < HU; ++u) {
+      const long long c = c0 + G * u;
+      const long long cw = c - lane;  // the wave's first chunk
+      if ((cw + 64) * VPT <= n) {
+        // the whole wave maps full chunks: its outputs are one run of 64 W words
+        T o[VPT * F];
+#pragma unroll
+        for (int j = 0; j < VPT; ++j) {
+          T m[F];
+          homkermap_one<T, ORDER, POW2>(v[u][j], tsh, numSub, sub, minExp, maxExp, m);
+#pragma unroll
+          for (int f = 0; f < F; ++f) o[j * F + f] = m[f];
+        }
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+          u32x4 w;
+          __builtin_memcpy(&w, &o[q * (16 / (int)sizeof(T))], 16);
+          stage[wv][lane * W + q] = w;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        u32x4* dst = reinterpret_cast<u32x4*>(out + cw * VPT * F);
+#pragma unroll
+        for (int q = 0; q < W; ++q) dst[lane + 64 * q] = stage[wv][lane + 64 * q];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      } else if (c < chunks) {
+        homkermap_map_store<T, ORDER, POW2, VPT, W>(v[u], n, c * VPT, tsh, numSub, sub, minExp, maxExp, out);
+      }
+    }
   }
 }
 
