@@ -224,13 +224,15 @@ __global__ __launch_bounds__(512, 4) void project_stream_kernel(const float* __r
       f32x4 xv[2];
 #pragma unroll
       for (int rt = 0; rt < 2; ++rt) xv[rt] = xa[rt] - cm;
+      // P as the A operand: the accumulators hold out^T tiles, so that a lane
+      // ends with four consecutive columns of one row (a 16-byte store)
 #pragma unroll
       for (int s4 = 0; s4 < 4; ++s4)
 #pragma unroll
         for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
           for (int t = 0; t < NT; ++t)
-            acc[rt][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv[rt][s4], pm[t][s4], acc[rt][t], 0, 0, 0);
+            acc[rt][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(pm[t][s4], xv[rt][s4], acc[rt][t], 0, 0, 0);
 #pragma unroll
       for (int rt = 0; rt < 2; ++rt) {
         xa[rt] = xb[rt];
@@ -242,27 +244,35 @@ __global__ __launch_bounds__(512, 4) void project_stream_kernel(const float* __r
     oc[0] = on[0];
     oc[1] = on[1];
     const long long row0 = blk * 32;
+    // lane (lr, lk) holds row 16 rt + lr, columns 16 t + 4 lk .. + 3
+    const bool vecOut = (D & 3) == 0 && (reinterpret_cast<uintptr_t>(out) & 15) == 0;
 #pragma unroll
     for (int rt = 0; rt < 2; ++rt) {
-      float scale[4] = {1.f, 1.f, 1.f, 1.f};
+      const long long row = row0 + 16 * rt + lr;
+      float scale = 1.f;
       if (renorm) {
+        float sq = 0.f;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          float sq = 0.f;
+        for (int t = 0; t < NT; ++t)
 #pragma unroll
-          for (int t = 0; t < NT; ++t) sq += acc[rt][t][r] * acc[rt][t][r];  // padding columns hold 0
-#pragma unroll
-          for (int o = 1; o < 16; o <<= 1) sq += __shfl_xor(sq, o, 64);
-          scale[r] = 1.f / fmaxf(1e-12f, sqrtf(sq));
-        }
+          for (int r = 0; r < 4; ++r) sq += acc[rt][t][r] * acc[rt][t][r];  // padding columns hold 0
+        sq += __shfl_xor(sq, 16, 64);
+        sq += __shfl_xor(sq, 32, 64);
+        scale = 1.f / fmaxf(1e-12f, sqrtf(sq));
       }
+      if (row < N) {
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const int col = 16 * t + lr;
+        for (int t = 0; t < NT; ++t) {
+          const int col = 16 * t + 4 * lk;
+          if (col >= D) continue;
+          const f32x4 v = acc[rt][t] * scale;
+          if (vecOut) {
+            *reinterpret_cast<f32x4*>(out + row * D + col) = v;
+          } else {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const long long row = row0 + 16 * rt + 4 * lk + r;
-          if (row < N && col < D) out[row * D + col] = acc[rt][t][r] * scale[r];
+            for (int r = 0; r < 4; ++r)
+              if (col + r < D) out[row * D + col + r] = v[r];
+          }
         }
       }
     }
