@@ -2523,7 +2523,7 @@ __global__ __launch_bounds__(FF_THREADS) void mfma_fused_finalize_kernel(
 constexpr size_t kRecordBudget = size_t(8) << 30;
 // Rows of a batch on which the learned gate is tried before it runs on the rest
 // (2048 groups of 32 descriptors decide whether it serves at least 1% of them).
-constexpr size_t kGateProbeRows = 65536;
+constexpr size_t kGateProbeRows = 16384;
 
 // Device tables of one encode, carved from one buffer:
 // [scales][g Dp][bias Kp][mup, isg, sinv Kp*Dp][Whi, Wlo Kp*2Dp][Wst Kp*2(2Dp+8)].
