@@ -2750,10 +2750,16 @@ static void run_mfma(float* enc, const float* means, const float* covs, const fl
         if (probe < rows) {
           VLHIP_CHECK(hipMemsetAsync(conf + probe, 0, rows - probe, ctx().stream));
           sparse(nullptr);
-          std::vector<unsigned char> hf(nG);
-          download(hf.data(), gflag, nG);
+          // only the groups of the images that start inside the probe are read back
+          size_t iEnd = b0, gEnd = 0;
+          while (iEnd < b1 && off[iEnd] - rowBase < probe) {
+            gEnd += (size_t)(gs[iEnd + 1] - gs[iEnd]);
+            ++iEnd;
+          }
+          std::vector<unsigned char> hf(gEnd ? gEnd : 1);
+          if (gEnd) download(hf.data(), gflag, gEnd);
           size_t inProbe = 0, servedProbe = 0, gi = 0;
-          for (size_t i = b0; i < b1; ++i) {
+          for (size_t i = b0; i < iEnd; ++i) {
             const uint64_t a = off[i] - rowBase, e = off[i + 1] - rowBase;
             for (uint64_t r = a; r < e; r += GR, ++gi) {
               if (std::min<uint64_t>(e, r + GR) > probe) continue;
