@@ -1457,6 +1457,9 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_pipe_kernel(
 #pragma unroll
       for (int c = 0; c < CT; ++c) acc[t][c] = f32x4{bc[c] + nm[0], bc[c] + nm[1], bc[c] + nm[2], bc[c] + nm[3]};
     }
+    // the logit products run at raised wave priority: the other wave of the
+    // SIMD then fills the matrix pipe's gaps with its vector and LDS work
+    __builtin_amdgcn_s_setprio(1);
     // the A fragments of step j + 1 are read while step j's products run
     V8 ah[2][2], al[2][2];
 #pragma unroll
@@ -1488,6 +1491,7 @@ __global__ __launch_bounds__(WAVES * 64, 1) void mfma_fisher_pipe_kernel(
         for (int c = 0; c < CT; ++c) acc[t][c] = mfma16(ah[u][t], wlo[c][j], acc[t][c]);
       __builtin_amdgcn_sched_barrier(0);
     }
+    __builtin_amdgcn_s_setprio(0);
   };
   // scaled posteriors q * sq of the group in A slot b as A fragments of the
   // moment contraction (padding rows have sq / sum = 0 and FAST argmax -1)
@@ -1746,6 +1750,7 @@ __global__ __launch_bounds__(512, 1) void mfma_fisher_pair_kernel(
       const f32x4 nm = *reinterpret_cast<const f32x4*>(&Sf[16 * t + 4 * lg]);
       acc[t] = f32x4{bc + nm[0], bc + nm[1], bc + nm[2], bc + nm[3]};
     }
+    __builtin_amdgcn_s_setprio(1);  // logit products at raised wave priority, as above
     V8 ah[2][2], al[2][2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
@@ -1770,6 +1775,7 @@ __global__ __launch_bounds__(512, 1) void mfma_fisher_pair_kernel(
       for (int t = 0; t < 2; ++t) acc[t] = mfma16(ah[u][t], wlo[j], acc[t]);
       __builtin_amdgcn_sched_barrier(0);
     }
+    __builtin_amdgcn_s_setprio(0);
   };
   // statistics of this lane's rows 16t + 4lg + r of the group in A slot b
   auto row_stats = [&](const unsigned char* b, f32x4 (&rs)[2]) __attribute__((always_inline)) {
