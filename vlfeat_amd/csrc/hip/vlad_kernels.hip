@@ -707,18 +707,20 @@ static void launch_vlad_hard_cb(T* enc, const T* means, int D, int K, const T* x
 }
 
 // Blocks of 32 clusters when the images alone give the device >= 1024
-// workgroups; fewer clusters per block (down to 4) for small batches, since
+// workgroups; fewer clusters per block (down to one) for small batches, since
 // every workgroup walks all of its image's cluster indices.
 template <typename T>
 static void launch_vlad_hard(T* enc, const T* means, int D, int K, const T* x, const uint64_t* offsets, int numImages,
                              const uint32_t* hard, int flags) {
   int cb = 32;
-  while (cb > 4 && (size_t)numImages * (size_t)((K + cb - 1) / cb) < 1024) cb >>= 1;
+  while (cb > 1 && (size_t)numImages * (size_t)((K + cb - 1) / cb) < 1024) cb >>= 1;
   switch (cb) {
     case 32: launch_vlad_hard_cb<T, 32>(enc, means, D, K, x, offsets, numImages, hard, flags); break;
     case 16: launch_vlad_hard_cb<T, 16>(enc, means, D, K, x, offsets, numImages, hard, flags); break;
     case 8: launch_vlad_hard_cb<T, 8>(enc, means, D, K, x, offsets, numImages, hard, flags); break;
-    default: launch_vlad_hard_cb<T, 4>(enc, means, D, K, x, offsets, numImages, hard, flags); break;
+    case 4: launch_vlad_hard_cb<T, 4>(enc, means, D, K, x, offsets, numImages, hard, flags); break;
+    case 2: launch_vlad_hard_cb<T, 2>(enc, means, D, K, x, offsets, numImages, hard, flags); break;
+    default: launch_vlad_hard_cb<T, 1>(enc, means, D, K, x, offsets, numImages, hard, flags); break;
   }
 }
 
