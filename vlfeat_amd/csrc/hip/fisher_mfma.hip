@@ -503,7 +503,13 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
   constexpr int WS = 2 * Dp + 8;                 // LDS row stride (halves) of the staged weights
   constexpr int LG = LseGroups<NS>::value;
   constexpr int NT = LG * 64;
-  __shared__ __align__(16) H wsh[2][2][MB_COMPS][WS];  // [buffer][hi, lo][comp][feature]
+  // The two weight buffers ([hi, lo][comp][feature]) are separate LDS
+  // objects, used by every step in a parity fixed at compile time, so that
+  // the compiler sees that a step's fragment reads do not alias the copy of
+  // the next block into the other buffer and does not wait for that copy
+  // before the step's products.
+  __shared__ __align__(16) H wsh0[2][MB_COMPS][WS];
+  __shared__ __align__(16) H wsh1[2][MB_COMPS][WS];
   // Biases of the staged blocks (block B in slot B & 3), four copies each,
   // loaded with the block's copy and stored at the end of the step before it,
   // so that no global load is waited for inside the sweep.
@@ -546,12 +552,13 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
 
   // Stage the wave's 32 centred descriptors x' = x - g (zero past D and past
   // the image) in LDS with 16-byte loads; the weight buffers are free until
-  // the sweep starts and hold at least LG waves x 32 rows x (Dp + 4) floats.
-  // Both operand layouts are then read from LDS.
+  // the sweep starts, and each holds the rows of half of the waves (32 rows x
+  // (Dp + 4) floats per wave).  Both operand layouts are then read from LDS.
   constexpr int XS = Dp + 4;
-  static_assert(LG * GR * XS * sizeof(float) <= sizeof(wsh), "staging must fit the weight buffers");
+  constexpr int HALF = (LG + 1) / 2;  // waves staging in each weight buffer
+  static_assert(HALF * GR * XS * sizeof(float) <= sizeof(wsh0), "staging must fit the weight buffers");
   const bool vecx = (D & 3) == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0;
-  float* xsh = reinterpret_cast<float*>(&wsh[0][0][0][0]) + w * GR * XS;
+  float* xsh = reinterpret_cast<float*>(w < HALF ? &wsh0[0][0][0] : &wsh1[0][0][0]) + (w % HALF) * GR * XS;
   {
     // every load of the lane issued before the first is used
     constexpr int XW = GR * (Dp / 4);  // 16-byte words of the wave's rows
@@ -654,11 +661,12 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
   const bool need = live && !(gflag != nullptr && gflag[grp] != 0);
   if (!__syncthreads_or(need)) return;
   const int nb = Kp / MB_COMPS;
-  auto buf = [&](int b) __attribute__((always_inline)) { return &wsh[b & 1][0][0][0]; };
-  // copy of block b into its buffer (every wave takes part, needed or not)
-  auto copy_block = [&](int b) __attribute__((always_inline)) {
+  H* const buf0 = &wsh0[0][0][0];  // even blocks
+  H* const buf1 = &wsh1[0][0][0];  // odd blocks
+  // copy of block b into its buffer d (every wave takes part, needed or not)
+  auto copy_block = [&](int b, H* d) __attribute__((always_inline)) {
     const char* src = reinterpret_cast<const char*>(Wst) + (size_t)b * BLOCK_BYTES;
-    unsigned char* dst = reinterpret_cast<unsigned char*>(buf(b));
+    unsigned char* dst = reinterpret_cast<unsigned char*>(d);
 #pragma unroll
     for (int k = 0; k < PW; ++k)
       if (w + LG * k < PIECES)
@@ -672,10 +680,10 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
   };
 #pragma unroll
   for (int k = 0; k < PW; ++k)
-    if (w + LG * k < PIECES) reinterpret_cast<u32x4*>(buf(0))[64 * (w + LG * k) + lane] = w0[k];
+    if (w + LG * k < PIECES) reinterpret_cast<u32x4*>(buf0)[64 * (w + LG * k) + lane] = w0[k];
   store_bias(0);
   if (nb > 1) {
-    copy_block(1);
+    copy_block(1, buf1);
     if (tid < MB_COMPS) bnext = bias[MB_COMPS + tid];
   }
   __syncthreads();
@@ -704,9 +712,11 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
     // the update of block b - 1 with the products of block b: row maxima and
     // the reference check with step 0, the branch (rare) after it, then the
     // terms of the 8 rows spread over steps 1 .. NS - 1
-    auto step = [&](f32x4 (&accNew)[4][2], f32x4 (&accOld)[4][2], int b) __attribute__((always_inline)) {
+    auto step = [&](f32x4 (&accNew)[4][2], f32x4 (&accOld)[4][2], int b, auto parity)
+                    __attribute__((always_inline)) {
+      constexpr bool odd = decltype(parity)::value != 0;  // b's parity
       if (b + 1 < nb) {
-        copy_block(b + 1);
+        copy_block(b + 1, odd ? buf0 : buf1);
         if (tid < MB_COMPS) bnext = bias[(b + 1) * MB_COMPS + tid];
       }
       if (need) {
@@ -714,7 +724,7 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
         init_of(b, bq);
         bool up = false;
         lse_products<H, NS>(
-            accNew, ahi, alo, buf(b), bq, lr, lg,
+            accNew, ahi, alo, odd ? buf1 : buf0, bq, lr, lg,
             [&](int j) __attribute__((always_inline)) {
               if (j == 0) {
                 up = lse_lazy_check(accOld, st);
@@ -737,16 +747,16 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
     if (need) {
       f32x4 bq[4];
       init_of(0, bq);
-      lse_products<H, NS>(accA, ahi, alo, buf(0), bq, lr, lg, no_work, no_work);
+      lse_products<H, NS>(accA, ahi, alo, buf0, bq, lr, lg, no_work, no_work);
     }
     finish_step(0);
     int b = 1;
     for (; b + 1 < nb; b += 2) {
-      step(accB, accA, b);
-      step(accA, accB, b + 1);
+      step(accB, accA, b, std::integral_constant<int, 1>{});
+      step(accA, accB, b + 1, std::integral_constant<int, 0>{});
     }
     if (b < nb) {
-      step(accB, accA, b);
+      step(accB, accA, b, std::integral_constant<int, 1>{});
       if (need) lse_lazy_update(accB, st);
     } else {
       if (need) lse_lazy_update(accA, st);
@@ -776,16 +786,18 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
     };
     // the update of block b - 1 (the online log-sum-exp and first maximum),
     // its 8 rows spread over the NS steps of block b's products
-    auto step = [&](f32x4 (&accNew)[4][2], f32x4 (&accOld)[4][2], int b) __attribute__((always_inline)) {
+    auto step = [&](f32x4 (&accNew)[4][2], f32x4 (&accOld)[4][2], int b, auto parity)
+                    __attribute__((always_inline)) {
+      constexpr bool odd = decltype(parity)::value != 0;  // b's parity
       if (b + 1 < nb) {
-        copy_block(b + 1);
+        copy_block(b + 1, odd ? buf0 : buf1);
         if (tid < MB_COMPS) bnext = bias[(b + 1) * MB_COMPS + tid];
       }
       if (need) {
         float bc[4];
         bias_of(b - 1, bc);
         lse_products<H, NS>(
-            accNew, ahi, alo, buf(b), zero4, lr, lg,
+            accNew, ahi, alo, odd ? buf1 : buf0, zero4, lr, lg,
             [&](int j) __attribute__((always_inline)) {
 #pragma unroll
               for (int q = (8 * j) / NS; q < (8 * (j + 1)) / NS; ++q)
@@ -795,17 +807,17 @@ __global__ __launch_bounds__(LseGroups<NS>::value * 64, 1) void mfma_lse_kernel(
       }
       finish_step(b);
     };
-    if (need) lse_products<H, NS>(accA, ahi, alo, buf(0), zero4, lr, lg, no_work, no_work);
+    if (need) lse_products<H, NS>(accA, ahi, alo, buf0, zero4, lr, lg, no_work, no_work);
     finish_step(0);
     int b = 1;
     for (; b + 1 < nb; b += 2) {
-      step(accB, accA, b);
-      step(accA, accB, b + 1);
+      step(accB, accA, b, std::integral_constant<int, 1>{});
+      step(accA, accB, b + 1, std::integral_constant<int, 0>{});
     }
     float bc[4];
     bias_of(nb - 1, bc);
     if (b < nb) {
-      step(accB, accA, b);
+      step(accB, accA, b, std::integral_constant<int, 1>{});
       if (need) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) lse_row<true>(accB, bc, q >> 2, q & 3, (nb - 1) * MB_COMPS, lr, st);
