@@ -246,7 +246,7 @@ __global__ __launch_bounds__(256) void estep_kernel(
 // ---------------------------------------------------------------------------
 // M-step moments, gmm.c:1093-1149.  Workgroup (kb, split) owns MS_COMPS
 // components and the rows of one split; thread t owns component t/8 and the
-// dimensions d = (t%8) + 8j.  Posteriors below the threshold (0.01/K,
+// dimension pairs d = 2(t%8) + 16j, 2(t%8) + 16j + 1 (one 8-byte LDS read each).  Posteriors below the threshold (0.01/K,
 // gmm.c:1099) contribute nothing, exactly as the reference's `continue`.
 // Partial sums per split are written in T and reduced in FP64 afterwards.
 template <typename T, int DPT>
@@ -256,8 +256,9 @@ __global__ __launch_bounds__(256) void moments_kernel(
     T* __restrict__ S0p, T* __restrict__ S1p, T* __restrict__ S2p, T* __restrict__ massp) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* qs = reinterpret_cast<T*>(smem_raw);                    // [MS_ROWS][MS_COMPS + 1]
-  T* xs = qs + MS_ROWS * (MS_COMPS + 1);                      // [MS_ROWS][D + 1]
-  T* red = xs + (size_t)MS_ROWS * (D + 1);                    // [2][256]
+  const int XR = D + (D & 1) + 2;                             // row stride, even
+  T* xs = qs + MS_ROWS * (MS_COMPS + 1);                      // [MS_ROWS][XR]
+  T* red = xs + (size_t)MS_ROWS * XR;                         // [2][256]
 
   const int tid = threadIdx.x;
   const int c = tid >> 3, g = tid & 7;             // compute ownership
@@ -268,10 +269,13 @@ __global__ __launch_bounds__(256) void moments_kernel(
   const long long rEnd = min(N, rBegin + rowsPerSplit);
   const int kc = k0 + c;
 
+  static_assert(DPT % 2 == 0, "dimensions are owned in pairs");
+  // element j of the thread: dimension 2g + 16 (j / 2) + (j % 2)
+  auto dim_of = [&](int j) __attribute__((always_inline)) { return 2 * g + 16 * (j >> 1) + (j & 1); };
   T mu[DPT], s1[DPT], s2[DPT];
 #pragma unroll
   for (int j = 0; j < DPT; ++j) {
-    const int d = g + 8 * j;
+    const int d = dim_of(j);
     mu[j] = (kc < K && d < D) ? oldMeans[(size_t)kc * D + d] : T(0);
     s1[j] = T(0);
     s2[j] = T(0);
@@ -308,22 +312,31 @@ __global__ __launch_bounds__(256) void moments_kernel(
         const int idx = base + tid + 256 * u;
         if (idx < nr * D) {
           const int r = idx / D;
-          xs[r * (D + 1) + (idx - r * D)] = v[u];
+          xs[r * XR + (idx - r * D)] = v[u];
         }
       }
     }
     __syncthreads();
+    typedef T pair2 __attribute__((ext_vector_type(2)));
     for (int r = 0; r < nr; ++r) {
       const T p = qs[r * (MS_COMPS + 1) + c];
       if (p != T(0)) {
 #pragma unroll
-        for (int j = 0; j < DPT; ++j) {
-          const int d = g + 8 * j;
-          if (d < D) {
-            const T xv = xs[r * (D + 1) + d];
-            const T diff = xv - mu[j];
-            s1[j] = fma(p, xv, s1[j]);
-            s2[j] = fma(p * diff, diff, s2[j]);
+        for (int jj = 0; jj < DPT / 2; ++jj) {
+          const int d0 = 2 * g + 16 * jj;
+          if (d0 < D) {
+            // xs rows hold at least D + 1 values, so the pair's second (padding) read stays in the row
+            const pair2 xp = *reinterpret_cast<const pair2*>(&xs[r * XR + d0]);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              const int j = 2 * jj + h;
+              if (d0 + h < D) {
+                const T xv = xp[h];
+                const T diff = xv - mu[j];
+                s1[j] = fma(p, xv, s1[j]);
+                s2[j] = fma(p * diff, diff, s2[j]);
+              }
+            }
           }
         }
       }
@@ -333,7 +346,7 @@ __global__ __launch_bounds__(256) void moments_kernel(
   if (kc < K) {
 #pragma unroll
     for (int j = 0; j < DPT; ++j) {
-      const int d = g + 8 * j;
+      const int d = dim_of(j);
       if (d < D) {
         const size_t o = ((size_t)split * K + kc) * D + d;
         S1p[o] = s1[j];
@@ -492,7 +505,7 @@ static void launch_moments(const T* q, const T* x, const T* oldMeans, size_t N, 
   T* massp = S0p + (size_t)G * K;
   T* S1p = massp + (size_t)G * K;
   T* S2p = S1p + (size_t)G * KD;
-  const size_t shmem = (size_t)(MS_ROWS * (MS_COMPS + 1) + MS_ROWS * (D + 1) + 512) * sizeof(T);
+  const size_t shmem = (size_t)(MS_ROWS * (MS_COMPS + 1) + MS_ROWS * (D + (D & 1) + 2) + 512) * sizeof(T);
   if (N > 0) {
     VLHIP_LAUNCH("gmm_moments", (moments_kernel<T, DPT>), dim3(numKB, (unsigned)G), dim3(256), shmem,
                  q, x, oldMeans, (long long)N, (int)D, (int)K, rowsPerSplit, thr, S0p, S1p, S2p, massp);
